@@ -1,0 +1,162 @@
+/*
+ * nutcracker_core_abi.h -- the C-ABI of the MI355X variational-MPS core.
+ *
+ * The first group re-declares, with the same names and argument meaning, the
+ * gfortran-convention entry points that the reference's C++ layer binds in
+ * sources/common/core.cpp (every scalar passed by pointer, complex128 arrays in
+ * column-major order, 1-based MPO block indices and swap indices).  Linking the
+ * reference's host code against libnutcracker_hip.so instead of its Fortran core
+ * therefore needs no source change.  Each declaration cites the reference
+ * declaration it replaces and the Fortran routine whose behaviour it follows.
+ *
+ * The second group (nc_*) has no counterpart in the reference: device
+ * information, the last error message, the projector helpers that core.f95 keeps
+ * internal, and the device-resident sweep driver.
+ *
+ * Complex numbers are passed as nc_zcomplex, which has the layout of
+ * std::complex<double> and Fortran double complex.
+ */
+#ifndef NUTCRACKER_CORE_ABI_H
+#define NUTCRACKER_CORE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nc_zcomplex {
+    double re, im;
+} nc_zcomplex;
+
+/* ---- environment contractors ------------------------------------------- */
+
+/* core.cpp:235-245; core.f95:623-677.  new_left_environment(br,br,cr) from
+   left_environment(bl,bl,cl) and state_site_tensor(br,bl,d). */
+void contract_sos_left_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, const uint32_t* cr,
+                        const uint32_t* d, const nc_zcomplex* left_environment,
+                        const uint32_t* number_of_matrices, const uint32_t* sparse_operator_indices,
+                        const nc_zcomplex* sparse_operator_matrices, const nc_zcomplex* state_site_tensor,
+                        nc_zcomplex* new_left_environment);
+
+/* core.cpp:272-282; core.f95:679-848.  new_right_environment(bl,bl,cl). */
+void contract_sos_right_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, const uint32_t* cr,
+                         const uint32_t* d, const nc_zcomplex* right_environment,
+                         const uint32_t* number_of_matrices, const uint32_t* sparse_operator_indices,
+                         const nc_zcomplex* sparse_operator_matrices, const nc_zcomplex* state_site_tensor,
+                         nc_zcomplex* new_right_environment);
+
+/* core.cpp:33-44; core.f95:1045-1088. */
+void compute_expectation_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, const uint32_t* cr,
+                          const uint32_t* d, const nc_zcomplex* left_environment,
+                          const nc_zcomplex* state_site_tensor, const uint32_t* number_of_matrices,
+                          const uint32_t* sparse_operator_indices, const nc_zcomplex* sparse_operator_matrices,
+                          const nc_zcomplex* right_environment, nc_zcomplex* expectation);
+
+/* core.cpp:179-193; core.f95:1142-1161. */
+void contract_expectation_boundaries_(const uint32_t* b, const uint32_t* c, const nc_zcomplex* left_boundary,
+                                      const nc_zcomplex* right_boundary, nc_zcomplex* expectation);
+
+/* core.cpp:72-84; core.f95:1090-1140.  optimization_matrix is (n, n), n = br*bl*d. */
+void compute_optimization_matrix_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, const uint32_t* cr,
+                                  const uint32_t* d, const nc_zcomplex* left_environment,
+                                  const uint32_t* number_of_matrices, const uint32_t* sparse_operator_indices,
+                                  const nc_zcomplex* sparse_operator_matrices, const nc_zcomplex* right_environment,
+                                  nc_zcomplex* optimization_matrix);
+
+/* ---- matvec stages (internal Fortran routines core.f95:532-621, bound by the
+   reference's f2py tests, tests/python/test.py:245-328) -------------------- */
+
+void iteration_stage_1_(const uint32_t* bl, const uint32_t* cl, const uint32_t* cr, const uint32_t* d,
+                        const nc_zcomplex* left_environment, const uint32_t* number_of_matrices,
+                        const uint32_t* sparse_operator_indices, const nc_zcomplex* sparse_operator_matrices,
+                        nc_zcomplex* iteration_stage_1_tensor);
+void iteration_stage_2_(const uint32_t* bl, const uint32_t* br, const uint32_t* cr, const uint32_t* d,
+                        const nc_zcomplex* iteration_stage_1_tensor, const nc_zcomplex* state_site_tensor,
+                        nc_zcomplex* iteration_stage_2_tensor);
+void iteration_stage_3_(const uint32_t* bl, const uint32_t* br, const uint32_t* cr, const uint32_t* d,
+                        const nc_zcomplex* iteration_stage_2_tensor, const nc_zcomplex* right_environment,
+                        nc_zcomplex* output_state_site_tensor);
+
+/* ---- site optimizer ------------------------------------------------------ */
+
+/* core.cpp:676-693; core.f95:1911-2609.  Returns info: 0 success, -14 no
+   converged eigenvalue, 5 / 10 / 11 projector-space errors, other negative
+   values for device failures.  `which` is two characters ("SR", "LR", "LM")
+   with no hidden length.  number_of_iterations: in = maximum restarts,
+   out = restarts performed (ARPACK iparam(3)). */
+uint32_t optimize_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, const uint32_t* cr, const uint32_t* d,
+                   const nc_zcomplex* left_environment, const uint32_t* number_of_matrices,
+                   const uint32_t* sparse_operator_indices, const nc_zcomplex* sparse_operator_matrices,
+                   const nc_zcomplex* right_environment, const uint32_t* number_of_projectors,
+                   const uint32_t* number_of_reflectors, const uint32_t* orthogonal_subspace_dimension,
+                   const nc_zcomplex* reflectors, const nc_zcomplex* coefficients, const uint32_t* swaps,
+                   const char* which, const double* tol, uint32_t* number_of_iterations,
+                   const nc_zcomplex* guess, nc_zcomplex* result, nc_zcomplex* eigenvalue, double* normal);
+
+/* ---- site move ------------------------------------------------------------ */
+
+/* core.cpp:580-587; core.f95:2708-2788.  Returns the SVD info (0 on success). */
+int32_t norm_denorm_going_left_(const uint32_t* bll, const uint32_t* bl, const uint32_t* br, const uint32_t* dl,
+                                const uint32_t* d, const nc_zcomplex* site_tensor_to_denormalize,
+                                const nc_zcomplex* site_tensor_to_normalize, nc_zcomplex* denormalized_site_tensor,
+                                nc_zcomplex* normalized_site_tensor);
+
+/* core.cpp:609-616; core.f95:2790-2878. */
+int32_t norm_denorm_going_right_(const uint32_t* bl, const uint32_t* br, const uint32_t* brr, const uint32_t* d,
+                                 const uint32_t* dr, const nc_zcomplex* site_tensor_to_normalize,
+                                 const nc_zcomplex* site_tensor_to_denormalize, nc_zcomplex* normalized_site_tensor,
+                                 nc_zcomplex* denormalized_site_tensor);
+
+/* ---- projectors ---------------------------------------------------------- */
+
+/* core.cpp:109-118; core.f95:1266-1339.  Writes a double into the first eight
+   bytes of *overlap, as the Fortran routine does. */
+void compute_overlap_with_projectors_(const uint32_t* number_of_projectors, const uint32_t* number_of_reflectors,
+                                      const nc_zcomplex* reflectors, const nc_zcomplex* coefficients,
+                                      const uint32_t* swaps, const uint32_t* vector_size, const nc_zcomplex* vector,
+                                      const nc_zcomplex* overlap);
+
+/* core.cpp:431-441; core.f95:1866-1909. */
+void filter_components_outside_orthog_(const uint32_t* full_space_dimension, const uint32_t* number_of_projectors,
+                                       const uint32_t* number_of_reflectors,
+                                       const uint32_t* orthogonal_subspace_dimension, const nc_zcomplex* reflectors,
+                                       const nc_zcomplex* coefficients, const uint32_t* swaps,
+                                       const nc_zcomplex* input, const nc_zcomplex* output);
+
+/* core.cpp:309-324; core.f95:850-896. */
+void contract_vs_left_(const uint32_t* b_left_old, const uint32_t* b_right_old, const uint32_t* b_left_new,
+                       const uint32_t* b_right_new, const uint32_t* d, const nc_zcomplex* left_environment,
+                       const nc_zcomplex* normalized_projector_site_tensor,
+                       const nc_zcomplex* normalized_state_site_tensor, nc_zcomplex* new_left_environment);
+
+/* core.cpp:338-348; core.f95:898-952. */
+void contract_vs_right_(const uint32_t* b_left_old, const uint32_t* b_right_old, const uint32_t* b_left_new,
+                        const uint32_t* b_right_new, const uint32_t* d, const nc_zcomplex* right_environment,
+                        const nc_zcomplex* normalized_projector_site_tensor,
+                        const nc_zcomplex* normalized_state_site_tensor, nc_zcomplex* new_right_environment);
+
+/* ---- entry points with no counterpart in the reference --------------------- */
+
+/* Message of the last failed call on this thread (the reference's C API keeps
+   a thread-local error string, sources/c-interface/errors.cpp:7-25). */
+const char* nc_last_error(void);
+
+/* Architecture name, compute-unit count, clock and memory of the current device. */
+int nc_device_info(char* name, int name_len, int* cu_count, double* clock_ghz, size_t* total_mem);
+
+/* project_into_orthogonal_space / unproject_from_orthogonal_space
+   (core.f95:1513-1664), internal to the reference's Fortran core. */
+void nc_project_into_orthogonal_space(uint32_t n, uint32_t nproj, uint32_t nrefl, uint32_t orth,
+                                      const nc_zcomplex* reflectors, const nc_zcomplex* coefficients,
+                                      const uint32_t* swaps, const nc_zcomplex* in, nc_zcomplex* out);
+void nc_unproject_from_orthogonal_space(uint32_t n, uint32_t nproj, uint32_t nrefl, uint32_t orth,
+                                        const nc_zcomplex* reflectors, const nc_zcomplex* coefficients,
+                                        const uint32_t* swaps, const nc_zcomplex* in, nc_zcomplex* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUTCRACKER_CORE_ABI_H */

@@ -1,0 +1,144 @@
+// Grouped sparse-MPO contractions on the device (FP64 complex).
+//
+// The reference builds a dense stage-1 tensor I(bl,d,cr,bl,d) and runs two
+// zgemms per matvec (core.f95:532-621).  Here the block list is contracted as
+//   A: P_{f,a} = X_a L_f            one GEMM per (distinct from f, physical a)
+//   B: Q_t(.,b) = sum_{n: to_n = t} sum_a W_n(a,b) P_{from_n,a}    (mix kernel)
+//   C: y = sum_t R_t Q_t            one K-segmented GEMM, K = br * n_to
+// which needs 8 d bl br (n_from bl + n_to br) + 8 nblk bl br d^2 flops instead
+// of 8 w d chi^3 (d+1).  The environment updates use the same three steps with
+// conj(psi) in place of R (left) or with R first and conj(psi) last (right).
+#include "nc_internal.hpp"
+
+#include <algorithm>
+
+namespace nc {
+
+void MpoSite::build(int d_, int cl_, int cr_, int nmat_, const uint32_t* idx1, const zd* mats_) {
+    if (d_ <= 0 || cl_ <= 0 || cr_ <= 0 || nmat_ <= 0) throw Error("operator site: bad dimensions", -2);
+    d = d_; cl = cl_; cr = cr_; nmat = nmat_;
+    idx.assign(idx1, idx1 + 2 * (size_t)nmat);
+    mats.assign(mats_, mats_ + (size_t)d * d * nmat);
+    froms.clear(); tos.clear();
+    for (int n = 0; n < nmat; ++n) {
+        int f = (int)idx[2 * n] - 1, t = (int)idx[2 * n + 1] - 1;
+        if (f < 0 || f >= cl || t < 0 || t >= cr) throw Error("operator site: block index out of range", -3);
+        froms.push_back(f);
+        tos.push_back(t);
+    }
+    std::sort(froms.begin(), froms.end());
+    froms.erase(std::unique(froms.begin(), froms.end()), froms.end());
+    std::sort(tos.begin(), tos.end());
+    tos.erase(std::unique(tos.begin(), tos.end()), tos.end());
+    blk_from.resize(nmat);
+    blk_to.resize(nmat);
+    for (int n = 0; n < nmat; ++n) {
+        int f = (int)idx[2 * n] - 1, t = (int)idx[2 * n + 1] - 1;
+        blk_from[n] = (int)(std::lower_bound(froms.begin(), froms.end(), f) - froms.begin());
+        blk_to[n] = (int)(std::lower_bound(tos.begin(), tos.end(), t) - tos.begin());
+    }
+    // mix tables, grouped by destination
+    for (int dir = 0; dir < 2; ++dir) {
+        const bool src_is_from = dir == 0;
+        const int ndst = (int)(src_is_from ? tos.size() : froms.size());
+        std::vector<int> ptr(ndst + 1, 0), src;
+        std::vector<zd> w;
+        for (int dst = 0; dst < ndst; ++dst) {
+            for (int n = 0; n < nmat; ++n) {
+                int bd = src_is_from ? blk_to[n] : blk_from[n];
+                if (bd != dst) continue;
+                src.push_back(src_is_from ? blk_from[n] : blk_to[n]);
+                for (int k = 0; k < d * d; ++k) w.push_back(mats[(size_t)n * d * d + k]);
+            }
+            ptr[dst + 1] = (int)src.size();
+        }
+        mix_ptr[dir].resize(ptr.size());
+        mix_src[dir].resize(src.size());
+        mix_W[dir].resize(w.size());
+        h2d(mix_ptr[dir].ptr, ptr.data(), ptr.size() * sizeof(int));
+        h2d(mix_src[dir].ptr, src.data(), src.size() * sizeof(int));
+        h2d(mix_W[dir].ptr, w.data(), w.size() * sizeof(zd));
+    }
+}
+
+// Step A of the matvec / left environment: P_{f,a} = X_a L_f
+static void step_a_left(int bl, int br, MpoSite const& op, const double2* L, const double2* X,
+                        double2* P, hipStream_t s) {
+    const int64_t rs = (int64_t)br * bl;
+    GemmPlan plan;
+    plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+    for (size_t fi = 0; fi < op.froms.size(); ++fi)
+        for (int a = 0; a < op.d; ++a) {
+            plan.begin_batch(P + ((int64_t)fi * op.d + a) * rs, br);
+            plan.add_seg(X + a * rs, br, L + (int64_t)op.froms[fi] * bl * bl, bl, bl);
+        }
+    zgemm_run(plan, s);
+}
+
+void heff_matvec(int bl, int br, MpoSite const& op, const double2* L, const double2* R,
+                 const double2* x, double2* y, Workspace& ws, hipStream_t s) {
+    const int64_t rs = (int64_t)br * bl;
+    const int d = op.d;
+    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
+    step_a_left(bl, br, op, L, x, ws.P.ptr, s);
+    mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
+    GemmPlan plan;
+    plan.M = br; plan.N = bl * d; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+    plan.begin_batch(y, br);
+    for (size_t ti = 0; ti < op.tos.size(); ++ti)
+        plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, ws.Q.ptr + (int64_t)ti * d * rs, br, br);
+    zgemm_run(plan, s);
+}
+
+void env_left(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
+              double2* newL, Workspace& ws, hipStream_t s) {
+    const int64_t rs = (int64_t)br * bl;
+    const int d = op.d;
+    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
+    step_a_left(bl, br, op, L, psi, ws.P.ptr, s);
+    mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
+    if ((int)op.tos.size() < op.cr) dzero(newL, (size_t)br * br * op.cr * sizeof(double2));
+    GemmPlan plan;
+    plan.M = br; plan.N = br; plan.opA = OpN; plan.opB = OpC; plan.accumulate = false;
+    for (size_t ti = 0; ti < op.tos.size(); ++ti) {
+        plan.begin_batch(newL + (int64_t)op.tos[ti] * br * br, br);
+        plan.add_seg(ws.Q.ptr + (int64_t)ti * d * rs, br, psi, br, bl * d);
+    }
+    zgemm_run(plan, s);
+}
+
+void env_right(int bl, int br, MpoSite const& op, const double2* R, const double2* psi,
+               double2* newR, Workspace& ws, hipStream_t s) {
+    const int64_t rs = (int64_t)br * bl;
+    const int d = op.d;
+    ws.ensure(rs * d * op.tos.size(), rs * d * op.froms.size());
+    {   // P'_{t,a} = R_t Psi_a
+        GemmPlan plan;
+        plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+        for (size_t ti = 0; ti < op.tos.size(); ++ti)
+            for (int a = 0; a < d; ++a) {
+                plan.begin_batch(ws.P.ptr + ((int64_t)ti * d + a) * rs, br);
+                plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, psi + a * rs, br, br);
+            }
+        zgemm_run(plan, s);
+    }
+    mix_blocks(op, false, ws.P.ptr, ws.Q.ptr, rs, s);
+    if ((int)op.froms.size() < op.cl) dzero(newR, (size_t)bl * bl * op.cl * sizeof(double2));
+    GemmPlan plan;  // N_f = sum_b Psi_b^H Q'_{f,b}
+    plan.M = bl; plan.N = bl; plan.opA = OpC; plan.opB = OpN; plan.accumulate = false;
+    for (size_t fi = 0; fi < op.froms.size(); ++fi) {
+        plan.begin_batch(newR + (int64_t)op.froms[fi] * bl * bl, bl);
+        for (int b = 0; b < d; ++b)
+            plan.add_seg(psi + b * rs, br, ws.Q.ptr + ((int64_t)fi * d + b) * rs, br, br);
+    }
+    zgemm_run(plan, s);
+}
+
+zd expectation(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
+               const double2* R, Workspace& ws, hipStream_t s) {
+    ZBuf N((size_t)bl * bl * op.cl);
+    env_right(bl, br, op, R, psi, N.ptr, ws, s);
+    return trace_pair(N.ptr, L, bl, op.cl, s);
+}
+
+}  // namespace nc

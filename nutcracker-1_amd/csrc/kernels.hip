@@ -1,0 +1,593 @@
+// gfx950 device kernels of the variational-MPS core (FP64 complex path).
+//
+//  * zgemm_f64: batched, K-segmented complex GEMM on v_mfma_f64_16x16x4_f64.
+//    Complex products use four real MFMAs per 16x16x4 step (re*re - im*im,
+//    re*im + im*re) into separate real / imaginary accumulators.
+//  * mix: the sparse MPO block mixing Q(., b, dst) = sum W_n(a,b) P(., a, src).
+//  * vector kernels: reductions, axpy and friends for the Arnoldi solver and
+//    the Householder projectors.
+#include "nc_internal.hpp"
+
+#include <cstring>
+
+namespace nc {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// ----------------------------------------------------------------------------
+// Complex GEMM, 64 x 64 output tile per 256-thread workgroup (4 waves in 2 x 2,
+// each wave 32 x 32 = 2 x 2 MFMA tiles), K staged through LDS 16 at a time.
+// ----------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+
+__device__ __forceinline__ double2 zload(const double2* __restrict__ p) { return *p; }
+__device__ __forceinline__ double2 zconj(double2 v) { return make_double2(v.x, -v.y); }
+
+template <int OPA, int OPB, bool ACC>
+__global__ __launch_bounds__(256) void zgemm_f64_kernel(const GemmBatch* __restrict__ batches,
+                                                        const GemmSeg* __restrict__ segs, int M,
+                                                        int N) {
+    __shared__ double2 As[GBK][GBM + GPAD];
+    __shared__ double2 Bs[GBK][GBN + GPAD];
+
+    const GemmBatch bt = batches[blockIdx.z];
+    const int m0 = blockIdx.x * GBM;
+    const int n0 = blockIdx.y * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave & 1) * 32;
+    const int wn = (wave >> 1) * 32;
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+
+    f64x4 acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = f64x4{0, 0, 0, 0};
+            acc_im[i][j] = f64x4{0, 0, 0, 0};
+        }
+
+    for (int sidx = bt.seg_begin; sidx < bt.seg_end; ++sidx) {
+        const GemmSeg sg = segs[sidx];
+        const double2* __restrict__ A = sg.A;
+        const double2* __restrict__ B = sg.B;
+        const int K = sg.K;
+        for (int k0 = 0; k0 < K; k0 += GBK) {
+            // ---- A tile: op(A)(m, k), m in [m0, m0+64), k in [k0, k0+16)
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                const int id = tid + 256 * it;
+                int mm, kk;
+                if (OPA == OpN) { mm = id & 63; kk = id >> 6; }
+                else            { kk = id & 15; mm = id >> 4; }
+                const int gm = m0 + mm, gk = k0 + kk;
+                double2 v = make_double2(0.0, 0.0);
+                if (gm < M && gk < K) {
+                    if (OPA == OpN) v = zload(A + (int64_t)gm + (int64_t)gk * sg.lda);
+                    else {
+                        v = zload(A + (int64_t)gk + (int64_t)gm * sg.lda);
+                        if (OPA == OpC) v = zconj(v);
+                    }
+                }
+                As[kk][mm] = v;
+            }
+            // ---- B tile: op(B)(k, n)
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                const int id = tid + 256 * it;
+                int nn, kk;
+                if (OPB == OpN) { kk = id & 15; nn = id >> 4; }
+                else            { nn = id & 63; kk = id >> 6; }
+                const int gn = n0 + nn, gk = k0 + kk;
+                double2 v = make_double2(0.0, 0.0);
+                if (gn < N && gk < K) {
+                    if (OPB == OpN) v = zload(B + (int64_t)gk + (int64_t)gn * sg.ldb);
+                    else {
+                        v = zload(B + (int64_t)gn + (int64_t)gk * sg.ldb);
+                        if (OPB == OpC) v = zconj(v);
+                    }
+                }
+                Bs[kk][nn] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kq = 0; kq < GBK / 4; ++kq) {
+                const int k = kq * 4 + lk;
+                double2 a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a[i] = As[k][wm + i * 16 + lrow];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) b[j] = Bs[k][wn + j * 16 + lrow];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const double are = a[i].x, aim = a[i].y, naim = -a[i].y;
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, b[j].x, acc_re[i][j], 0, 0, 0);
+                        acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(naim, b[j].y, acc_re[i][j], 0, 0, 0);
+                        acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, b[j].y, acc_im[i][j], 0, 0, 0);
+                        acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, b[j].x, acc_im[i][j], 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- store: lane holds column (lane & 15), rows 4 * (lane >> 4) + v
+    double2* __restrict__ C = bt.C;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 16 + lrow;
+            if (col >= N) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + 4 * lk + v;
+                if (row < M) {
+                    double2* p = C + (int64_t)row + (int64_t)col * bt.ldc;
+                    double2 o = make_double2(acc_re[i][j][v], acc_im[i][j][v]);
+                    if (ACC) {
+                        double2 c = *p;
+                        o.x += c.x;
+                        o.y += c.y;
+                    }
+                    *p = o;
+                }
+            }
+        }
+}
+
+template <int OPA, int OPB>
+static void launch_ops(bool acc, dim3 grid, GemmBatch const* b, GemmSeg const* s, int M, int N,
+                       hipStream_t st) {
+    if (acc)
+        hipLaunchKernelGGL((zgemm_f64_kernel<OPA, OPB, true>), grid, dim3(256), 0, st, b, s, M, N);
+    else
+        hipLaunchKernelGGL((zgemm_f64_kernel<OPA, OPB, false>), grid, dim3(256), 0, st, b, s, M, N);
+}
+
+void zgemm_launch(int opA, int opB, bool accumulate, int M, int N, GemmBatch const* d_batches,
+                  GemmSeg const* d_segs, int nbatch, int /*max_k_total*/, hipStream_t s) {
+    if (M <= 0 || N <= 0 || nbatch <= 0) return;
+    dim3 grid((M + GBM - 1) / GBM, (N + GBN - 1) / GBN, nbatch);
+#define NC_OPS(a, b) \
+    if (opA == a && opB == b) { launch_ops<a, b>(accumulate, grid, d_batches, d_segs, M, N, s); NC_HIP(hipGetLastError()); return; }
+    NC_OPS(OpN, OpN) NC_OPS(OpN, OpT) NC_OPS(OpN, OpC)
+    NC_OPS(OpT, OpN) NC_OPS(OpC, OpN) NC_OPS(OpC, OpC) NC_OPS(OpT, OpT)
+#undef NC_OPS
+    throw Error("zgemm_launch: unsupported operand combination");
+}
+
+// Descriptor arena: a ring of device slots, each written by a stream-ordered
+// copy from pinned host memory, so building a plan never synchronizes.
+namespace {
+struct DescArena {
+    static constexpr size_t kSlot = 64 * 1024;
+    static constexpr int kSlots = 64;
+    char* dev = nullptr;
+    char* host = nullptr;
+    hipEvent_t done[kSlots];
+    int next = 0;
+    DescArena() {
+        NC_HIP(hipMalloc(&dev, kSlot * kSlots));
+        NC_HIP(hipHostMalloc((void**)&host, kSlot * kSlots, hipHostMallocDefault));
+        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventRecord(done[i], Context::get().stream));
+    }
+};
+DescArena& arena() {
+    static DescArena* a = new DescArena();
+    return *a;
+}
+}  // namespace
+
+void zgemm_run(GemmPlan const& plan, hipStream_t s) {
+    if (plan.batches.empty()) return;
+    size_t bbytes = plan.batches.size() * sizeof(GemmBatch);
+    size_t sbytes = plan.segs.size() * sizeof(GemmSeg);
+    size_t soff = (bbytes + 255) & ~size_t(255);
+    if (soff + sbytes > DescArena::kSlot) throw Error("zgemm_run: too many GEMM descriptors");
+    DescArena& a = arena();
+    int slot = a.next;
+    a.next = (a.next + 1) % DescArena::kSlots;
+    // the host staging of this slot may still be read by an earlier copy
+    NC_HIP(hipEventSynchronize(a.done[slot]));
+    char* h = a.host + slot * DescArena::kSlot;
+    char* d = a.dev + slot * DescArena::kSlot;
+    std::memcpy(h, plan.batches.data(), bbytes);
+    std::memcpy(h + soff, plan.segs.data(), sbytes);
+    NC_HIP(hipMemcpyAsync(d, h, soff + sbytes, hipMemcpyHostToDevice, s));
+    zgemm_launch(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, (GemmBatch const*)d,
+                 (GemmSeg const*)(d + soff), (int)plan.batches.size(), plan.max_k(), s);
+    NC_HIP(hipEventRecord(a.done[slot], s));
+}
+
+// ----------------------------------------------------------------------------
+// Mix kernel.  Blocks are grouped by destination (CSR); each thread owns one
+// element e of the slab and produces all d outputs of one destination.
+// ----------------------------------------------------------------------------
+struct MixDesc {
+    int d, ndst;
+    int64_t rs;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void mix_kernel(const double2* __restrict__ P, double2* __restrict__ Q,
+                                                  const int* __restrict__ dst_ptr,
+                                                  const int* __restrict__ blk_src,
+                                                  const double2* __restrict__ W, MixDesc md) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int dst = blockIdx.y;
+    if (e >= md.rs) return;
+    const int d = (D > 0) ? D : md.d;
+    double2 q[4] = {};
+    for (int t = dst_ptr[dst]; t < dst_ptr[dst + 1]; ++t) {
+        const int src = blk_src[t];
+        const double2* Wn = W + (int64_t)t * d * d;
+        for (int a = 0; a < d; ++a) {
+            const double2 p = P[((int64_t)src * d + a) * md.rs + e];
+            for (int b = 0; b < d; ++b) {
+                const double2 w = Wn[a + b * d];
+                q[b].x += w.x * p.x - w.y * p.y;
+                q[b].y += w.x * p.y + w.y * p.x;
+            }
+        }
+    }
+    for (int b = 0; b < d; ++b) Q[((int64_t)dst * d + b) * md.rs + e] = q[b];
+}
+
+void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q, int64_t rs,
+                hipStream_t s) {
+    if (op.d > 4) throw Error("mix_blocks: physical dimension above 4 is not supported");
+    const int dir = src_is_from ? 0 : 1;
+    const int ndst = (int)(src_is_from ? op.tos.size() : op.froms.size());
+    if (ndst == 0 || rs == 0) return;
+    MixDesc md{op.d, ndst, rs};
+    dim3 grid((unsigned)((rs + 255) / 256), (unsigned)ndst);
+    const int* ptr = op.mix_ptr[dir].ptr;
+    const int* srcs = op.mix_src[dir].ptr;
+    const double2* W = op.mix_W[dir].ptr;
+    switch (op.d) {
+        case 2: hipLaunchKernelGGL(mix_kernel<2>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        case 3: hipLaunchKernelGGL(mix_kernel<3>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        default: hipLaunchKernelGGL(mix_kernel<0>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+    }
+    NC_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------
+// Vector kernels
+// ----------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 512;
+
+__device__ __forceinline__ double warp_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// partial[j * gridDim.x + block] = sum over the block's grid-stride range of conj(V_j) w
+__global__ __launch_bounds__(RED_THREADS) void zdotc_multi_kernel(const double2* __restrict__ V, int64_t ldv,
+                                                                  int m, const double2* __restrict__ w,
+                                                                  int64_t n, double2* __restrict__ partial) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    for (int j = 0; j < m; ++j) {
+        double re = 0, im = 0;
+        const double2* Vj = V + (int64_t)j * ldv;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            const double2 a = Vj[i], b = w[i];
+            re += a.x * b.x + a.y * b.y;
+            im += a.x * b.y - a.y * b.x;
+        }
+        re = warp_sum(re);
+        im = warp_sum(im);
+        if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tr = 0, ti = 0;
+            for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+            partial[(int64_t)j * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+        }
+        __syncthreads();
+    }
+}
+
+// final reduction: out[j] = sum_b partial[j * nb + b]; one block per j
+__global__ __launch_bounds__(RED_THREADS) void reduce_partials_kernel(const double2* __restrict__ partial, int nb,
+                                                                      double2* __restrict__ out) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    const int j = blockIdx.x;
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        re += partial[(int64_t)j * nb + b].x;
+        im += partial[(int64_t)j * nb + b].y;
+    }
+    re = warp_sum(re);
+    im = warp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+        out[j] = make_double2(tr, ti);
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void zdotu_kernel(const double2* __restrict__ x, const double2* __restrict__ y,
+                                                            int64_t n, double2* __restrict__ partial) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    double re = 0, im = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = x[i], b = y[i];
+        re += a.x * b.x - a.y * b.y;
+        im += a.x * b.y + a.y * b.x;
+    }
+    re = warp_sum(re);
+    im = warp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+        partial[blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void trace_pair_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
+                                                                 int64_t b, int64_t c, double2* __restrict__ partial) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    double re = 0, im = 0;
+    const int64_t total = b * b * c;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % b, j = (e / b) % b, k = e / (b * b);
+        const double2 x = A[e], y = B[j + i * b + k * b * b];
+        re += x.x * y.x - x.y * y.y;
+        im += x.x * y.y + x.y * y.x;
+    }
+    re = warp_sum(re);
+    im = warp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+        partial[blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
+__global__ void zgemv_sub_kernel(const double2* __restrict__ V, int64_t ldv, int m, const double2* __restrict__ h,
+                                 double2* __restrict__ w, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = w[i];
+        for (int j = 0; j < m; ++j) {
+            const double2 v = V[(int64_t)j * ldv + i], c = h[j];
+            acc.x -= v.x * c.x - v.y * c.y;
+            acc.y -= v.x * c.y + v.y * c.x;
+        }
+        w[i] = acc;
+    }
+}
+
+__global__ void zgemv_n_kernel(const double2* __restrict__ V, int64_t ldv, int m, const double2* __restrict__ c,
+                               double2* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = make_double2(0, 0);
+        for (int j = 0; j < m; ++j) {
+            const double2 v = V[(int64_t)j * ldv + i], k = c[j];
+            acc.x += v.x * k.x - v.y * k.y;
+            acc.y += v.x * k.y + v.y * k.x;
+        }
+        out[i] = acc;
+    }
+}
+
+__global__ void zaxpby_kernel(double2* __restrict__ y, double2 a, const double2* __restrict__ x, double2 b, int64_t n,
+                              int has_x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 yv = y[i];
+        double2 o;
+        o.x = b.x * yv.x - b.y * yv.y;
+        o.y = b.x * yv.y + b.y * yv.x;
+        if (has_x) {
+            const double2 xv = x[i];
+            o.x += a.x * xv.x - a.y * xv.y;
+            o.y += a.x * xv.y + a.y * xv.x;
+        }
+        y[i] = o;
+    }
+}
+
+__global__ void conj_copy_kernel(double2* __restrict__ dst, const double2* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_double2(v.x, -v.y);
+    }
+}
+
+__global__ void permute_102_kernel(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n0, int64_t n1,
+                                   int64_t n2) {
+    const int64_t total = n0 * n1 * n2;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        // dst(j, i, k) = src(i, j, k), e enumerates dst
+        const int64_t j = e % n1, i = (e / n1) % n0, k = e / (n0 * n1);
+        dst[e] = src[i + j * n0 + k * n0 * n1];
+    }
+}
+
+__global__ void scale_rows_kernel(double2* __restrict__ X, const double* __restrict__ sv, int64_t m, int64_t ncol) {
+    const int64_t total = m * ncol;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const double f = sv[e % m];
+        double2 v = X[e];
+        X[e] = make_double2(v.x * f, v.y * f);
+    }
+}
+
+__global__ void add_identity_kernel(double2* __restrict__ X, int64_t m, double2 a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        double2 v = X[i + i * m];
+        X[i + i * m] = make_double2(v.x + a.x, v.y + a.y);
+    }
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+namespace {
+// pinned result buffer for reductions
+struct RedScratch {
+    DevBuf<double2> partial, out;
+    double2* host = nullptr;
+    RedScratch() {
+        partial.resize((size_t)RED_BLOCKS * 64);
+        out.resize(64);
+        NC_HIP(hipHostMalloc((void**)&host, 64 * sizeof(double2), hipHostMallocDefault));
+    }
+};
+RedScratch& red() {
+    static RedScratch* r = new RedScratch();
+    return *r;
+}
+int red_blocks(int64_t n) {
+    int64_t b = (n + RED_THREADS - 1) / RED_THREADS;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+}  // namespace
+
+void zdotc_multi(const double2* V, int64_t ldv, int m, const double2* w, int64_t n, zd* h_host, hipStream_t s) {
+    if (m <= 0) return;
+    auto& r = red();
+    int done = 0;
+    while (done < m) {
+        int mm = std::min(64, m - done);
+        int nb = red_blocks(n);
+        hipLaunchKernelGGL(zdotc_multi_kernel, dim3(nb), dim3(RED_THREADS), 0, s, V + (int64_t)done * ldv, ldv, mm, w, n,
+                           r.partial.ptr);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(mm), dim3(RED_THREADS), 0, s, r.partial.ptr, nb, r.out.ptr);
+        NC_HIP(hipGetLastError());
+        NC_HIP(hipMemcpyAsync(r.host, r.out.ptr, mm * sizeof(double2), hipMemcpyDeviceToHost, s));
+        NC_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < mm; ++j) h_host[done + j] = zd(r.host[j].x, r.host[j].y);
+        done += mm;
+    }
+}
+
+double dznrm2(const double2* x, int64_t n, hipStream_t s) {
+    zd h;
+    zdotc_multi(x, n, 1, x, n, &h, s);
+    return std::sqrt(std::max(0.0, h.real()));
+}
+
+zd zdotu(const double2* x, const double2* y, int64_t n, hipStream_t s) {
+    auto& r = red();
+    int nb = red_blocks(n);
+    hipLaunchKernelGGL(zdotu_kernel, dim3(nb), dim3(RED_THREADS), 0, s, x, y, n, r.partial.ptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(RED_THREADS), 0, s, r.partial.ptr, nb, r.out.ptr);
+    NC_HIP(hipGetLastError());
+    NC_HIP(hipMemcpyAsync(r.host, r.out.ptr, sizeof(double2), hipMemcpyDeviceToHost, s));
+    NC_HIP(hipStreamSynchronize(s));
+    return zd(r.host[0].x, r.host[0].y);
+}
+
+zd trace_pair(const double2* A, const double2* B, int64_t b, int64_t c, hipStream_t s) {
+    auto& r = red();
+    int nb = red_blocks(b * b * c);
+    hipLaunchKernelGGL(trace_pair_kernel, dim3(nb), dim3(RED_THREADS), 0, s, A, B, b, c, r.partial.ptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(RED_THREADS), 0, s, r.partial.ptr, nb, r.out.ptr);
+    NC_HIP(hipGetLastError());
+    NC_HIP(hipMemcpyAsync(r.host, r.out.ptr, sizeof(double2), hipMemcpyDeviceToHost, s));
+    NC_HIP(hipStreamSynchronize(s));
+    return zd(r.host[0].x, r.host[0].y);
+}
+
+namespace {
+// small coefficient vectors are staged through a pinned ring
+struct CoefRing {
+    static constexpr int kSlots = 64, kLen = 128;
+    double2* host = nullptr;
+    DevBuf<double2> dev;
+    hipEvent_t done[kSlots];
+    int next = 0;
+    CoefRing() {
+        NC_HIP(hipHostMalloc((void**)&host, kSlots * kLen * sizeof(double2), hipHostMallocDefault));
+        dev.resize(kSlots * kLen);
+        for (int i = 0; i < kSlots; ++i) {
+            NC_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+            NC_HIP(hipEventRecord(done[i], Context::get().stream));
+        }
+    }
+    const double2* stage(const zd* c, int m, hipStream_t s) {
+        if (m > kLen) throw Error("coefficient vector too long");
+        int slot = next;
+        next = (next + 1) % kSlots;
+        NC_HIP(hipEventSynchronize(done[slot]));
+        double2* h = host + slot * kLen;
+        for (int j = 0; j < m; ++j) h[j] = make_double2(c[j].real(), c[j].imag());
+        double2* d = dev.ptr + slot * kLen;
+        NC_HIP(hipMemcpyAsync(d, h, m * sizeof(double2), hipMemcpyHostToDevice, s));
+        NC_HIP(hipEventRecord(done[slot], s));
+        return d;
+    }
+};
+CoefRing& coef() {
+    static CoefRing* c = new CoefRing();
+    return *c;
+}
+}  // namespace
+
+void zgemv_sub(const double2* V, int64_t ldv, int m, const zd* h, double2* w, int64_t n, hipStream_t s) {
+    if (m <= 0) return;
+    const double2* dh = coef().stage(h, m, s);
+    hipLaunchKernelGGL(zgemv_sub_kernel, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, m, dh, w, n);
+    NC_HIP(hipGetLastError());
+}
+
+void zgemv_n(const double2* V, int64_t ldv, int m, const zd* c, double2* out, int64_t n, hipStream_t s) {
+    const double2* dc = coef().stage(c, m, s);
+    hipLaunchKernelGGL(zgemv_n_kernel, dim3(grid_for(n)), dim3(256), 0, s, V, ldv, m, dc, out, n);
+    NC_HIP(hipGetLastError());
+}
+
+void zaxpby(double2* y, zd a, const double2* x, zd b, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zaxpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, y, make_double2(a.real(), a.imag()), x,
+                       make_double2(b.real(), b.imag()), n, x != nullptr ? 1 : 0);
+    NC_HIP(hipGetLastError());
+}
+void zscal(double2* x, zd a, int64_t n, hipStream_t s) { zaxpby(x, zd(0, 0), nullptr, a, n, s); }
+void zaxpy(double2* y, zd a, const double2* x, int64_t n, hipStream_t s) { zaxpby(y, a, x, zd(1, 0), n, s); }
+
+void zconj_copy(double2* dst, const double2* src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(conj_copy_kernel, dim3(grid_for(n)), dim3(256), 0, s, dst, src, n);
+    NC_HIP(hipGetLastError());
+}
+void conj_inplace(double2* X, int64_t n, hipStream_t s) { zconj_copy(X, X, n, s); }
+
+void permute_102(const double2* src, double2* dst, int64_t n0, int64_t n1, int64_t n2, hipStream_t s) {
+    hipLaunchKernelGGL(permute_102_kernel, dim3(grid_for(n0 * n1 * n2)), dim3(256), 0, s, src, dst, n0, n1, n2);
+    NC_HIP(hipGetLastError());
+}
+
+void scale_rows(double2* X, const double* sv, int64_t m, int64_t ncol, hipStream_t s) {
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(grid_for(m * ncol)), dim3(256), 0, s, X, sv, m, ncol);
+    NC_HIP(hipGetLastError());
+}
+
+void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s) {
+    hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, X, m,
+                       make_double2(alpha.real(), alpha.imag()));
+    NC_HIP(hipGetLastError());
+}
+
+}  // namespace nc

@@ -1,0 +1,296 @@
+// Internal declarations of the MI355X (gfx950) variational-MPS core.
+//
+// Data layout (identical to the reference's Fortran core, core.f95): every tensor
+// is complex128 stored as interleaved (re, im) doubles in column-major order.
+//   state site    psi(r, l, s)   shape (br, bl, d)
+//   left env      L(i, j, k)     shape (bl, bl, cl)
+//   right env     R(r', r, k)    shape (br, br, cr)
+//   MPO blocks    idx(2, n) 1-based (from, to), mats(d, d, n) = W_n(a, b)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace nc {
+
+using zd = std::complex<double>;
+
+// ----------------------------------------------------------------------------
+// Errors
+// ----------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(std::string const& m, int c = -1) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what, const char* file, int line);
+#define NC_HIP(x) ::nc::check_hip((x), #x, __FILE__, __LINE__)
+
+void set_last_error(std::string const& m);
+std::string const& last_error();
+
+// ----------------------------------------------------------------------------
+// Device context: one stream per process on the current device, a caching
+// device allocator, and pinned staging for host <-> device copies.
+// ----------------------------------------------------------------------------
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    static Context& get();
+    void sync();
+};
+
+// Caching device allocator: blocks are rounded up to 4 KiB (or to the next
+// power-of-two quarter above 64 MiB) and kept on per-size free lists.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+size_t dev_bytes_in_use();
+void dev_release_cache();
+
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { resize(count); }
+    DevBuf(DevBuf const&) = delete;
+    DevBuf& operator=(DevBuf const&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) dev_free(ptr, n * sizeof(T));
+        ptr = nullptr; n = 0;
+    }
+    void resize(size_t count) {
+        if (count == n) return;
+        release();
+        if (count) ptr = static_cast<T*>(dev_alloc(count * sizeof(T)));
+        n = count;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+using ZBuf = DevBuf<double2>;
+
+void h2d(void* dst, void const* src, size_t bytes);
+void d2h(void* dst, void const* src, size_t bytes);
+void d2d(void* dst, void const* src, size_t bytes);
+void dzero(void* dst, size_t bytes);
+
+// ----------------------------------------------------------------------------
+// Hand-written complex GEMM (gfx950 MFMA).  C(M x N) = sum_s opA(A_s) opB(B_s)
+// (+ C when accumulate), batched over independent C's that share M and N.
+// op: 0 = N, 1 = T, 2 = C (conjugate transpose).
+// ----------------------------------------------------------------------------
+struct GemmSeg {
+    const double2* A;
+    const double2* B;
+    int64_t lda, ldb;
+    int32_t K, _pad;
+};
+struct GemmBatch {
+    double2* C;
+    int64_t ldc;
+    int32_t seg_begin, seg_end;
+};
+
+enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
+
+// Launch with descriptors already resident on the device.
+void zgemm_launch(int opA, int opB, bool accumulate, int M, int N,
+                  GemmBatch const* d_batches, GemmSeg const* d_segs, int nbatch,
+                  int max_k_total, hipStream_t s);
+
+// Host-side builder: keeps descriptor lists and uploads them per launch into a
+// small reusable device arena.
+struct GemmPlan {
+    std::vector<GemmBatch> batches;
+    std::vector<GemmSeg> segs;
+    int M = 0, N = 0, opA = OpN, opB = OpN;
+    bool accumulate = false;
+    void clear() { batches.clear(); segs.clear(); }
+    void begin_batch(double2* C, int64_t ldc) {
+        batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size()});
+    }
+    void add_seg(const double2* A, int64_t lda, const double2* B, int64_t ldb, int K) {
+        segs.push_back(GemmSeg{A, B, lda, ldb, K, 0});
+        batches.back().seg_end = (int32_t)segs.size();
+    }
+    int max_k() const;
+};
+
+// Upload-and-launch (descriptor arena is stream ordered).
+void zgemm_run(GemmPlan const& plan, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Sparse MPO block list, prepared for grouped contraction.
+// ----------------------------------------------------------------------------
+struct MpoSite {
+    int d = 0, cl = 0, cr = 0, nmat = 0;
+    std::vector<uint32_t> idx;   // 2 * nmat, 1-based (from, to)
+    std::vector<zd> mats;        // d * d * nmat, W_n(a, b) column-major
+    // distinct from / to indices (0-based, sorted) and each block's position in them
+    std::vector<int> froms, tos;
+    std::vector<int> blk_from, blk_to;  // compacted index per block
+    // destination-grouped block tables for the mix kernel, per direction
+    // (dir 0: source = from, destination = to; dir 1: source = to, destination = from)
+    DevBuf<int> mix_ptr[2], mix_src[2];
+    DevBuf<double2> mix_W[2];
+    void build(int d, int cl, int cr, int nmat, const uint32_t* idx1, const zd* mats);
+};
+
+// Mix kernel: Q(e, b, dst) = sum_{n : dst(n) = dst} sum_a W_n(a, b) P(e, a, src(n))
+// where e runs over an element slab of size rs.  P has d * nsrc slabs, Q d * ndst.
+void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q,
+                int64_t rs, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Contractions (FP64, grouped).  All pointers are device pointers.
+// ----------------------------------------------------------------------------
+struct Workspace {
+    ZBuf P, Q;
+    void ensure(size_t nP, size_t nQ) {
+        if (P.n < nP) P.resize(nP);
+        if (Q.n < nQ) Q.resize(nQ);
+    }
+};
+
+// y = H_eff x (no projectors), the product of core.f95's iteration_stage_1/2/3.
+void heff_matvec(int bl, int br, MpoSite const& op, const double2* L, const double2* R,
+                 const double2* x, double2* y, Workspace& ws, hipStream_t s);
+// New left environment (br, br, cr) from L (bl, bl, cl) and psi (core.f95:623-677).
+void env_left(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
+              double2* newL, Workspace& ws, hipStream_t s);
+// New right environment (bl, bl, cl) from R (br, br, cr) and psi (core.f95:679-848).
+void env_right(int bl, int br, MpoSite const& op, const double2* R, const double2* psi,
+               double2* newR, Workspace& ws, hipStream_t s);
+// <psi|H|psi> at a site (core.f95:1045-1088).
+zd expectation(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
+               const double2* R, Workspace& ws, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Vector kernels (Arnoldi / projectors), device pointers, length n.
+// ----------------------------------------------------------------------------
+// h[j] = sum_i conj(V[j*ldv + i]) w[i], j < m   (result to host)
+void zdotc_multi(const double2* V, int64_t ldv, int m, const double2* w, int64_t n, zd* h_host,
+                 hipStream_t s);
+// w -= sum_j V_j h[j]
+void zgemv_sub(const double2* V, int64_t ldv, int m, const zd* h, double2* w, int64_t n,
+               hipStream_t s);
+double dznrm2(const double2* x, int64_t n, hipStream_t s);
+zd zdotu(const double2* x, const double2* y, int64_t n, hipStream_t s);  // sum x*y (no conj)
+void zscal(double2* x, zd a, int64_t n, hipStream_t s);
+void zaxpy(double2* y, zd a, const double2* x, int64_t n, hipStream_t s);  // y += a x
+// y = a x + b y  (general combination)
+void zaxpby(double2* y, zd a, const double2* x, zd b, int64_t n, hipStream_t s);
+// out = V c (m columns)
+void zgemv_n(const double2* V, int64_t ldv, int m, const zd* c, double2* out, int64_t n,
+             hipStream_t s);
+void zconj_copy(double2* dst, const double2* src, int64_t n, hipStream_t s);
+// permute a (n0, n1, n2) tensor to (n1, n0, n2)
+void permute_102(const double2* src, double2* dst, int64_t n0, int64_t n1, int64_t n2,
+                 hipStream_t s);
+// scale rows: X(i, j) *= sv[i] for an (m x ncol) matrix
+void scale_rows(double2* X, const double* sv, int64_t m, int64_t ncol, hipStream_t s);
+void conj_inplace(double2* X, int64_t n, hipStream_t s);
+// add alpha * I to an (m x m) matrix
+void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s);
+// sum_k sum_ij A(i, j, k) B(j, i, k)
+zd trace_pair(const double2* A, const double2* B, int64_t b, int64_t c, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Projectors: Householder reflectors (LAPACK storage) applied to vectors.
+// ----------------------------------------------------------------------------
+struct Projector {
+    int n = 0, nproj = 0, nrefl = 0, orth = 0;
+    std::vector<uint32_t> swaps;  // nrefl, 1-based
+    std::vector<zd> coeffs;       // nrefl
+    DevBuf<double2> d_refl;       // n * nproj
+    bool active() const { return nproj > 0; }
+    void set(int n, int nproj, int nrefl, int orth, const zd* refl, const zd* coeffs,
+             const uint32_t* swaps);
+    // full (n) -> orthogonal (orth), and back
+    void project(const double2* full, double2* out, double2* tmp, hipStream_t s) const;
+    void unproject(const double2* in, double2* full, hipStream_t s) const;
+    double overlap(const double2* v, double2* tmp, hipStream_t s) const;
+};
+
+// ----------------------------------------------------------------------------
+// Eigen-solver (implicitly restarted Arnoldi, nev = 1) with device vectors.
+// ----------------------------------------------------------------------------
+struct Operator {
+    int64_t n = 0;
+    virtual void apply(const double2* x, double2* y, hipStream_t s) = 0;
+    virtual ~Operator() = default;
+};
+
+struct IramResult {
+    int info = 0;        // 0 ok, -14 no converged Ritz value, other < 0 failures
+    int restarts = 0;    // ARPACK iparam(3)
+    int64_t matvecs = 0;
+    zd eigenvalue{0, 0};
+};
+
+// which: "SR", "LR", "LM".  guess / result are device vectors of length op.n.
+IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_restarts,
+                const double2* guess, double2* result, hipStream_t s);
+
+// Small dense host eigen-problems.
+// Eigenvalues and right eigenvectors of a general complex upper Hessenberg matrix.
+void hessenberg_eig(int m, std::vector<zd> const& H, std::vector<zd>& w, std::vector<zd>& V);
+// Hermitian (upper triangle) eigen-decomposition, ascending eigenvalues.
+void hermitian_eig(int m, std::vector<zd> const& A, std::vector<double>& w, std::vector<zd>& V);
+
+// ----------------------------------------------------------------------------
+// optimize_ (core.f95:1911-2609) on the device.
+// ----------------------------------------------------------------------------
+struct OptimizeArgs {
+    int bl, br, cl, cr, d;
+    const double2* L;  // device
+    const double2* R;  // device
+    MpoSite const* op;
+    Projector const* proj;  // may be inactive
+    const char* which;
+    double tol;
+    int max_iterations;
+    const double2* guess;  // device, full space
+    double2* result;       // device, full space
+    int ncv = 3;
+};
+struct OptimizeOut {
+    int info = 0;
+    int iterations = 0;
+    int strategy = 0;
+    int64_t matvecs = 0;
+    zd eigenvalue{0, 0};
+    double normal = 0;
+};
+OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Site move: polar split of an (m x ncol) matrix, m <= ncol.
+//   U_p = U Vh (rows orthonormal), P = U S U^H, so that A = P U_p.
+// Returns the SVD info code (0 on success).  ``Pconj`` receives conj(P) (m x m).
+// ----------------------------------------------------------------------------
+int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s);
+
+// norm_denorm_going_left / right on device tensors (core.f95:2708-2878).
+int norm_denorm_left(int bl, int bm, int br, int dl, int dr, const double2* to_denorm,
+                     const double2* to_norm, double2* denorm_out, double2* norm_out, hipStream_t s);
+int norm_denorm_right(int bl, int bm, int br, int dl, int dr, const double2* to_norm,
+                      const double2* to_denorm, double2* norm_out, double2* denorm_out,
+                      hipStream_t s);
+
+}  // namespace nc

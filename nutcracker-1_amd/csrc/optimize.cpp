@@ -1,0 +1,277 @@
+// Site optimizer: the reference's optimize dispatcher and its three strategies
+// (core.f95:1911-2609), with every vector operation on the device, plus the
+// Householder projector algebra used for excited states (core.f95:464-504,
+// 1266-1339, 1513-1664).
+#include "nc_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace nc {
+
+// ----------------------------------------------------------------------------
+// Projectors
+// ----------------------------------------------------------------------------
+__global__ void swap_entries_kernel(double2* x, const uint32_t* swaps, int nsw, int forward) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (forward) {
+        for (int i = 0; i < nsw; ++i) {
+            int s = (int)swaps[i] - 1;
+            if (s != i) { double2 t = x[i]; x[i] = x[s]; x[s] = t; }
+        }
+    } else {
+        for (int i = nsw - 1; i >= 0; --i) {
+            int s = (int)swaps[i] - 1;
+            if (s != i) { double2 t = x[i]; x[i] = x[s]; x[s] = t; }
+        }
+    }
+}
+
+// explicit reflector vectors: v_i (zeros above i, 1 at i, stored part below) and conj(v_i)
+__global__ void build_reflectors_kernel(const double2* refl, double2* V, double2* Vc, int64_t n, int nrefl) {
+    const int64_t total = n * nrefl;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e % n;
+        const int i = (int)(e / n);
+        double2 v;
+        if (j < i) v = make_double2(0, 0);
+        else if (j == i) v = make_double2(1, 0);
+        else v = refl[e];
+        V[e] = v;
+        Vc[e] = make_double2(v.x, -v.y);
+    }
+}
+
+namespace {
+struct ProjectorDevice {
+    ZBuf V, Vc;           // nrefl explicit reflectors
+    DevBuf<uint32_t> swaps;
+    std::vector<zd> Rtop;  // nrefl x nrefl upper block of the stored reflectors
+};
+}  // namespace
+
+static thread_local std::vector<std::pair<Projector const*, std::shared_ptr<ProjectorDevice>>> g_proj_cache;
+
+static ProjectorDevice& proj_dev(Projector const& p) {
+    for (auto& kv : g_proj_cache)
+        if (kv.first == &p) return *kv.second;
+    throw Error("projector used before set()", -30);
+}
+
+void Projector::set(int n_, int nproj_, int nrefl_, int orth_, const zd* refl, const zd* coeffs_,
+                    const uint32_t* swaps_) {
+    n = n_; nproj = nproj_; nrefl = nrefl_; orth = orth_;
+    if (nrefl > nproj || nrefl > n) throw Error("projector: more reflectors than vectors", -31);
+    swaps.assign(swaps_, swaps_ + nrefl);
+    coeffs.assign(coeffs_, coeffs_ + nrefl);
+    for (auto s : swaps)
+        if (s < 1 || (int)s > nrefl) throw Error("projector: swap index out of range", -32);
+    d_refl.resize((size_t)n * std::max(1, nproj));
+    if (nproj) h2d(d_refl.ptr, refl, (size_t)n * nproj * sizeof(double2));
+    auto dev = std::make_shared<ProjectorDevice>();
+    if (nrefl) {
+        dev->V.resize((size_t)n * nrefl);
+        dev->Vc.resize((size_t)n * nrefl);
+        hipStream_t s = Context::get().stream;
+        unsigned g = (unsigned)std::min<int64_t>(4096, ((int64_t)n * nrefl + 255) / 256);
+        hipLaunchKernelGGL(build_reflectors_kernel, dim3(g), dim3(256), 0, s, d_refl.ptr, dev->V.ptr, dev->Vc.ptr,
+                           (int64_t)n, nrefl);
+        NC_HIP(hipGetLastError());
+        dev->swaps.resize(nrefl);
+        h2d(dev->swaps.ptr, swaps.data(), nrefl * sizeof(uint32_t));
+        dev->Rtop.resize((size_t)nrefl * nrefl);
+        for (int j = 0; j < nrefl; ++j)
+            for (int i = 0; i < nrefl; ++i) dev->Rtop[(size_t)i + (size_t)j * nrefl] = refl[(size_t)i + (size_t)j * n];
+    }
+    for (auto it = g_proj_cache.begin(); it != g_proj_cache.end(); ++it)
+        if (it->first == this) { g_proj_cache.erase(it); break; }
+    g_proj_cache.push_back({this, dev});
+}
+
+// x <- x H_0 H_1 ... (forward, zunmqr 'R','N') or x <- x H_{k-1}^H ... H_0^H ('R','C')
+static void apply_reflectors_row(Projector const& p, double2* x, bool conj_trans, hipStream_t s) {
+    ProjectorDevice& dv = proj_dev(p);
+    const int64_t n = p.n;
+    for (int t = 0; t < p.nrefl; ++t) {
+        const int i = conj_trans ? p.nrefl - 1 - t : t;
+        const zd tau = conj_trans ? std::conj(p.coeffs[i]) : p.coeffs[i];
+        if (tau == zd(0, 0)) continue;
+        // x H = x - tau (x v) v^H
+        const zd sdot = zdotu(x, dv.V.ptr + (size_t)i * n, n, s);
+        zaxpy(x, -tau * sdot, dv.Vc.ptr + (size_t)i * n, n, s);
+    }
+}
+
+void Projector::project(const double2* full, double2* out, double2* tmp, hipStream_t s) const {
+    if (!active()) { d2d(out, full, (size_t)n * sizeof(double2)); return; }
+    d2d(tmp, full, (size_t)n * sizeof(double2));
+    apply_reflectors_row(*this, tmp, false, s);
+    if (nrefl) {
+        hipLaunchKernelGGL(swap_entries_kernel, dim3(1), dim3(1), 0, s, tmp, proj_dev(*this).swaps.ptr, nrefl, 1);
+        NC_HIP(hipGetLastError());
+    }
+    d2d(out, tmp + (n - orth), (size_t)orth * sizeof(double2));
+}
+
+void Projector::unproject(const double2* in, double2* full, hipStream_t s) const {
+    if (!active()) { d2d(full, in, (size_t)n * sizeof(double2)); return; }
+    if (n - orth > 0) dzero(full, (size_t)(n - orth) * sizeof(double2));
+    d2d(full + (n - orth), in, (size_t)orth * sizeof(double2));
+    if (nrefl) {
+        hipLaunchKernelGGL(swap_entries_kernel, dim3(1), dim3(1), 0, s, full, proj_dev(*this).swaps.ptr, nrefl, 0);
+        NC_HIP(hipGetLastError());
+    }
+    apply_reflectors_row(*this, full, true, s);
+}
+
+double Projector::overlap(const double2* v, double2* tmp, hipStream_t s) const {
+    if (!active()) return 0.0;
+    // compute_overlap_with_projectors (core.f95:1266-1339)
+    d2d(tmp, v, (size_t)n * sizeof(double2));
+    apply_reflectors_row(*this, tmp, false, s);
+    if (nrefl) {
+        auto& dv = proj_dev(*this);
+        hipLaunchKernelGGL(swap_entries_kernel, dim3(1), dim3(1), 0, s, tmp, dv.swaps.ptr, nrefl, 1);
+        hipLaunchKernelGGL(swap_entries_kernel, dim3(1), dim3(1), 0, s, tmp, dv.swaps.ptr, nrefl, 0);
+        NC_HIP(hipGetLastError());
+    }
+    std::vector<zd> head(std::max(1, nrefl));
+    if (nrefl) {
+        std::vector<double2> h(nrefl);
+        d2h(h.data(), tmp, nrefl * sizeof(double2));
+        for (int i = 0; i < nrefl; ++i) head[i] = zd(h[i].x, h[i].y);
+    }
+    double acc = 0;
+    if (nrefl) {
+        auto& R = proj_dev(*this).Rtop;
+        for (int j = 0; j < nrefl; ++j) {  // w_j = sum_{i <= j} R(i, j) head_i  (ztrmv 'U','T')
+            zd wj = 0;
+            for (int i = 0; i <= j; ++i) wj += R[(size_t)i + (size_t)j * nrefl] * head[i];
+            acc += std::norm(wj);
+        }
+    }
+    for (int j = nrefl; j < nproj; ++j) {
+        zd wj = zdotu(d_refl.ptr + (size_t)j * n, tmp, n, s);
+        acc += std::norm(wj);
+    }
+    return std::sqrt(acc);
+}
+
+// ----------------------------------------------------------------------------
+// operate_on (core.f95:2529-2556): unproject -> H_eff -> project
+// ----------------------------------------------------------------------------
+namespace {
+struct HeffOperator : Operator {
+    OptimizeArgs const& a;
+    Workspace& ws;
+    ZBuf full_in, full_out, tmp;
+    HeffOperator(OptimizeArgs const& a_, Workspace& ws_) : a(a_), ws(ws_) {
+        const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+        n = a.proj->active() ? a.proj->orth : nfull;
+        if (a.proj->active()) {
+            full_in.resize(nfull);
+            full_out.resize(nfull);
+            tmp.resize(nfull);
+        }
+    }
+    void apply(const double2* x, double2* y, hipStream_t s) override {
+        if (!a.proj->active()) {
+            heff_matvec(a.bl, a.br, *a.op, a.L, a.R, x, y, ws, s);
+            return;
+        }
+        a.proj->unproject(x, full_in.ptr, s);
+        heff_matvec(a.bl, a.br, *a.op, a.L, a.R, full_in.ptr, full_out.ptr, ws, s);
+        a.proj->project(full_out.ptr, y, tmp.ptr, s);
+    }
+};
+}  // namespace
+
+// strategy 1 (core.f95:2049-2129): dense projected H_eff and a Hermitian eigen-solve
+static void strategy_1(OptimizeArgs const& a, HeffOperator& op, OptimizeOut& out, hipStream_t s) {
+    const int m = (int)op.n;
+    ZBuf e((size_t)m), col((size_t)m), pv((size_t)m);
+    std::vector<zd> H((size_t)m * m);
+    std::vector<double2> hcol(m);
+    for (int j = 0; j < m; ++j) {
+        dzero(e.ptr, (size_t)m * sizeof(double2));
+        double2 one = make_double2(1, 0);
+        NC_HIP(hipMemcpyAsync(e.ptr + j, &one, sizeof(double2), hipMemcpyHostToDevice, s));
+        op.apply(e.ptr, col.ptr, s);
+        ++out.matvecs;
+        d2h(hcol.data(), col.ptr, (size_t)m * sizeof(double2));
+        for (int i = 0; i < m; ++i) H[(size_t)i + (size_t)j * m] = zd(hcol[i].x, hcol[i].y);
+    }
+    std::vector<double> w;
+    std::vector<zd> V;
+    hermitian_eig(m, H, w, V);
+    int k;
+    const char* which = a.which;
+    if (which[0] == 'S' && which[1] == 'R') k = 0;
+    else if (which[0] == 'L' && which[1] == 'R') k = m - 1;
+    else if (which[0] == 'L' && which[1] == 'M') k = (std::abs(w[m - 1]) > std::abs(w[0])) ? m - 1 : 0;
+    else throw Error(std::string("Invalid eigenvalue requested: ") + std::string(which, 2), -40);
+    std::vector<double2> hv(m);
+    for (int i = 0; i < m; ++i) hv[i] = make_double2(V[(size_t)i + (size_t)k * m].real(), V[(size_t)i + (size_t)k * m].imag());
+    h2d(pv.ptr, hv.data(), (size_t)m * sizeof(double2));
+    a.proj->unproject(pv.ptr, a.result, s);
+    out.eigenvalue = zd(w[k], 0.0);
+    out.info = 0;
+    out.iterations = 0;
+}
+
+OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
+    OptimizeOut out;
+    const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+    const int orth = a.proj->active() ? a.proj->orth : (int)nfull;
+    out.iterations = a.max_iterations;
+    if (orth > nfull) { out.info = 5; return out; }
+    double overlap = 0.0;
+    if (a.proj->active()) {
+        ZBuf tmp((size_t)nfull);
+        overlap = a.proj->overlap(a.guess, tmp.ptr, s);
+    }
+    if (orth <= 0) { out.info = 10; return out; }
+    if (orth < nfull && overlap > 1e-7) { out.info = 11; return out; }
+
+    HeffOperator op(a, ws);
+    if (orth <= 4) {
+        out.strategy = 1;
+        strategy_1(a, op, out, s);
+    } else {
+        out.strategy = (a.bl * a.br < a.cl * a.cr) ? 2 : 3;
+        ZBuf pguess((size_t)orth), pres((size_t)orth), tmp((size_t)nfull);
+        a.proj->project(a.guess, pguess.ptr, tmp.ptr, s);
+        int ncv = a.ncv;
+        int iterations = a.max_iterations;
+        IramResult r = iram(op, ncv, a.which, a.tol, iterations, pguess.ptr, pres.ptr, s);
+        out.matvecs += r.matvecs;
+        iterations = r.restarts;
+        bool done = false;
+        if (r.info == -14) {
+            if (nfull < iterations) {
+                strategy_1(a, op, out, s);
+                done = true;
+            } else {
+                while (ncv <= iterations) {
+                    ncv = ncv * 3 / 2;
+                    r = iram(op, ncv, a.which, a.tol, iterations, pguess.ptr, pres.ptr, s);
+                    out.matvecs += r.matvecs;
+                    iterations = r.restarts;
+                    if (r.info != -14) break;
+                }
+            }
+        }
+        if (!done) {
+            out.info = r.info;
+            out.iterations = iterations;
+            out.eigenvalue = r.eigenvalue;
+            if (r.info == 0) a.proj->unproject(pres.ptr, a.result, s);
+            else dzero(a.result, (size_t)nfull * sizeof(double2));
+        }
+    }
+    out.normal = dznrm2(a.result, nfull, s);
+    return out;
+}
+
+}  // namespace nc

@@ -1,0 +1,336 @@
+"""Python mirror of the reference's ``Nutcracker::Core`` wrappers
+(includes/nutcracker/core.hpp:34-302, sources/common/core.cpp) over the C-ABI of
+libnutcracker_hip.so.
+
+Each function takes and returns numpy complex128 arrays in Fortran order, with the
+same dimension arguments, in the same order, as the C++ wrapper of the same name.
+Every call runs on the GPU: if the shared library is missing or no device is
+visible, the call raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+CPLX = np.complex128
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnutcracker_hip.so")
+
+_u32p = C.POINTER(C.c_uint32)
+_zp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+_SIGNATURES = {
+    "contract_sos_left_": (None, [_u32p] * 5 + [_zp, _u32p, _u32p, _zp, _zp, _zp]),
+    "contract_sos_right_": (None, [_u32p] * 5 + [_zp, _u32p, _u32p, _zp, _zp, _zp]),
+    "compute_expectation_": (None, [_u32p] * 5 + [_zp, _zp, _u32p, _u32p, _zp, _zp, _zp]),
+    "contract_expectation_boundaries_": (None, [_u32p, _u32p, _zp, _zp, _zp]),
+    "compute_optimization_matrix_": (None, [_u32p] * 5 + [_zp, _u32p, _u32p, _zp, _zp, _zp]),
+    "iteration_stage_1_": (None, [_u32p] * 4 + [_zp, _u32p, _u32p, _zp, _zp]),
+    "iteration_stage_2_": (None, [_u32p] * 4 + [_zp, _zp, _zp]),
+    "iteration_stage_3_": (None, [_u32p] * 4 + [_zp, _zp, _zp]),
+    "optimize_": (C.c_uint32, [_u32p] * 5 + [_zp, _u32p, _u32p, _zp, _zp, _u32p, _u32p, _u32p, _zp, _zp, _u32p,
+                                            C.c_char_p, _dp, _u32p, _zp, _zp, _zp, _dp]),
+    "norm_denorm_going_left_": (C.c_int32, [_u32p] * 5 + [_zp] * 4),
+    "norm_denorm_going_right_": (C.c_int32, [_u32p] * 5 + [_zp] * 4),
+    "compute_overlap_with_projectors_": (None, [_u32p, _u32p, _zp, _zp, _u32p, _u32p, _zp, _zp]),
+    "filter_components_outside_orthog_": (None, [_u32p] * 4 + [_zp, _zp, _u32p, _zp, _zp]),
+    "contract_vs_left_": (None, [_u32p] * 5 + [_zp] * 4),
+    "contract_vs_right_": (None, [_u32p] * 5 + [_zp] * 4),
+    "nc_last_error": (C.c_char_p, []),
+    "nc_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(C.c_size_t)]),
+    "nc_project_into_orthogonal_space": (None, [C.c_uint32] * 4 + [_zp, _zp, _u32p, _zp, _zp]),
+    "nc_unproject_from_orthogonal_space": (None, [C.c_uint32] * 4 + [_zp, _zp, _u32p, _zp, _zp]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP core (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libnutcracker_hip.so is not built (run python nutcracker-1_amd/build.py); "
+                "the MI355X core has no CPU fallback")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def last_error() -> str:
+    v = lib().nc_last_error()
+    return v.decode() if v else ""
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(128)
+    cu = C.c_int(0)
+    clk = C.c_double(0)
+    mem = C.c_size_t(0)
+    rc = lib().nc_device_info(name, 128, C.byref(cu), C.byref(clk), C.byref(mem))
+    if rc != 0:
+        raise RuntimeError(last_error())
+    return {"arch": name.value.decode(), "compute_units": cu.value, "clock_ghz": clk.value, "memory": mem.value}
+
+
+def _u(v) -> C.POINTER(C.c_uint32):
+    return C.pointer(C.c_uint32(int(v)))
+
+
+def _z(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _zin(a, n: int) -> np.ndarray:
+    a = np.asarray(a, dtype=CPLX)
+    a = np.asfortranarray(a)
+    if a.size != n:
+        raise ValueError("array has %d elements, expected %d" % (a.size, n))
+    return a
+
+
+def _idx(idx) -> Tuple[np.ndarray, int]:
+    i = np.asfortranarray(np.asarray(idx, dtype=np.uint32).reshape(2, -1, order="F"))
+    return i, i.shape[1]
+
+
+def _check_error(before: str = ""):
+    msg = last_error()
+    if msg and msg != before:
+        raise RuntimeError(msg)
+
+
+class _ErrorWatch:
+    def __enter__(self):
+        self.before = last_error()
+        return self
+
+    def __exit__(self, *exc):
+        if exc[0] is None:
+            _check_error(self.before)
+        return False
+
+
+def contract_sos_left(bl, br, cl, cr, d, L, idx, mats, psi) -> np.ndarray:
+    """core.hpp contract_sos_left; returns the new left environment (br, br, cr)."""
+    i, nmat = _idx(idx)
+    L = _zin(L, bl * bl * cl)
+    m = _zin(mats, d * d * nmat)
+    p = _zin(psi, br * bl * d)
+    out = np.zeros((br, br, cr), CPLX, order="F")
+    with _ErrorWatch():
+        lib().contract_sos_left_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(L), _u(nmat),
+                                 i.ctypes.data_as(_u32p), _z(m), _z(p), _z(out))
+    return out
+
+
+def contract_sos_right(bl, br, cl, cr, d, R, idx, mats, psi) -> np.ndarray:
+    """core.hpp contract_sos_right; returns the new right environment (bl, bl, cl)."""
+    i, nmat = _idx(idx)
+    R = _zin(R, br * br * cr)
+    m = _zin(mats, d * d * nmat)
+    p = _zin(psi, br * bl * d)
+    out = np.zeros((bl, bl, cl), CPLX, order="F")
+    with _ErrorWatch():
+        lib().contract_sos_right_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(R), _u(nmat),
+                                  i.ctypes.data_as(_u32p), _z(m), _z(p), _z(out))
+    return out
+
+
+def compute_expectation(bl, br, cl, cr, d, L, psi, idx, mats, R) -> complex:
+    i, nmat = _idx(idx)
+    L = _zin(L, bl * bl * cl)
+    R = _zin(R, br * br * cr)
+    m = _zin(mats, d * d * nmat)
+    p = _zin(psi, br * bl * d)
+    out = np.zeros(1, CPLX)
+    with _ErrorWatch():
+        lib().compute_expectation_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(L), _z(p), _u(nmat),
+                                   i.ctypes.data_as(_u32p), _z(m), _z(R), _z(out))
+    return complex(out[0])
+
+
+def contract_expectation_boundaries(b, c, L, R) -> complex:
+    L = _zin(L, b * b * c)
+    R = _zin(R, b * b * c)
+    out = np.zeros(1, CPLX)
+    with _ErrorWatch():
+        lib().contract_expectation_boundaries_(_u(b), _u(c), _z(L), _z(R), _z(out))
+    return complex(out[0])
+
+
+def compute_optimization_matrix(bl, br, cl, cr, d, L, idx, mats, R) -> np.ndarray:
+    i, nmat = _idx(idx)
+    L = _zin(L, bl * bl * cl)
+    R = _zin(R, br * br * cr)
+    m = _zin(mats, d * d * nmat)
+    n = br * bl * d
+    out = np.zeros((n, n), CPLX, order="F")
+    with _ErrorWatch():
+        lib().compute_optimization_matrix_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(L), _u(nmat),
+                                           i.ctypes.data_as(_u32p), _z(m), _z(R), _z(out))
+    return out
+
+
+def iteration_stage_1(bl, cl, cr, d, L, idx, mats) -> np.ndarray:
+    i, nmat = _idx(idx)
+    L = _zin(L, bl * bl * cl)
+    m = _zin(mats, d * d * nmat)
+    out = np.zeros((bl, d, cr, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        lib().iteration_stage_1_(_u(bl), _u(cl), _u(cr), _u(d), _z(L), _u(nmat), i.ctypes.data_as(_u32p),
+                                 _z(m), _z(out))
+    return out
+
+
+def iteration_stage_2(bl, br, cr, d, I, x) -> np.ndarray:
+    I = _zin(I, bl * d * cr * bl * d)
+    x = _zin(x, br * bl * d)
+    out = np.zeros((br, cr, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        lib().iteration_stage_2_(_u(bl), _u(br), _u(cr), _u(d), _z(I), _z(x), _z(out))
+    return out
+
+
+def iteration_stage_3(bl, br, cr, d, T, R) -> np.ndarray:
+    T = _zin(T, br * cr * bl * d)
+    R = _zin(R, br * br * cr)
+    out = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        lib().iteration_stage_3_(_u(bl), _u(br), _u(cr), _u(d), _z(T), _z(R), _z(out))
+    return out
+
+
+def optimize(bl, br, cl, cr, d, L, idx, mats, R, nproj, nrefl, orth, reflectors, coeffs, swaps,
+             which, tol, iterations, guess):
+    """core.hpp optimize.  Returns (info, iterations, result(br,bl,d), eigenvalue, normal);
+    info follows the reference: 0, -14, 5, 10, 11."""
+    i, nmat = _idx(idx)
+    L = _zin(L, bl * bl * cl)
+    R = _zin(R, br * br * cr)
+    m = _zin(mats, d * d * nmat)
+    n = br * bl * d
+    g = _zin(guess, n)
+    if nproj:
+        refl = _zin(reflectors, n * nproj)
+        co = _zin(coeffs, max(1, nrefl)) if nrefl else np.zeros(1, CPLX)
+        sw = np.ascontiguousarray(np.asarray(swaps, np.uint32).reshape(-1)[:max(1, nrefl)]) if nrefl else np.ones(1, np.uint32)
+    else:
+        refl = np.zeros(1, CPLX)
+        co = np.zeros(1, CPLX)
+        sw = np.ones(1, np.uint32)
+    res = np.zeros((br, bl, d), CPLX, order="F")
+    ev = np.zeros(1, CPLX)
+    normal = C.c_double(0.0)
+    it = C.c_uint32(int(iterations))
+    tolv = C.c_double(float(tol))
+    w = which.encode() if isinstance(which, str) else bytes(which)
+    with _ErrorWatch():
+        info = lib().optimize_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(L), _u(nmat), i.ctypes.data_as(_u32p),
+                               _z(m), _z(R), _u(nproj), _u(nrefl), _u(orth), _z(refl), _z(co),
+                               sw.ctypes.data_as(_u32p), w, C.byref(tolv), C.byref(it), _z(g), _z(res), _z(ev),
+                               C.byref(normal))
+    info = C.c_int32(info).value
+    return info, int(it.value), res, complex(ev[0]), float(normal.value)
+
+
+def norm_denorm_going_left(bll, bl, br, dl, d, to_denorm, to_norm):
+    """core.hpp norm_denorm_going_left (C-side names): to_denorm (bl, bll, dl),
+    to_norm (br, bl, d).  Returns (info, denormalized, normalized)."""
+    td = _zin(to_denorm, bl * bll * dl)
+    tn = _zin(to_norm, br * bl * d)
+    od = np.zeros((bl, bll, dl), CPLX, order="F")
+    on = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        info = lib().norm_denorm_going_left_(_u(bll), _u(bl), _u(br), _u(dl), _u(d), _z(td), _z(tn), _z(od), _z(on))
+    return int(info), od, on
+
+
+def norm_denorm_going_right(bl, br, brr, d, dr, to_norm, to_denorm):
+    """core.hpp norm_denorm_going_right (C-side names): to_norm (br, bl, d),
+    to_denorm (brr, br, dr).  Returns (info, normalized, denormalized)."""
+    tn = _zin(to_norm, br * bl * d)
+    td = _zin(to_denorm, brr * br * dr)
+    on = np.zeros((br, bl, d), CPLX, order="F")
+    od = np.zeros((brr, br, dr), CPLX, order="F")
+    with _ErrorWatch():
+        info = lib().norm_denorm_going_right_(_u(bl), _u(br), _u(brr), _u(d), _u(dr), _z(tn), _z(td), _z(on), _z(od))
+    return int(info), on, od
+
+
+def compute_overlap_with_projectors(nproj, nrefl, reflectors, coeffs, swaps, n, vector) -> float:
+    refl = _zin(reflectors, n * nproj) if nproj else np.zeros(1, CPLX)
+    co = _zin(coeffs, nrefl) if nrefl else np.zeros(1, CPLX)
+    sw = np.ascontiguousarray(np.asarray(swaps, np.uint32).reshape(-1)) if nrefl else np.ones(1, np.uint32)
+    v = _zin(vector, n)
+    out = np.zeros(1, CPLX)
+    with _ErrorWatch():
+        lib().compute_overlap_with_projectors_(_u(nproj), _u(nrefl), _z(refl), _z(co), sw.ctypes.data_as(_u32p),
+                                               _u(n), _z(v), _z(out))
+    return float(out.view(np.float64)[0])
+
+
+def project_into_orthogonal_space(n, nproj, nrefl, orth, reflectors, coeffs, swaps, vector) -> np.ndarray:
+    refl = _zin(reflectors, n * nproj) if nproj else np.zeros(1, CPLX)
+    co = _zin(coeffs, nrefl) if nrefl else np.zeros(1, CPLX)
+    sw = np.ascontiguousarray(np.asarray(swaps, np.uint32).reshape(-1)) if nrefl else np.ones(1, np.uint32)
+    v = _zin(vector, n)
+    out = np.zeros(orth if nproj else n, CPLX)
+    with _ErrorWatch():
+        lib().nc_project_into_orthogonal_space(n, nproj, nrefl, orth, _z(refl), _z(co), sw.ctypes.data_as(_u32p),
+                                               _z(v), _z(out))
+    return out
+
+
+def unproject_from_orthogonal_space(n, nproj, nrefl, orth, reflectors, coeffs, swaps, vector) -> np.ndarray:
+    refl = _zin(reflectors, n * nproj) if nproj else np.zeros(1, CPLX)
+    co = _zin(coeffs, nrefl) if nrefl else np.zeros(1, CPLX)
+    sw = np.ascontiguousarray(np.asarray(swaps, np.uint32).reshape(-1)) if nrefl else np.ones(1, np.uint32)
+    v = _zin(vector, orth if nproj else n)
+    out = np.zeros(n, CPLX)
+    with _ErrorWatch():
+        lib().nc_unproject_from_orthogonal_space(n, nproj, nrefl, orth, _z(refl), _z(co), sw.ctypes.data_as(_u32p),
+                                                 _z(v), _z(out))
+    return out
+
+
+def filter_components_outside_orthog(n, nproj, nrefl, orth, reflectors, coeffs, swaps, vector) -> np.ndarray:
+    refl = _zin(reflectors, n * nproj) if nproj else np.zeros(1, CPLX)
+    co = _zin(coeffs, nrefl) if nrefl else np.zeros(1, CPLX)
+    sw = np.ascontiguousarray(np.asarray(swaps, np.uint32).reshape(-1)) if nrefl else np.ones(1, np.uint32)
+    v = _zin(vector, n)
+    out = np.zeros(n, CPLX)
+    with _ErrorWatch():
+        lib().filter_components_outside_orthog_(_u(n), _u(nproj), _u(nrefl), _u(orth), _z(refl), _z(co),
+                                                sw.ctypes.data_as(_u32p), _z(v), _z(out))
+    return out
+
+
+def contract_vs_left(blo, bro, bln, brn, d, L, proj_site, state_site) -> np.ndarray:
+    L = _zin(L, bln * blo)
+    P = _zin(proj_site, blo * d * bro)
+    S = _zin(state_site, brn * bln * d)
+    out = np.zeros((brn, bro), CPLX, order="F")
+    with _ErrorWatch():
+        lib().contract_vs_left_(_u(blo), _u(bro), _u(bln), _u(brn), _u(d), _z(L), _z(P), _z(S), _z(out))
+    return out
+
+
+def contract_vs_right(blo, bro, bln, brn, d, R, proj_site, state_site) -> np.ndarray:
+    R = _zin(R, bro * brn)
+    P = _zin(proj_site, blo * d * bro)
+    S = _zin(state_site, brn * bln * d)
+    out = np.zeros((blo, bln), CPLX, order="F")
+    with _ErrorWatch():
+        lib().contract_vs_right_(_u(blo), _u(bro), _u(bln), _u(brn), _u(d), _z(R), _z(P), _z(S), _z(out))
+    return out

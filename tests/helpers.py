@@ -1,0 +1,54 @@
+"""Shared fixtures for the parity tests: seeded random inputs in the reference's
+layouts, following the generators of tests/python/test.py:175-196."""
+
+import json
+import os
+
+import numpy as np
+
+CPLX = np.complex128
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_goldens.json")
+
+
+def goldens():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def z(pairs):
+    return np.array([complex(a, b) for a, b in pairs], dtype=CPLX)
+
+
+def crand(rng, *shape):
+    """tests/python/test.py:175-177: uniform in [-1,1] + i [-1,1]."""
+    return np.asfortranarray((rng.random(shape) * 2 - 1) + 1j * (rng.random(shape) * 2 - 1))
+
+
+def random_sparse_matrices(rng, cl, cr, d, symmetric=False, nblocks=None):
+    """tests/python/test.py:180-189 -- random (from, to) pairs and d x d blocks."""
+    nb = int(rng.integers(2, cl + cr + 1)) if nblocks is None else nblocks
+    idx = np.array([(int(rng.integers(1, cl + 1)), int(rng.integers(1, cr + 1))) for _ in range(nb)],
+                   dtype=np.uint32).T.copy(order="F")
+    mats = crand(rng, d, d, nb)
+    if symmetric:
+        mats = np.asfortranarray(mats + mats.transpose(1, 0, 2).conj())
+    return idx, mats
+
+
+def hermitian_env(rng, b, c):
+    e = crand(rng, b, b, c)
+    return np.asfortranarray(e + e.transpose(1, 0, 2).conj())
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = max(np.linalg.norm(b.ravel()), 1e-300)
+    return float(np.linalg.norm((a - b).ravel()) / den)
+
+
+def phase_overlap(a, b):
+    """|<a|b>| / (|a||b|): 1 for vectors equal up to a phase."""
+    a = np.asarray(a).ravel(order="F")
+    b = np.asarray(b).ravel(order="F")
+    return abs(np.vdot(a, b)) / (np.linalg.norm(a) * np.linalg.norm(b))
