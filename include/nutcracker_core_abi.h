@@ -155,6 +155,36 @@ void nc_unproject_from_orthogonal_space(uint32_t n, uint32_t nproj, uint32_t nre
                                         const nc_zcomplex* reflectors, const nc_zcomplex* coefficients,
                                         const uint32_t* swaps, const nc_zcomplex* in, nc_zcomplex* out);
 
+/* ---- device-resident sweep driver ------------------------------------------
+   The host loop of the reference's Chain / BaseChain (chain.cpp:147-237,
+   base_chain.cpp:81-123, chain.hpp:248-315) for a fixed bond dimension, with
+   every state site and environment kept in HBM.  Operator sites are passed
+   concatenated: idx_all holds 2*nmat[i] entries per site, mats_all d[i]^2*nmat[i];
+   bonds holds n_sites+1 bond dimensions.  Functions return 0 or an error code
+   (optimizer failures use the reference's info codes, or 101-104 for the
+   post-checks of optimizer.cpp:281-310 and base_chain.cpp:97-99). */
+typedef struct nc_chain nc_chain;
+
+nc_chain* nc_chain_create(uint32_t n_sites, const uint32_t* d, const uint32_t* cl, const uint32_t* cr,
+                          const uint32_t* nmat, const uint32_t* idx_all, const nc_zcomplex* mats_all,
+                          const uint32_t* bonds);
+void nc_chain_destroy(nc_chain* chain);
+int nc_chain_set_options(nc_chain* chain, double site_tol, double sweep_tol, double sanity,
+                         uint32_t max_iterations, const char* which, uint32_t ncv, int rethrow);
+/* raw (unnormalised) site tensor (br, bl, d); nc_chain_reset canonicalises */
+int nc_chain_set_raw_site(nc_chain* chain, uint32_t site, const nc_zcomplex* data);
+int nc_chain_get_site(nc_chain* chain, uint32_t site, nc_zcomplex* out);
+int nc_chain_reset(nc_chain* chain);
+int nc_chain_optimize_site(nc_chain* chain);
+int nc_chain_move(nc_chain* chain, int direction);
+int nc_chain_sweep(nc_chain* chain);
+int nc_chain_sweep_until_converged(nc_chain* chain);
+int nc_chain_energy(nc_chain* chain, double* energy, int* current_site);
+int nc_chain_expectation(nc_chain* chain, nc_zcomplex* out);
+/* counts: matvecs, site solves, moves, sweeps, failures;
+   times_ms: optimize, site move, environment update, reset */
+int nc_chain_stats(nc_chain* chain, int64_t* counts, double* times_ms);
+
 #ifdef __cplusplus
 }
 #endif

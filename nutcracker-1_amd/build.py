@@ -31,7 +31,9 @@ SOURCES = [
     "iram.cpp",
     "optimize.cpp",
     "polar.cpp",
+    "chain.cpp",
     "abi.cpp",
+    "chain_abi.cpp",
 ]
 
 CXXFLAGS = [

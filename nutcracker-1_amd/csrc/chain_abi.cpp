@@ -1,0 +1,155 @@
+// C-ABI of the device-resident sweep driver (no counterpart in the reference's
+// Fortran core: it replaces the host loop of chain.cpp / base_chain.cpp that
+// would otherwise cross the ABI once per contraction).
+#include "nc_chain.hpp"
+#include "../../include/nutcracker_core_abi.h"
+
+#include <cstring>
+
+using namespace nc;
+
+struct nc_chain {
+    std::unique_ptr<Chain> chain;
+};
+
+namespace {
+template <class F>
+int guarded_chain(const char* name, F&& f) {
+    try {
+        return f();
+    } catch (Error const& e) {
+        set_last_error(std::string(name) + ": " + e.what());
+        fprintf(stderr, "[nutcracker-hip] %s: %s\n", name, e.what());
+        return e.code != 0 ? e.code : -1;
+    } catch (std::exception const& e) {
+        set_last_error(std::string(name) + ": " + e.what());
+        fprintf(stderr, "[nutcracker-hip] %s: %s\n", name, e.what());
+        return -1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+nc_chain* nc_chain_create(uint32_t n_sites, const uint32_t* d, const uint32_t* cl, const uint32_t* cr,
+                          const uint32_t* nmat, const uint32_t* idx_all, const nc_zcomplex* mats_all,
+                          const uint32_t* bonds) {
+    nc_chain* out = nullptr;
+    guarded_chain("nc_chain_create", [&] {
+        std::vector<std::unique_ptr<MpoSite>> ops;
+        size_t io = 0, mo = 0;
+        for (uint32_t i = 0; i < n_sites; ++i) {
+            auto op = std::make_unique<MpoSite>();
+            op->build((int)d[i], (int)cl[i], (int)cr[i], (int)nmat[i], idx_all + io,
+                      reinterpret_cast<const zd*>(mats_all) + mo);
+            io += 2 * (size_t)nmat[i];
+            mo += (size_t)d[i] * d[i] * nmat[i];
+            ops.push_back(std::move(op));
+        }
+        for (uint32_t i = 0; i + 1 < n_sites; ++i)
+            if (cr[i] != cl[i + 1]) throw Error("operator bond mismatch between sites", -65);
+        if (cl[0] != 1 || cr[n_sites - 1] != 1) throw Error("the chain's outer operator bonds must be 1", -66);
+        std::vector<int> b(bonds, bonds + n_sites + 1);
+        if (b.front() != 1 || b.back() != 1) throw Error("the chain's outer state bonds must be 1", -67);
+        auto* c = new nc_chain();
+        c->chain = std::make_unique<Chain>((int)n_sites, std::move(ops), std::move(b));
+        out = c;
+        return 0;
+    });
+    return out;
+}
+
+void nc_chain_destroy(nc_chain* c) { delete c; }
+
+int nc_chain_set_options(nc_chain* c, double site_tol, double sweep_tol, double sanity, uint32_t max_iterations,
+                         const char* which, uint32_t ncv, int rethrow) {
+    return guarded_chain("nc_chain_set_options", [&] {
+        auto& o = c->chain->opts;
+        o.site_tol = site_tol;
+        o.sweep_tol = sweep_tol;
+        o.sanity = sanity;
+        o.max_iterations = (int)max_iterations;
+        if (!(which[0] == 'S' && which[1] == 'R') && !(which[0] == 'L' && (which[1] == 'R' || which[1] == 'M')))
+            throw Error("which must be SR, LR or LM", -68);
+        o.which[0] = which[0];
+        o.which[1] = which[1];
+        o.ncv = (int)(ncv < 3 ? 3 : ncv);
+        o.rethrow = rethrow != 0;
+        return 0;
+    });
+}
+
+int nc_chain_set_raw_site(nc_chain* c, uint32_t i, const nc_zcomplex* data) {
+    return guarded_chain("nc_chain_set_raw_site", [&] {
+        if ((int)i >= c->chain->N) throw Error("site index out of range", -69);
+        c->chain->set_raw_site((int)i, reinterpret_cast<const double2*>(data));
+        return 0;
+    });
+}
+
+int nc_chain_get_site(nc_chain* c, uint32_t i, nc_zcomplex* out) {
+    return guarded_chain("nc_chain_get_site", [&] {
+        if ((int)i >= c->chain->N) throw Error("site index out of range", -69);
+        c->chain->get_site((int)i, reinterpret_cast<double2*>(out));
+        return 0;
+    });
+}
+
+int nc_chain_reset(nc_chain* c) {
+    return guarded_chain("nc_chain_reset", [&] { c->chain->reset(); return 0; });
+}
+
+int nc_chain_optimize_site(nc_chain* c) {
+    return guarded_chain("nc_chain_optimize_site", [&] { return c->chain->optimize_site(); });
+}
+
+int nc_chain_move(nc_chain* c, int direction) {
+    return guarded_chain("nc_chain_move", [&] {
+        if (direction > 0) c->chain->move_right();
+        else c->chain->move_left();
+        return 0;
+    });
+}
+
+int nc_chain_sweep(nc_chain* c) {
+    return guarded_chain("nc_chain_sweep", [&] { c->chain->sweep(); return 0; });
+}
+
+int nc_chain_sweep_until_converged(nc_chain* c) {
+    return guarded_chain("nc_chain_sweep_until_converged", [&] { c->chain->sweep_until_converged(); return 0; });
+}
+
+int nc_chain_energy(nc_chain* c, double* energy, int* current_site) {
+    return guarded_chain("nc_chain_energy", [&] {
+        *energy = c->chain->energy;
+        if (current_site) *current_site = c->chain->current;
+        return 0;
+    });
+}
+
+int nc_chain_expectation(nc_chain* c, nc_zcomplex* out) {
+    return guarded_chain("nc_chain_expectation", [&] {
+        zd e = c->chain->site_expectation();
+        out->re = e.real();
+        out->im = e.imag();
+        return 0;
+    });
+}
+
+int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
+    return guarded_chain("nc_chain_stats", [&] {
+        auto const& s = c->chain->stats;
+        counts[0] = s.matvecs;
+        counts[1] = s.site_solves;
+        counts[2] = s.moves;
+        counts[3] = s.sweeps;
+        counts[4] = s.failures;
+        times_ms[0] = s.t_opt_ms;
+        times_ms[1] = s.t_move_ms;
+        times_ms[2] = s.t_env_ms;
+        times_ms[3] = s.t_reset_ms;
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// Device-resident sweep driver (see chain.cpp).
+#pragma once
+
+#include "nc_internal.hpp"
+
+namespace nc {
+
+struct ChainOptions {
+    // chain_options.cpp:21-29 defaults
+    double site_tol = 1e-12;
+    double sweep_tol = 1e-12;
+    double sanity = 1e-12;
+    int max_iterations = 10000;
+    char which[3] = {'S', 'R', 0};
+    int ncv = 3;
+    bool rethrow = false;  // raise on optimizer failures instead of skipping the site
+};
+
+struct ChainStats {
+    int64_t matvecs = 0, site_solves = 0, moves = 0, sweeps = 0, failures = 0;
+    double t_opt_ms = 0, t_move_ms = 0, t_env_ms = 0, t_reset_ms = 0;
+};
+
+struct Chain {
+    int N;
+    std::vector<std::unique_ptr<MpoSite>> ops;
+    std::vector<int> bonds;  // N + 1 bond dimensions
+    std::vector<ZBuf> sites;
+    std::vector<ZBuf> left_envs, right_envs;  // left_envs[i]: sites < i; right_envs[i]: sites >= i
+    ZBuf trivial;
+    Workspace ws;
+    ChainOptions opts;
+    ChainStats stats;
+    int current = 0;
+    bool optimized = false;
+    double energy = 0;
+    int last_status = 0;
+
+    Chain(int n_sites, std::vector<std::unique_ptr<MpoSite>> ops, std::vector<int> bonds);
+    size_t site_size(int i) const;
+    void set_raw_site(int i, const double2* host);
+    void get_site(int i, double2* host) const;
+    void normalize_right_site(int i, hipStream_t s);
+    void reset();
+    zd site_expectation();
+    void move_right();
+    void move_left();
+    int optimize_site();
+    void sweep();
+    void sweep_until_converged();
+};
+
+}  // namespace nc

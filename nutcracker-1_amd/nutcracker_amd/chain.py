@@ -1,0 +1,163 @@
+"""Python handle on the device-resident sweep driver (nc_chain_* in the C-ABI),
+mirroring the methods of the reference's Chain (includes/nutcracker/chain.hpp)."""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import core
+from .fixtures import raw_state
+from .operators import OperatorSite, compute_bandwidth_dimension_sequence, maximum_bandwidth_dimension
+
+CPLX = np.complex128
+
+
+class OptimizerFailure(RuntimeError):
+    pass
+
+
+_CHAIN_SIGS = {
+    "nc_chain_create": (C.c_void_p, [C.c_uint32] + [C.POINTER(C.c_uint32)] * 5 + [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "nc_chain_destroy": (None, [C.c_void_p]),
+    "nc_chain_set_options": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_char_p,
+                                       C.c_uint32, C.c_int]),
+    "nc_chain_set_raw_site": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "nc_chain_get_site": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "nc_chain_reset": (C.c_int, [C.c_void_p]),
+    "nc_chain_optimize_site": (C.c_int, [C.c_void_p]),
+    "nc_chain_move": (C.c_int, [C.c_void_p, C.c_int]),
+    "nc_chain_sweep": (C.c_int, [C.c_void_p]),
+    "nc_chain_sweep_until_converged": (C.c_int, [C.c_void_p]),
+    "nc_chain_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "nc_chain_expectation": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+}
+
+
+def _lib():
+    h = core.lib()
+    if not getattr(h, "_chain_sigs", False):
+        for name, (res, args) in _CHAIN_SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        h._chain_sigs = True
+    return h
+
+
+def _u32(values) -> "C.Array":
+    arr = np.ascontiguousarray(np.asarray(values, dtype=np.uint32))
+    return arr, arr.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Chain:
+    """A chain of operator sites solved at a fixed bond dimension ``chi``.
+
+    Parameters follow ChainOptions (chain_options.cpp:21-29); ``seed`` selects the
+    raw initial state (nutcracker_amd.fixtures), or pass ``raw_sites``."""
+
+    def __init__(self, ops: Sequence[OperatorSite], chi: int, seed: int = 0,
+                 raw_sites: Optional[List[np.ndarray]] = None, site_tol: float = 1e-12,
+                 sweep_tol: float = 1e-12, sanity: float = 1e-12, max_iterations: int = 10000,
+                 which: str = "SR", ncv: int = 3, rethrow: bool = False):
+        self.ops = list(ops)
+        self.n = len(self.ops)
+        dims = [o.d for o in self.ops]
+        chi = min(chi, maximum_bandwidth_dimension(dims))
+        self.bonds = compute_bandwidth_dimension_sequence(chi, dims) if self.n > 1 else [1, 1]
+        lib = _lib()
+        self._keep = []
+        d, dp = _u32([o.d for o in self.ops])
+        cl, clp = _u32([o.cl for o in self.ops])
+        cr, crp = _u32([o.cr for o in self.ops])
+        nm, nmp = _u32([o.nmat for o in self.ops])
+        idx = np.ascontiguousarray(np.concatenate([np.asarray(o.idx, np.uint32).reshape(-1, order="F") for o in self.ops]))
+        mats = np.ascontiguousarray(np.concatenate([np.asarray(o.mats, CPLX).reshape(-1, order="F") for o in self.ops]))
+        bonds, bp = _u32(self.bonds)
+        self._h = lib.nc_chain_create(self.n, dp, clp, crp, nmp, idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      mats.ctypes.data, bp)
+        if not self._h:
+            raise RuntimeError(core.last_error())
+        self._check(lib.nc_chain_set_options(self._h, site_tol, sweep_tol, sanity, max_iterations, which.encode(),
+                                             ncv, 1 if rethrow else 0))
+        if raw_sites is None:
+            raw_sites = raw_state(seed, dims, self.bonds)
+        for i, a in enumerate(raw_sites):
+            a = np.asfortranarray(np.asarray(a, CPLX))
+            if a.shape != (self.bonds[i + 1], self.bonds[i], dims[i]):
+                raise ValueError("raw site %d has shape %s" % (i, a.shape))
+            self._check(lib.nc_chain_set_raw_site(self._h, i, a.ctypes.data))
+        self.reset()
+
+    def _check(self, rc, failure_ok=False):
+        if rc != 0:
+            if failure_ok:
+                return rc
+            msg = core.last_error()
+            if rc > 0 or rc == -14:
+                raise OptimizerFailure("optimizer failure %d: %s" % (rc, msg))
+            raise RuntimeError(msg or ("error %d" % rc))
+        return 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().nc_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- Chain methods -----------------------------------------------------
+    def reset(self):
+        self._check(_lib().nc_chain_reset(self._h))
+
+    def optimize_site(self) -> int:
+        return _lib().nc_chain_optimize_site(self._h)
+
+    def move(self, direction: int):
+        self._check(_lib().nc_chain_move(self._h, 1 if direction > 0 else -1))
+
+    def perform_optimization_sweep(self):
+        self._check(_lib().nc_chain_sweep(self._h))
+
+    def sweep_until_converged(self):
+        self._check(_lib().nc_chain_sweep_until_converged(self._h))
+
+    @property
+    def energy(self) -> float:
+        e = C.c_double(0)
+        cur = C.c_int(0)
+        self._check(_lib().nc_chain_energy(self._h, C.byref(e), C.byref(cur)))
+        return e.value
+
+    @property
+    def current_site(self) -> int:
+        e = C.c_double(0)
+        cur = C.c_int(0)
+        self._check(_lib().nc_chain_energy(self._h, C.byref(e), C.byref(cur)))
+        return cur.value
+
+    def expectation(self) -> complex:
+        out = np.zeros(1, CPLX)
+        self._check(_lib().nc_chain_expectation(self._h, out.ctypes.data))
+        return complex(out[0])
+
+    def site(self, i: int) -> np.ndarray:
+        d = self.ops[i].d
+        out = np.zeros((self.bonds[i + 1], self.bonds[i], d), CPLX, order="F")
+        self._check(_lib().nc_chain_get_site(self._h, i, out.ctypes.data))
+        return out
+
+    def stats(self) -> dict:
+        counts = (C.c_int64 * 5)()
+        times = (C.c_double * 4)()
+        self._check(_lib().nc_chain_stats(self._h, counts, times))
+        return {"matvecs": counts[0], "site_solves": counts[1], "moves": counts[2], "sweeps": counts[3],
+                "failures": counts[4], "t_optimize_ms": times[0], "t_move_ms": times[1], "t_env_ms": times[2],
+                "t_reset_ms": times[3]}

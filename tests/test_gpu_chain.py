@@ -1,0 +1,82 @@
+"""Device-resident sweeps against the reference's known energies and against the
+oracle's sweep driver on the same seeded initial state."""
+
+import numpy as np
+import pytest
+
+from helpers import goldens
+
+from oracle.chain import OracleChain
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from nutcracker_amd import chain, operators, fixtures
+    return chain, operators, fixtures
+
+
+@pytest.mark.parametrize("case", goldens()["tfim_sweep_until_converged"]["cases"])
+def test_tfim_sweep_until_converged(mods, case):
+    """tests/chain.cpp:286-305 (relative 1e-7)."""
+    chain, ops_mod, _ = mods
+    print("case", case, flush=True)
+    ops = ops_mod.transverse_ising_operator(case["sites"], case["coupling"])
+    ch = chain.Chain(ops, case["chi"], seed=7, rethrow=True)
+    ch.sweep_until_converged()
+    assert abs(ch.energy - case["energy"]) <= 1e-7 * abs(case["energy"])
+
+
+@pytest.mark.parametrize("n_chi", goldens()["external_field_sweep"]["cases"])
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_external_field_one_sweep(mods, n_chi, d):
+    """tests/chain.cpp:250-284: one sweep reaches E = -N."""
+    chain, ops_mod, _ = mods
+    n, chis = n_chi
+    m = np.diag([-1.0] + [1.0] * (d - 1)).astype(complex)
+    for chi in chis:
+        ch = chain.Chain(ops_mod.external_field_operator(n, m), chi, seed=3, rethrow=True)
+        ch.perform_optimization_sweep()
+        assert abs(ch.energy + n) <= 1e-7 * n
+
+
+@pytest.mark.parametrize("model", ["tfim", "xxz", "spin1"])
+def test_sweeps_match_oracle(mods, model):
+    """Same MPO, same raw initial state: energies after each sweep agree with the
+    oracle (ARPACK + zgesdd) to 1e-10 relative."""
+    chain, ops_mod, fixtures = mods
+    if model == "tfim":
+        ops, chi = ops_mod.tfim_zz_x_operator(12, 0.5), 16
+    elif model == "xxz":
+        ops, chi = ops_mod.xxz_operator(12, 2, 1.0, 1.0), 16
+    else:
+        ops, chi = ops_mod.xxz_operator(8, 3, 1.0, 1.0), 12
+    ch = chain.Chain(ops, chi, seed=11, rethrow=True)
+    dims = [o.d for o in ops]
+    raw = fixtures.raw_state(11, dims, ch.bonds)
+    oc_ = OracleChain(ops, ch.bonds, raw, rethrow=True)
+    assert abs(ch.energy - oc_.energy) <= 1e-12 * abs(oc_.energy)
+    for sweep in range(3):
+        ch.perform_optimization_sweep()
+        oc_.sweep()
+        print(model, sweep, ch.energy, oc_.energy, flush=True)
+        assert abs(ch.energy - oc_.energy) <= 1e-10 * abs(oc_.energy)
+    st = ch.stats()
+    assert st["failures"] == 0 and oc_.failures == 0
+
+
+def test_state_invariants_along_moves(mods):
+    """tests/chain.cpp:143-169: the state norm is 1 and <H> is real and equal to
+    the chain energy after every move."""
+    chain, ops_mod, _ = mods
+    ops = ops_mod.xxz_operator(10, 2, 1.0, 0.8)
+    ch = chain.Chain(ops, 8, seed=5, rethrow=True)
+    e0 = ch.energy
+    for direction in [1] * 9 + [-1] * 9:
+        ch.move(direction)
+        ex = ch.expectation()
+        assert abs(ex.imag) <= 1e-10 * abs(ex)
+        assert abs(ex.real - e0) <= 1e-10 * abs(e0)
+        psi = ch.site(ch.current_site)
+        assert abs(np.linalg.norm(psi) - 1) < 1e-12
