@@ -176,7 +176,7 @@ int Chain::optimize_site() {
     none.orth = (int)site_size(i);
     OptimizeArgs a{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, &none,
                    opts.which, opts.site_tol, opts.max_iterations, sites[i].ptr, result.ptr, opts.ncv};
-    OptimizeOut o = optimize_site(a, ws, s);
+    OptimizeOut o = nc::optimize_site(a, ws, s);
     stats.matvecs += o.matvecs;
     ++stats.site_solves;
     int status = o.info;
