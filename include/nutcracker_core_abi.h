@@ -163,6 +163,15 @@ void nc_unproject_from_orthogonal_space(uint32_t n, uint32_t nproj, uint32_t nre
    bonds holds n_sites+1 bond dimensions.  Functions return 0 or an error code
    (optimizer failures use the reference's info codes, or 101-104 for the
    post-checks of optimizer.cpp:281-310 and base_chain.cpp:97-99). */
+/* Timing of `reps` H_eff matvecs (after `warmup` untimed ones) with every operand
+   resident in HBM; y receives the last result.  total_ms from HIP events on the
+   library's stream; phase_ms (optional, 3 entries) the average durations of the
+   step-A GEMM launch, the MPO mix kernel and the step-C GEMM launch. */
+int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                  const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L, const nc_zcomplex* R,
+                  const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
+                  double* phase_ms);
+
 typedef struct nc_chain nc_chain;
 
 nc_chain* nc_chain_create(uint32_t n_sites, const uint32_t* d, const uint32_t* cl, const uint32_t* cr,

@@ -34,6 +34,7 @@ SOURCES = [
     "chain.cpp",
     "abi.cpp",
     "chain_abi.cpp",
+    "bench_abi.cpp",
 ]
 
 CXXFLAGS = [

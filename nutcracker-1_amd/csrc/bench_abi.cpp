@@ -1,0 +1,81 @@
+// Timing entry point for the H_eff matvec with every operand resident in HBM.
+// Timing uses HIP events recorded on the library's stream (the stream every
+// kernel of the matvec is launched on).
+#include "nc_internal.hpp"
+#include "../../include/nutcracker_core_abi.h"
+
+using namespace nc;
+
+extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                             const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
+                             const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
+                             uint32_t reps, double* total_ms, double* phase_ms) {
+    try {
+        auto& c = Context::get();
+        hipStream_t s = c.stream;
+        MpoSite op;
+        op.build((int)d, (int)cl, (int)cr, (int)nmat, idx, reinterpret_cast<const zd*>(mats));
+        const size_t nL = (size_t)bl * bl * cl, nR = (size_t)br * br * cr, nx = (size_t)br * bl * d;
+        ZBuf dL(nL), dR(nR), dx(nx), dy(nx);
+        h2d(dL.ptr, L, nL * sizeof(double2));
+        h2d(dR.ptr, R, nR * sizeof(double2));
+        h2d(dx.ptr, x, nx * sizeof(double2));
+        Workspace ws;
+        for (uint32_t i = 0; i < warmup; ++i) heff_matvec((int)bl, (int)br, op, dL.ptr, dR.ptr, dx.ptr, dy.ptr, ws, s);
+        NC_HIP(hipStreamSynchronize(s));
+        hipEvent_t e0, e1;
+        NC_HIP(hipEventCreate(&e0));
+        NC_HIP(hipEventCreate(&e1));
+        NC_HIP(hipEventRecord(e0, s));
+        for (uint32_t i = 0; i < reps; ++i) heff_matvec((int)bl, (int)br, op, dL.ptr, dR.ptr, dx.ptr, dy.ptr, ws, s);
+        NC_HIP(hipEventRecord(e1, s));
+        NC_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *total_ms = ms;
+        if (phase_ms) {
+            // per-phase averages: the two GEMM launches (step A, step C) and the mix kernel
+            phase_ms[0] = phase_ms[1] = phase_ms[2] = 0;
+            hipEvent_t ev[4];
+            for (auto& e : ev) NC_HIP(hipEventCreate(&e));
+            const uint32_t preps = reps < 3 ? reps : 3;
+            const int64_t rs = (int64_t)br * bl;
+            ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
+            for (uint32_t i = 0; i < preps; ++i) {
+                GemmPlan pa;
+                pa.M = (int)br; pa.N = (int)bl;
+                for (size_t fi = 0; fi < op.froms.size(); ++fi)
+                    for (uint32_t a = 0; a < d; ++a) {
+                        pa.begin_batch(ws.P.ptr + ((int64_t)fi * d + a) * rs, br);
+                        pa.add_seg(dx.ptr + a * rs, br, dL.ptr + (int64_t)op.froms[fi] * bl * bl, bl, (int)bl);
+                    }
+                GemmPlan pc;
+                pc.M = (int)br; pc.N = (int)(bl * d);
+                pc.begin_batch(dy.ptr, br);
+                for (size_t ti = 0; ti < op.tos.size(); ++ti)
+                    pc.add_seg(dR.ptr + (int64_t)op.tos[ti] * br * br, br, ws.Q.ptr + (int64_t)ti * d * rs, br, (int)br);
+                NC_HIP(hipEventRecord(ev[0], s));
+                zgemm_run(pa, s);
+                NC_HIP(hipEventRecord(ev[1], s));
+                mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
+                NC_HIP(hipEventRecord(ev[2], s));
+                zgemm_run(pc, s);
+                NC_HIP(hipEventRecord(ev[3], s));
+                NC_HIP(hipEventSynchronize(ev[3]));
+                float t;
+                NC_HIP(hipEventElapsedTime(&t, ev[0], ev[1])); phase_ms[0] += t / preps;
+                NC_HIP(hipEventElapsedTime(&t, ev[1], ev[2])); phase_ms[1] += t / preps;
+                NC_HIP(hipEventElapsedTime(&t, ev[2], ev[3])); phase_ms[2] += t / preps;
+            }
+            for (auto& e : ev) NC_HIP(hipEventDestroy(e));
+        }
+        NC_HIP(hipEventDestroy(e0));
+        NC_HIP(hipEventDestroy(e1));
+        d2h(y, dy.ptr, nx * sizeof(double2));
+        return 0;
+    } catch (std::exception const& e) {
+        set_last_error(std::string("nc_bench_heff: ") + e.what());
+        fprintf(stderr, "[nutcracker-hip] nc_bench_heff: %s\n", e.what());
+        return -1;
+    }
+}
