@@ -2,20 +2,12 @@
 
 * ``core``: the reference's Nutcracker::Core entry points over the C-ABI of
   libnutcracker_hip.so (HIP kernels for gfx950).
+* ``chain``: the device-resident sweep driver.
 * ``operators``: MPO sites in the reference's from/to block format and the
   bond-dimension helpers of utilities.cpp.
+* ``fixtures``: seeded raw initial states.
 """
 
-from . import operators  # noqa: F401
+from . import fixtures, operators  # noqa: F401
 
-__all__ = ["core", "operators"]
-
-
-def __getattr__(name):
-    if name == "core":
-        from . import core
-        return core
-    if name == "chain":
-        from . import chain
-        return chain
-    raise AttributeError(name)
+__all__ = ["core", "chain", "fixtures", "operators"]
