@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(const GemmBatch* __restr
         }
     }
 
-    // ---- store: lane holds column (lane & 15), rows 4 * (lane >> 4) + v
+    // ---- store: for the f64 16x16x4 MFMA a lane holds column (lane & 15) and
+    // the strided rows (lane >> 4) + 4 v, v = 0..3
     double2* __restrict__ C = bt.C;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(const GemmBatch* __restr
             if (col >= N) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm + i * 16 + 4 * lk + v;
+                const int row = m0 + wm + i * 16 + lk + 4 * v;
                 if (row < M) {
                     double2* p = C + (int64_t)row + (int64_t)col * bt.ldc;
                     double2 o = make_double2(acc_re[i][j][v], acc_im[i][j][v]);

@@ -9,6 +9,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cmath>
 #include <mutex>
 
 namespace nc {
@@ -37,6 +38,10 @@ int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pco
     if (m <= 0 || ncol < m) throw Error("polar_split: need 0 < m <= n", -51);
     rocblas_handle h = solver_handle();
     if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error("rocblas_set_stream failed", -52);
+    // the SVD is only given finite input: a non-finite matrix is reported as a
+    // failed normalisation (NormalizationError) instead
+    const double anorm = dznrm2(A, (int64_t)m * ncol, s);
+    if (!std::isfinite(anorm)) return 1000;
     ZBuf Aw((size_t)m * ncol), U((size_t)m * m), Vt((size_t)m * ncol), Us((size_t)m * m);
     DevBuf<double> S((size_t)m), E((size_t)std::max(1, m));
     DevBuf<int> info(1);
