@@ -117,7 +117,7 @@ def test_site_move_right(core, dims):
     assert rel(after, before) < 1e-12
 
 
-@pytest.mark.parametrize("dims", [(2, 2, 1, 2, 2), (3, 4, 2, 2, 2), (8, 16, 4, 2, 2), (40, 32, 16, 2, 2),
+@pytest.mark.parametrize("dims", [(2, 2, 1, 2, 2), (3, 4, 2, 2, 2), (8, 16, 8, 2, 2), (40, 32, 16, 2, 2),
                                   (130, 128, 64, 3, 2)])
 def test_site_move_left(core, dims):
     """norm_denorm_going_left (C-side names bll, bl, br, dl, d)."""
@@ -188,3 +188,12 @@ def test_optimize_with_projectors(core):
             info2, *_ = core.optimize(bl, br, cl, cr, d, L, idx, mats, R, k, rank, orth, refl, tau, swaps,
                                       "SR", 1e-12, 10000, bad)
             assert info2 == 11
+
+
+def test_site_move_rejects_too_few_degrees_of_freedom(core):
+    """core.f95:2740-2744 / 2823-2827 stop the process; the C-ABI reports an error instead."""
+    rng = np.random.default_rng(0)
+    with pytest.raises(RuntimeError, match="degrees of freedom"):
+        core.norm_denorm_going_left(8, 16, 4, 2, 2, crand(rng, 16, 8, 2), crand(rng, 4, 16, 2))
+    with pytest.raises(RuntimeError, match="degrees of freedom"):
+        core.norm_denorm_going_right(2, 16, 4, 2, 2, crand(rng, 16, 2, 2), crand(rng, 4, 16, 2))
