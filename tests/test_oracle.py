@@ -1,0 +1,306 @@
+"""Pins the oracle (CPU) before it is trusted as the checker: reference golden
+vectors, known answers, closed forms, invariants and brute force on tiny inputs."""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import crand, goldens, hermitian_env, random_sparse_matrices, rel, z
+
+import oracle.core as oc
+from oracle.chain import OracleChain
+
+
+def _idx(e):
+    return np.array(e["idx"], dtype=np.uint32).reshape(2, -1, order="F")
+
+
+# ---------------------------------------------------------------------------
+# Exact goldens (tests/boundaries.cpp)
+# ---------------------------------------------------------------------------
+def test_contract_sos_goldens():
+    g = goldens()
+    e = g["contract_sos_left_non_trivial"]
+    D = e["dims"]
+    out = oc.contract_sos_left(D["bl"], D["br"], D["cl"], D["cr"], D["d"], z(e["L"]), _idx(e), z(e["mats"]), z(e["psi"]))
+    assert np.array_equal(out.ravel(order="F"), z(e["expected"]))
+    e = g["contract_sos_right_non_trivial"]
+    D = e["dims"]
+    out = oc.contract_sos_right(D["bl"], D["br"], D["cl"], D["cr"], D["d"], z(e["R"]), _idx(e), z(e["mats"]), z(e["psi"]))
+    assert np.array_equal(out.ravel(order="F"), z(e["expected"]))
+    for key, fn in [("contract_sos_left_trivial_operator_dimension_2", oc.contract_sos_left),
+                    ("contract_sos_right_trivial_operator_dimension_2", oc.contract_sos_right)]:
+        e = g[key]
+        D = e["dims"]
+        env = z(e["L"] if "L" in e else e["R"])
+        out = fn(D["bl"], D["br"], D["cl"], D["cr"], D["d"], env, _idx(e), z(e["mats"]), z(e["psi"]))
+        assert np.array_equal(out.ravel(order="F"), z(e["expected"]))
+
+
+def test_expectation_goldens():
+    g = goldens()
+    e = g["contract_expectation_boundaries_non_trivial"]
+    assert oc.contract_expectation_boundaries(3, 2, z(e["L"]), z(e["R"])) == complex(*e["expected"])
+    for cse in g["compute_expectation_trivial"]["cases"]:
+        psi = z(cse["psi"])
+        v = oc.compute_expectation(1, 1, 1, 1, len(psi), np.ones(1), psi, np.array(cse["idx"]).reshape(2, 1),
+                                   z(cse["mats"]), np.ones(1))
+        assert v == complex(*cse["expected"])
+
+
+def test_contract_vs_goldens():
+    g = goldens()
+    e = g["contract_vs_left_non_trivial"]
+    D = e["dims"]
+    out = oc.contract_vs_left(D["blo"], D["bro"], D["bln"], D["brn"], D["d"], z(e["L"]), z(e["proj"]), z(e["psi"]))
+    assert np.array_equal(out.ravel(order="F"), z(e["expected"]))
+    e = g["contract_vs_right_non_trivial"]
+    D = e["dims"]
+    out = oc.contract_vs_right(D["blo"], D["bro"], D["bln"], D["brn"], D["d"], z(e["R"]), z(e["proj"]), z(e["psi"]))
+    assert np.array_equal(out.ravel(order="F"), z(e["expected"]))
+
+
+# ---------------------------------------------------------------------------
+# Brute force on tiny inputs: explicit loops over the defining sums
+# ---------------------------------------------------------------------------
+def _brute_matvec(bl, br, d, L, R, idx, mats, x):
+    y = np.zeros((br, bl, d), complex)
+    for n in range(idx.shape[1]):
+        f, t = int(idx[0, n]) - 1, int(idx[1, n]) - 1
+        for rp, j, b, r, i, a in itertools.product(range(br), range(bl), range(d), range(br), range(bl), range(d)):
+            y[rp, j, b] += R[rp, r, t] * L[i, j, f] * mats[a, b, n] * x[r, i, a]
+    return y
+
+
+def test_matvec_and_environments_brute_force():
+    rng = np.random.default_rng(0)
+    d, bl, br, cl, cr = 2, 3, 2, 2, 3
+    L = crand(rng, bl, bl, cl)
+    R = crand(rng, br, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d)
+    x = crand(rng, br, bl, d)
+    y = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x)
+    assert rel(y, _brute_matvec(bl, br, d, L, R, idx, mats, x)) < 1e-14
+    M = oc.compute_optimization_matrix(bl, br, cl, cr, d, L, idx, mats, R)
+    assert rel(M @ x.ravel(order="F"), y.ravel(order="F")) < 1e-14
+    # left environment: N(r,r',k2) = sum L(i,j,f) W(a,b) psi(r,i,a) conj psi(r',j,b)
+    N = np.zeros((br, br, cr), complex)
+    for n in range(idx.shape[1]):
+        f, t = int(idx[0, n]) - 1, int(idx[1, n]) - 1
+        N[:, :, t] += np.einsum("ij,ab,ria,sjb->rs", L[:, :, f], mats[:, :, n], x, x.conj())
+    assert rel(oc.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, x), N) < 1e-14
+    # right environment: N(j',j,k1) = sum R(r',r,t) W(a,b) psi(r,j,a) conj psi(r',j',b)
+    N = np.zeros((bl, bl, cl), complex)
+    for n in range(idx.shape[1]):
+        f, t = int(idx[0, n]) - 1, int(idx[1, n]) - 1
+        N[:, :, f] += np.einsum("sr,ab,rja,skb->kj", R[:, :, t], mats[:, :, n], x, x.conj())
+    assert rel(oc.contract_sos_right(bl, br, cl, cr, d, R, idx, mats, x), N) < 1e-14
+
+
+def test_environment_consistency_invariant():
+    """tests/boundaries.cpp:468-548."""
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        d, bl, br, cl, cr = (int(rng.integers(1, 5)), int(rng.integers(1, 9)), int(rng.integers(1, 9)),
+                             int(rng.integers(1, 5)), int(rng.integers(1, 5)))
+        L = crand(rng, bl, bl, cl)
+        R = crand(rng, br, br, cr)
+        idx, mats = random_sparse_matrices(rng, cl, cr, d)
+        psi = crand(rng, br, bl, d)
+        e = oc.compute_expectation(bl, br, cl, cr, d, L, psi, idx, mats, R)
+        a = oc.contract_expectation_boundaries(br, cr, oc.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, psi), R)
+        b = oc.contract_expectation_boundaries(bl, cl, L, oc.contract_sos_right(bl, br, cl, cr, d, R, idx, mats, psi))
+        assert abs(a - e) <= 1e-10 * abs(e) and abs(b - e) <= 1e-10 * abs(e)
+
+
+# ---------------------------------------------------------------------------
+# Optimizer: KATs and eigh
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", goldens()["optimizer_modes"]["cases"])
+def test_optimizer_modes_kat(case):
+    rng = np.random.default_rng(2)
+    mats = np.diag(np.array(case["diag"], complex)).reshape(4, 4, 1, order="F")
+    info, it, res, ev, nrm = oc.optimize(1, 1, 1, 1, 4, np.ones(1), np.array([[1], [1]]), mats, np.ones(1), 0, 0, 4,
+                                         None, None, None, case["which"], 0.0, 10000, crand(rng, 4))
+    assert info == 0 and it == 0 and abs(ev - case["eigenvalue"]) < 1e-12
+    assert np.all(np.abs(res.ravel()[:3]) < 1e-15)
+
+
+@pytest.mark.parametrize("shape", [(2, 6, 7, 2, 3), (3, 5, 5, 3, 2), (2, 2, 3, 3, 3), (4, 3, 3, 2, 2)])
+def test_optimizer_against_eigh(shape):
+    d, bl, br, cl, cr = shape
+    rng = np.random.default_rng(sum(shape))
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    n = d * bl * br
+    st = {}
+    info, it, res, ev, nrm = oc.optimize(bl, br, cl, cr, d, L, idx, mats, R, 0, 0, n, None, None, None, "SR", 1e-12,
+                                         10000, crand(rng, br, bl, d), stats=st)
+    M = oc.compute_optimization_matrix(bl, br, cl, cr, d, L, idx, mats, R)
+    w = np.linalg.eigvalsh(M)
+    assert info == 0 and abs(ev.real - w[0]) <= 1e-10 * max(1, abs(w[0]))
+    assert abs(nrm - 1) < 1e-12
+    rq = np.vdot(res.ravel(order="F"), M @ res.ravel(order="F")).real
+    assert abs(rq - w[0]) <= 1e-10 * max(1, abs(w[0]))
+
+
+def test_optimizer_maxiter_forces_minus_14():
+    """maxiter = 1 forces info -14 on a non-trivial site (strategy 3)."""
+    rng = np.random.default_rng(4)
+    d, bl, br, cl, cr = 2, 8, 8, 2, 2
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    info, it, res, ev, nrm = oc.optimize(bl, br, cl, cr, d, L, idx, mats, R, 0, 0, d * bl * br, None, None, None,
+                                         "SR", 1e-14, 1, crand(rng, br, bl, d))
+    assert info == -14
+
+
+def test_projector_convention_and_optimizer_kat():
+    """tests/python/test.py:978-1126: diag(4,3,2,1) with e_4 projected out gives 2 and e_3."""
+    mats = np.diag([4, 3, 2, 1]).astype(complex).reshape(4, 4, 1, order="F")
+    for proj in (np.array([0, 0, 0, 1], complex), np.array([0, 0, 0, 1j], complex)):
+        rank, refl, tau, swaps = oc.convert_vectors_to_reflectors(4, 1, proj.reshape(4, 1))
+        assert rank == 1
+        rng = np.random.default_rng(5)
+        guess = oc.unproject_from_orthogonal_space(4, 1, rank, 3, refl, tau, swaps, crand(rng, 3))
+        guess /= np.linalg.norm(guess)
+        info, it, res, ev, nrm = oc.optimize(1, 1, 1, 1, 4, np.ones(1), np.array([[1], [1]]), mats, np.ones(1), 1,
+                                             rank, 3, refl, tau, swaps, "SR", 0.0, 10000, guess)
+        r = res.ravel()
+        assert info == 0 and abs(ev - 2) < 1e-12
+        assert abs(np.dot(proj, r)) < 1e-12
+        assert np.allclose(r / r[2], [0, 0, 1, 0])
+    mats = np.diag([5, 4, 3, 2, 1]).astype(complex).reshape(5, 5, 1, order="F")
+    P = np.array([[0, 0, 0, 0, 1], [0, 0, 0, 1, 0]], complex).T
+    rank, refl, tau, swaps = oc.convert_vectors_to_reflectors(5, 2, P)
+    assert rank == 2
+    guess = oc.unproject_from_orthogonal_space(5, 2, rank, 3, refl, tau, swaps, crand(np.random.default_rng(6), 3))
+    info, it, res, ev, nrm = oc.optimize(1, 1, 1, 1, 5, np.ones(1), np.array([[1], [1]]), mats, np.ones(1), 2, rank, 3,
+                                         refl, tau, swaps, "SR", 0.0, 10000, guess / np.linalg.norm(guess))
+    assert info == 0 and abs(ev - 3) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Site move: polar uniqueness and state preservation
+# ---------------------------------------------------------------------------
+def test_polar_site_move():
+    rng = np.random.default_rng(7)
+    bl, bm, br, dl, dr = 3, 5, 4, 2, 2
+    tn = crand(rng, bm, bl, dl)
+    td = crand(rng, br, bm, dr)
+    info, on, od = oc.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    A = tn.reshape(bm, bl * dl, order="F")
+    Up = on.reshape(bm, bl * dl, order="F")
+    assert info == 0
+    assert np.abs(Up @ Up.conj().T - np.eye(bm)).max() < 1e-13
+    P = A @ Up.conj().T                           # A = P U_p with P Hermitian PSD
+    assert np.abs(P - P.conj().T).max() < 1e-12 and np.linalg.eigvalsh(P).min() > 0
+    assert rel(P @ Up, A) < 1e-13
+    before = np.einsum("mla,rmb->labr", tn, td)
+    assert rel(np.einsum("mla,rmb->labr", on, od), before) < 1e-13
+    info, od2, on2 = oc.norm_denorm_going_left(bl, bm, br, dl, dr, tn, td)
+    assert info == 0 and rel(np.einsum("mla,rmb->labr", od2, on2), before) < 1e-13
+    B = np.transpose(on2, (1, 0, 2)).reshape(bm, br * dr, order="F")
+    assert np.abs(B @ B.conj().T - np.eye(bm)).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# End-to-end: sweep driver against the reference's energies and closed forms
+# ---------------------------------------------------------------------------
+def _tfim_free_fermion_levels(n, coupling, levels):
+    """Exact spectrum of H = sum Z_i + c sum X_i X_{i+1} (open chain) by the
+    Jordan-Wigner / Bogoliubov closed form; checked against exact diagonalisation below."""
+    # H = -c sum -X X ... map to the standard form -J sum s_x s_x - h sum s_z with h = -1, J = -c
+    h, J = -1.0, -coupling
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 2 * h
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -J
+        B[i, i + 1] = -J
+        B[i + 1, i] = J
+    lam = np.sqrt(np.clip(np.linalg.eigvalsh((A - B) @ (A + B)), 0, None))
+    e0 = 0.5 * (np.trace(A) - lam.sum()) - n * h
+    es = [e0 + sum(lam[list(s)]) for r in range(n + 1) for s in itertools.combinations(range(n), r)]
+    return sorted(es)[:levels]
+
+
+def _exact_levels(ops, levels):
+    """Dense exact diagonalisation of an MPO chain (small N only)."""
+    T = np.ones((1, 1, 1), complex)  # (state, state', operator bond)
+    for op in ops:
+        W = np.zeros((op.cl, op.cr, op.d, op.d), complex)
+        for n in range(op.nmat):
+            W[int(op.idx[0, n]) - 1, int(op.idx[1, n]) - 1] += op.mats[:, :, n].T  # M(b,a) back to row-major
+        T = np.einsum("xyk,klab->xaybl", T, W).reshape(T.shape[0] * op.d, T.shape[1] * op.d, op.cr)
+    H = T[:, :, 0]
+    return np.sort(np.linalg.eigvalsh(H))[:levels]
+
+
+def test_tfim_closed_form_against_ed():
+    from nutcracker_amd.operators import transverse_ising_operator
+    for n, c in [(4, 0.3), (6, 1.0), (7, 2.0)]:
+        ed = _exact_levels(transverse_ising_operator(n, c), 6)
+        ff = _tfim_free_fermion_levels(n, c, 6)
+        assert np.allclose(ed, ff, atol=1e-10)
+
+
+def test_tfim_golden_levels_closed_form():
+    """tests/stress.cpp:107-131 and chain.cpp:390-429 reproduce the closed form."""
+    g = goldens()
+    for cse in g["tfim_four_levels"]["cases"]:
+        ff = _tfim_free_fermion_levels(cse["sites"], cse["coupling"], 4)
+        assert np.allclose(ff, cse["levels"], atol=1e-7)
+    for cse in g["tfim_optimize_chain"]["cases"]:
+        ff = _tfim_free_fermion_levels(cse["sites"], cse["coupling"], 1)
+        assert abs(abs(ff[0]) - cse["abs_energy"]) <= 1e-10 * cse["abs_energy"]
+
+
+@pytest.mark.parametrize("case", goldens()["tfim_sweep_until_converged"]["cases"])
+def test_oracle_chain_tfim(case):
+    from nutcracker_amd.fixtures import raw_state
+    from nutcracker_amd.operators import (compute_bandwidth_dimension_sequence, maximum_bandwidth_dimension,
+                                          transverse_ising_operator)
+    n = case["sites"]
+    ops = transverse_ising_operator(n, case["coupling"])
+    dims = [2] * n
+    bonds = compute_bandwidth_dimension_sequence(min(case["chi"], maximum_bandwidth_dimension(dims)), dims)
+    ch = OracleChain(ops, bonds, raw_state(7, dims, bonds), rethrow=True)
+    ch.sweep_until_converged()
+    assert abs(ch.energy - case["energy"]) <= 1e-7 * abs(case["energy"])
+
+
+def test_oracle_chain_external_field():
+    from nutcracker_amd.fixtures import raw_state
+    from nutcracker_amd.operators import compute_bandwidth_dimension_sequence, external_field_operator
+    for n, chis in goldens()["external_field_sweep"]["cases"]:
+        for chi in chis:
+            dims = [2] * n
+            bonds = compute_bandwidth_dimension_sequence(min(chi, 2 ** ((n + 1) // 2)), dims) if n > 1 else [1, 1]
+            ch = OracleChain(external_field_operator(n, np.diag([-1.0, 1.0]).astype(complex)), bonds,
+                             raw_state(3, dims, bonds), rethrow=True)
+            ch.sweep()
+            assert abs(ch.energy + n) <= 1e-7 * n
+
+
+def test_config1_exact_levels_closed_form():
+    """Config 1 (H = -sum ZZ - 0.5 sum X, N = 20): the closed form gives the four
+    lowest levels used for the excited-state check."""
+    # the unitary X <-> Z maps -ZZ - h X onto -XX - h Z, i.e. the form above with h' = h
+    n, hx = 20, 0.5
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 2 * hx
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -1.0
+        B[i, i + 1] = -1.0
+        B[i + 1, i] = 1.0
+    lam = np.sort(np.sqrt(np.clip(np.linalg.eigvalsh((A - B) @ (A + B)), 0, None)))
+    e0 = 0.5 * (np.trace(A) - lam.sum()) - n * hx
+    levels = sorted([e0, e0 + lam[0], e0 + lam[1], e0 + lam[0] + lam[1], e0 + lam[2]])[:4]
+    assert np.allclose(levels, [-20.400217867027, -20.400216436515, -19.373376103501, -19.373374672990], atol=1e-9)
