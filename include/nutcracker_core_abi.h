@@ -172,6 +172,10 @@ int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d
                   const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
                   double* phase_ms);
 
+/* Site-move statistics since load: out[0] polar splits, out[1] Newton-Schulz
+   iterations, out[2] splits that fell back to the SVD (rank-deficient input). */
+int nc_polar_stats(int64_t* out);
+
 typedef struct nc_chain nc_chain;
 
 nc_chain* nc_chain_create(uint32_t n_sites, const uint32_t* d, const uint32_t* cl, const uint32_t* cr,

@@ -79,3 +79,12 @@ extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr,
         return -1;
     }
 }
+
+// Site-move statistics: polar splits, Newton-Schulz iterations, SVD fallbacks.
+extern "C" int nc_polar_stats(int64_t* out) {
+    nc::PolarStats p = nc::polar_stats();
+    out[0] = p.calls;
+    out[1] = p.iterations;
+    out[2] = p.svd_fallbacks;
+    return 0;
+}

@@ -285,6 +285,12 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
 // Returns the SVD info code (0 on success).  ``Pconj`` receives conj(P) (m x m).
 // ----------------------------------------------------------------------------
 int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s);
+// the same through rocSOLVER's SVD (used when the iteration cannot converge)
+int polar_split_svd(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s);
+struct PolarStats {
+    int64_t calls = 0, iterations = 0, svd_fallbacks = 0;
+};
+PolarStats polar_stats();
 
 // norm_denorm_going_left / right on device tensors (core.f95:2708-2878).
 int norm_denorm_left(int bl, int bm, int br, int dl, int dr, const double2* to_denorm,

@@ -34,14 +34,106 @@ __global__ void scale_cols_kernel(double2* X, const double* sv, int64_t m, int64
     }
 }
 
+// out(i,j) = conj of the Hermitian part of P: 0.5 (conj(P(i,j)) + P(j,i)); one
+// thread per unordered pair, so the update is in place
+__global__ void conj_hermitian_part_kernel(double2* P, int64_t m) {
+    const int64_t total = m * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % m, j = e / m;
+        if (i > j) continue;
+        const double2 a = P[i + j * m], b = P[j + i * m];
+        const double2 nij = make_double2(0.5 * (a.x + b.x), 0.5 * (b.y - a.y));
+        const double2 nji = make_double2(0.5 * (b.x + a.x), 0.5 * (a.y - b.y));
+        P[i + j * m] = nij;
+        P[j + i * m] = nji;
+    }
+}
+
+namespace {
+PolarStats g_polar_stats;
+
+// Scaled Newton-Schulz iteration for the unitary polar factor of a wide
+// matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X,  with the
+// equioscillation scaling alpha = sqrt(3 / (1 + l + l^2)) for an assumed lower
+// bound l of the smallest singular value (X is first divided by its Frobenius
+// norm, so every singular value lies in (0, 1]).  The scaled cubic maps
+// [l, 1] onto [p(l), 1] and grows singular values below l by at least 3/2 per
+// step, so it converges for any full-rank input; the Gram matrix of each step
+// gives ||X X^H - I||_F for free.  Only GEMMs of this library are used.
+bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStream_t s, int* iters_out) {
+    const double nrm = dznrm2(A, (int64_t)m * n, s);
+    if (!(nrm > 0) || !std::isfinite(nrm)) return false;
+    ZBuf X0((size_t)m * n), X1((size_t)m * n), G((size_t)m * m);
+    d2d(X0.ptr, A, (size_t)m * n * sizeof(double2));
+    zscal(X0.ptr, zd(1.0 / nrm, 0), (int64_t)m * n, s);
+    double l = 1e-3;
+    const int max_iters = 100;
+    bool last = false;
+    for (int it = 0; it < max_iters; ++it) {
+        GemmPlan g;  // G = X X^H
+        g.M = m; g.N = m; g.opA = OpN; g.opB = OpC;
+        g.begin_batch(G.ptr, m);
+        g.add_seg(X0.ptr, m, X0.ptr, m, (int)n);
+        zgemm_run(g, s);
+        add_identity(G.ptr, m, zd(-1, 0), s);              // E = X X^H - I
+        const double err = dznrm2(G.ptr, (int64_t)m * m, s);
+        if (!std::isfinite(err)) return false;
+        if (last || err < 1e-15 * m) {
+            d2d(Up, X0.ptr, (size_t)m * n * sizeof(double2));
+            *iters_out = it;
+            return true;
+        }
+        double alpha = 1.0;
+        if (l < 1.0 - 1e-3) alpha = std::sqrt(3.0 / (1.0 + l + l * l));
+        // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
+        zscal(G.ptr, zd(-0.5 * alpha * alpha * alpha, 0), (int64_t)m * m, s);
+        add_identity(G.ptr, m, zd(0.5 * alpha * (3.0 - alpha * alpha), 0), s);
+        GemmPlan u;  // X1 = H X0
+        u.M = m; u.N = (int)n; u.opA = OpN; u.opB = OpN;
+        u.begin_batch(X1.ptr, m);
+        u.add_seg(G.ptr, m, X0.ptr, m, m);
+        zgemm_run(u, s);
+        std::swap(X0.ptr, X1.ptr);
+        l = std::min(1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l));
+        // cubic convergence: once every |sigma^2 - 1| < 1e-7 one more step reaches round-off
+        if (err < 1e-7) last = true;
+    }
+    return false;
+}
+}  // namespace
+
+PolarStats polar_stats() { return g_polar_stats; }
+
 int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s) {
     if (m <= 0 || ncol < m) throw Error("polar_split: need 0 < m <= n", -51);
-    rocblas_handle h = solver_handle();
-    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error("rocblas_set_stream failed", -52);
     // the SVD is only given finite input: a non-finite matrix is reported as a
     // failed normalisation (NormalizationError) instead
     const double anorm = dznrm2(A, (int64_t)m * ncol, s);
     if (!std::isfinite(anorm)) return 1000;
+    ++g_polar_stats.calls;
+    int iters = 0;
+    if (polar_newton_schulz(m, ncol, A, Up, s, &iters)) {
+        g_polar_stats.iterations += iters;
+        // P = A U_p^H is Hermitian positive semi-definite; keep its Hermitian part
+        GemmPlan p;
+        p.M = m; p.N = m; p.opA = OpN; p.opB = OpC;
+        p.begin_batch(Pconj, m);
+        p.add_seg(A, m, Up, m, (int)ncol);
+        zgemm_run(p, s);
+        unsigned gsz = (unsigned)std::min<int64_t>(4096, ((int64_t)m * m + 255) / 256);
+        hipLaunchKernelGGL(conj_hermitian_part_kernel, dim3(gsz), dim3(256), 0, s, Pconj, (int64_t)m);
+        NC_HIP(hipGetLastError());
+        return 0;
+    }
+    // rank-deficient (or numerically so) input: the SVD completes the polar
+    // factor with an orthonormal basis of the null directions, as zgesdd does
+    ++g_polar_stats.svd_fallbacks;
+    return polar_split_svd(m, ncol, A, Up, Pconj, s);
+}
+
+int polar_split_svd(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s) {
+    rocblas_handle h = solver_handle();
+    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error("rocblas_set_stream failed", -52);
     ZBuf Aw((size_t)m * ncol), U((size_t)m * m), Vt((size_t)m * ncol), Us((size_t)m * m);
     DevBuf<double> S((size_t)m), E((size_t)std::max(1, m));
     DevBuf<int> info(1);

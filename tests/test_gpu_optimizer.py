@@ -197,3 +197,22 @@ def test_site_move_rejects_too_few_degrees_of_freedom(core):
         core.norm_denorm_going_left(8, 16, 4, 2, 2, crand(rng, 16, 8, 2), crand(rng, 4, 16, 2))
     with pytest.raises(RuntimeError, match="degrees of freedom"):
         core.norm_denorm_going_right(2, 16, 4, 2, 2, crand(rng, 16, 2, 2), crand(rng, 4, 16, 2))
+
+
+@pytest.mark.parametrize("rank", [1, 3])
+def test_site_move_rank_deficient(core, rank):
+    """A rank-deficient site (a product state, as the external-field chains reach):
+    the polar factor is completed with orthonormal rows and the state is kept."""
+    rng = np.random.default_rng(40 + rank)
+    bl, bm, br, dl, dr = 4, 6, 5, 2, 2
+    u = crand(rng, bm, rank)
+    v = crand(rng, rank, bl * dl)
+    tn = np.asfortranarray((u @ v).reshape(bm, bl, dl, order="F"))
+    td = crand(rng, br, bm, dr)
+    info, on, od = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    assert info == 0
+    A = on.reshape(bm, bl * dl, order="F")
+    assert np.abs(A @ A.conj().T - np.eye(bm)).max() < 1e-12
+    before = np.einsum("mla,rmb->labr", tn, td)
+    after = np.einsum("mla,rmb->labr", on, od)
+    assert rel(after, before) < 1e-12
