@@ -26,6 +26,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 SOURCES = [
     "runtime.cpp",
     "kernels.hip",
+    "gemm.hip",
     "contract.cpp",
     "eigen_host.cpp",
     "iram.cpp",

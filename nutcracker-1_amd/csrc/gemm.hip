@@ -1,0 +1,470 @@
+// Complex GEMM for gfx950 on MFMA, FP64 (v_mfma_f64_16x16x4_f64) and FP32
+// (v_mfma_f32_16x16x4_f32), batched and K-segmented:
+//     C_b (M x N) = sum_s opA(A_s) opB(B_s)  (+ C_b when accumulating)
+// Output tile 64 x 64 per 256-thread workgroup (4 waves in 2 x 2, each wave 2 x 2
+// MFMA tiles of 16 x 16); K is staged through LDS in steps of BK, with the next
+// step's global loads issued before the current step's MFMAs (register
+// prefetch).  Complex products use either four real MFMAs per 16x16x4 step
+// (4M: re*re - im*im, re*im + im*re) or Gauss's three (3M: re*re, im*im,
+// (re+im)*(re+im), recombined at the end), which is normwise stable and saves
+// a quarter of the matrix-core work.
+//
+// Lane layouts (measured on the MI355X): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k; the f64 accumulator holds rows (lane / 16) + 4 v of column
+// lane % 16, the f32 one rows 4 (lane / 16) + v.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; the linear workgroup
+// id is remapped so that each XCD works on a contiguous block of output tiles
+// (neighbouring tiles share rows of A or columns of B in that XCD's L2).
+#include "nc_internal.hpp"
+
+#include <cstring>
+
+namespace nc {
+
+namespace {
+
+template <typename T>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    typedef double2 type;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Cplx<float> {
+    typedef float2 type;
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+
+constexpr int TBM = 64, TBN = 64, TPAD = 1;
+
+template <typename T>
+struct SegT {
+    const typename Cplx<T>::type* A;
+    const typename Cplx<T>::type* B;
+    int64_t lda, ldb;
+    int32_t K, _pad;
+};
+template <typename T>
+struct BatchT {
+    typename Cplx<T>::type* C;
+    int64_t ldc;
+    int32_t seg_begin, seg_end;
+};
+
+template <typename T, int OPA, int OPB, int BK>
+__device__ __forceinline__ void load_tiles(const SegT<T>& sg, int k0, int m0, int n0, int M, int N, int tid,
+                                           typename Cplx<T>::type (&ra)[BK / 4], typename Cplx<T>::type (&rb)[BK / 4]) {
+    typedef typename Cplx<T>::type CT;
+#pragma unroll
+    for (int it = 0; it < BK / 4; ++it) {
+        const int id = tid + 256 * it;
+        int mm, kk;
+        if (OPA == OpN) { mm = id & 63; kk = id >> 6; }
+        else            { kk = id % BK; mm = id / BK; }
+        const int gm = m0 + mm, gk = k0 + kk;
+        CT v;
+        v.x = 0; v.y = 0;
+        if (gm < M && gk < sg.K) {
+            if (OPA == OpN) v = sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
+            else {
+                v = sg.A[(int64_t)gk + (int64_t)gm * sg.lda];
+                if (OPA == OpC) v.y = -v.y;
+            }
+        }
+        ra[it] = v;
+    }
+#pragma unroll
+    for (int it = 0; it < BK / 4; ++it) {
+        const int id = tid + 256 * it;
+        int nn, kk;
+        if (OPB == OpN) { kk = id % BK; nn = id / BK; }
+        else            { nn = id & 63; kk = id >> 6; }
+        const int gn = n0 + nn, gk = k0 + kk;
+        CT v;
+        v.x = 0; v.y = 0;
+        if (gn < N && gk < sg.K) {
+            if (OPB == OpN) v = sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+            else {
+                v = sg.B[(int64_t)gn + (int64_t)gk * sg.ldb];
+                if (OPB == OpC) v.y = -v.y;
+            }
+        }
+        rb[it] = v;
+    }
+}
+
+template <typename T, int OPA, int OPB, int BK>
+__device__ __forceinline__ void store_tiles(int tid, typename Cplx<T>::type (*As)[TBM + TPAD],
+                                            typename Cplx<T>::type (*Bs)[TBN + TPAD],
+                                            const typename Cplx<T>::type (&ra)[BK / 4],
+                                            const typename Cplx<T>::type (&rb)[BK / 4]) {
+#pragma unroll
+    for (int it = 0; it < BK / 4; ++it) {
+        const int id = tid + 256 * it;
+        int mm, kk;
+        if (OPA == OpN) { mm = id & 63; kk = id >> 6; }
+        else            { kk = id % BK; mm = id / BK; }
+        As[kk][mm] = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < BK / 4; ++it) {
+        const int id = tid + 256 * it;
+        int nn, kk;
+        if (OPB == OpN) { kk = id % BK; nn = id / BK; }
+        else            { nn = id & 63; kk = id >> 6; }
+        Bs[kk][nn] = rb[it];
+    }
+}
+
+template <typename T, int OPA, int OPB, bool ACC, bool M3, int BK>
+__global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict__ batches,
+                                                    const SegT<T>* __restrict__ segs, int M, int N, int tiles_m,
+                                                    int tiles_n) {
+    typedef typename Cplx<T>::type CT;
+    typedef typename Cplx<T>::v4 V4;
+    __shared__ CT As[BK][TBM + TPAD];
+    __shared__ CT Bs[BK][TBN + TPAD];
+
+    // XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
+    // contiguous range of tiles (row-major over the tile grid)
+    const int ntiles = tiles_m * tiles_n;
+    const int w = blockIdx.x;
+    int tile = w;
+    if ((ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+
+    const BatchT<T> bt = batches[blockIdx.y];
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave & 1) * 32;
+    const int wn = (wave >> 1) * 32;
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+
+    constexpr int NACC = M3 ? 3 : 2;
+    V4 acc[NACC][2][2];
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = V4{0, 0, 0, 0};
+
+    CT ra[BK / 4], rb[BK / 4];
+    int sidx = bt.seg_begin;
+    int k0 = 0;
+    if (sidx < bt.seg_end) {
+        SegT<T> sg = segs[sidx];
+        load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
+    }
+    while (sidx < bt.seg_end) {
+        store_tiles<T, OPA, OPB, BK>(tid, As, Bs, ra, rb);
+        __syncthreads();
+        // advance and prefetch the next stage
+        k0 += BK;
+        if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
+        if (sidx < bt.seg_end) {
+            SegT<T> sg = segs[sidx];
+            load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
+        }
+#pragma unroll
+        for (int kq = 0; kq < BK / 4; ++kq) {
+            const int k = kq * 4 + lk;
+            CT a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[k][wm + i * 16 + lrow];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[k][wn + j * 16 + lrow];
+            if (M3) {
+                T as[2], bs[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) as[i] = a[i].x + a[i].y;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bs[j] = b[j].x + b[j].y;
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                        acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].y, acc[1][i][j]);
+                        acc[2][i][j] = Cplx<T>::mfma(as[i], bs[j], acc[2][i][j]);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const T nai = -a[i].y;
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                        acc[0][i][j] = Cplx<T>::mfma(nai, b[j].y, acc[0][i][j]);
+                        acc[1][i][j] = Cplx<T>::mfma(a[i].x, b[j].y, acc[1][i][j]);
+                        acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].x, acc[1][i][j]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    CT* __restrict__ C = bt.C;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 16 + lrow;
+            if (col >= N) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + Cplx<T>::row(lk, v);
+                if (row < M) {
+                    CT o;
+                    if (M3) {
+                        const T t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
+                        o.x = t1 - t2;
+                        o.y = t3 - t1 - t2;
+                    } else {
+                        o.x = acc[0][i][j][v];
+                        o.y = acc[1][i][j][v];
+                    }
+                    CT* p = C + (int64_t)row + (int64_t)col * bt.ldc;
+                    if (ACC) {
+                        CT c = *p;
+                        o.x += c.x;
+                        o.y += c.y;
+                    }
+                    *p = o;
+                }
+            }
+        }
+}
+
+// C_b = sum_c W_{b,c} (+ C_b) for the split-K partial products
+template <typename CT>
+__global__ void splitk_reduce_kernel(const CT* __restrict__ W, int64_t slice, int nsplit, CT* const* __restrict__ Cptr,
+                                     const int64_t* __restrict__ ldc, int M, int N, int accumulate) {
+    const int b = blockIdx.y;
+    const int64_t total = (int64_t)M * N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % M, c = e / M;
+        CT s;
+        s.x = 0; s.y = 0;
+        for (int q = 0; q < nsplit; ++q) {
+            const CT v = W[((int64_t)b * nsplit + q) * slice + e];
+            s.x += v.x;
+            s.y += v.y;
+        }
+        CT* p = Cptr[b] + r + c * ldc[b];
+        if (accumulate) {
+            const CT o = *p;
+            s.x += o.x;
+            s.y += o.y;
+        }
+        *p = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: descriptor staging ring and launch
+// ---------------------------------------------------------------------------
+struct Arena {
+    static constexpr size_t kSlot = 128 * 1024;
+    static constexpr int kSlots = 64;
+    char* dev = nullptr;
+    char* host = nullptr;
+    hipEvent_t done[kSlots];
+    int next = 0;
+    Arena() {
+        NC_HIP(hipMalloc(&dev, kSlot * kSlots));
+        NC_HIP(hipHostMalloc((void**)&host, kSlot * kSlots, hipHostMallocDefault));
+        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventRecord(done[i], Context::get().stream));
+    }
+    // copies `bytes` of host data into the next slot; returns the device address
+    char* stage(const void* const* parts, const size_t* sizes, int nparts, size_t* offsets, hipStream_t s, int* slot_out) {
+        size_t off = 0;
+        for (int i = 0; i < nparts; ++i) {
+            offsets[i] = off;
+            off = (off + sizes[i] + 255) & ~size_t(255);
+        }
+        if (off > kSlot) throw Error("GEMM descriptor list too long", -70);
+        int slot = next;
+        next = (next + 1) % kSlots;
+        NC_HIP(hipEventSynchronize(done[slot]));
+        char* h = host + slot * kSlot;
+        for (int i = 0; i < nparts; ++i) std::memcpy(h + offsets[i], parts[i], sizes[i]);
+        char* d = dev + slot * kSlot;
+        NC_HIP(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
+        *slot_out = slot;
+        return d;
+    }
+    void release(int slot, hipStream_t s) { NC_HIP(hipEventRecord(done[slot], s)); }
+};
+Arena& arena() {
+    static Arena* a = new Arena();
+    return *a;
+}
+
+template <typename T, int OPA, int OPB>
+void launch_typed(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, int M, int N, int tm, int tn,
+                  hipStream_t st) {
+#define NC_L(ACCV, M3V) \
+    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, 16>), grid, dim3(256), 0, st, b, s, M, N, tm, tn)
+    if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
+    else { if (m3) NC_L(false, true); else NC_L(false, false); }
+#undef NC_L
+}
+
+template <typename T>
+void launch_any(int opA, int opB, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, int M, int N,
+                int tm, int tn, hipStream_t st) {
+#define NC_O(a, bb) \
+    if (opA == a && opB == bb) { launch_typed<T, a, bb>(acc, m3, grid, b, s, M, N, tm, tn, st); return; }
+    NC_O(OpN, OpN) NC_O(OpN, OpT) NC_O(OpN, OpC) NC_O(OpT, OpN) NC_O(OpC, OpN) NC_O(OpC, OpC) NC_O(OpT, OpT)
+#undef NC_O
+    throw Error("GEMM: unsupported operand combination", -71);
+}
+
+GemmTuning g_tuning;
+
+// splits every batch's K range into `nsplit` nearly equal chunks (segment
+// boundaries respected by cutting segments), each chunk its own batch
+template <typename Seg>
+void split_segments(std::vector<Seg> const& segs, int sb, int se, int nsplit, int opA, int opB,
+                    std::vector<std::vector<Seg>>& out) {
+    int64_t total = 0;
+    for (int i = sb; i < se; ++i) total += segs[i].K;
+    out.assign(nsplit, {});
+    int64_t per = (total + nsplit - 1) / nsplit;
+    per = ((per + 15) / 16) * 16;
+    int chunk = 0;
+    int64_t filled = 0;
+    for (int i = sb; i < se; ++i) {
+        Seg s = segs[i];
+        int64_t k = 0;
+        while (k < s.K) {
+            if (chunk >= nsplit) chunk = nsplit - 1;
+            int64_t take = std::min<int64_t>(s.K - k, per - filled);
+            if (chunk == nsplit - 1) take = s.K - k;
+            Seg p = s;
+            p.K = (int32_t)take;
+            // advance the operand pointers to column / row k of op(A), op(B)
+            p.A = s.A + (opA == OpN ? k * s.lda : k);
+            p.B = s.B + (opB == OpN ? k : k * s.ldb);
+            out[chunk].push_back(p);
+            k += take;
+            filled += take;
+            if (filled >= per && chunk < nsplit - 1) { ++chunk; filled = 0; }
+        }
+    }
+}
+
+template <typename T>
+void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>> const& batches,
+              std::vector<SegT<T>> const& segs, hipStream_t s) {
+    typedef typename Cplx<T>::type CT;
+    if (batches.empty() || M <= 0 || N <= 0) return;
+    const int tm = (M + TBM - 1) / TBM, tn = (N + TBN - 1) / TBN;
+    const int64_t tiles = (int64_t)tm * tn * (int64_t)batches.size();
+    int64_t maxk = 0;
+    for (auto const& b : batches) {
+        int64_t k = 0;
+        for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
+        maxk = std::max(maxk, k);
+    }
+    // split K when the launch would leave most of the chip idle
+    int nsplit = 1;
+    if (g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
+        while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
+    }
+    const bool m3 = g_tuning.use_3m;
+    Arena& ar = arena();
+    if (nsplit == 1) {
+        const void* parts[2] = {batches.data(), segs.data()};
+        size_t sizes[2] = {batches.size() * sizeof(BatchT<T>), segs.size() * sizeof(SegT<T>)};
+        size_t offs[2];
+        int slot;
+        char* d = ar.stage(parts, sizes, 2, offs, s, &slot);
+        dim3 grid((unsigned)(tm * tn), (unsigned)batches.size());
+        launch_any<T>(opA, opB, accumulate, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), M,
+                      N, tm, tn, s);
+        NC_HIP(hipGetLastError());
+        ar.release(slot, s);
+        return;
+    }
+    // split-K: partial products into a workspace, then a reduction into C
+    const int64_t slice = (int64_t)M * N;
+    static thread_local DevBuf<char> work;
+    const size_t need = (size_t)slice * nsplit * batches.size() * sizeof(CT);
+    if (work.n < need) work.resize(need);
+    CT* W = (CT*)work.ptr;
+    std::vector<BatchT<T>> nb;
+    std::vector<SegT<T>> ns;
+    std::vector<CT*> cptr;
+    std::vector<int64_t> ldcs;
+    for (size_t b = 0; b < batches.size(); ++b) {
+        std::vector<std::vector<SegT<T>>> chunks;
+        split_segments(segs, batches[b].seg_begin, batches[b].seg_end, nsplit, opA, opB, chunks);
+        for (int q = 0; q < nsplit; ++q) {
+            BatchT<T> x;
+            x.C = W + ((int64_t)b * nsplit + q) * slice;
+            x.ldc = M;
+            x.seg_begin = (int32_t)ns.size();
+            for (auto const& sg : chunks[q]) ns.push_back(sg);
+            x.seg_end = (int32_t)ns.size();
+            nb.push_back(x);
+        }
+        cptr.push_back(batches[b].C);
+        ldcs.push_back(batches[b].ldc);
+    }
+    const void* parts[4] = {nb.data(), ns.data(), cptr.data(), ldcs.data()};
+    size_t sizes[4] = {nb.size() * sizeof(BatchT<T>), ns.size() * sizeof(SegT<T>), cptr.size() * sizeof(CT*),
+                       ldcs.size() * sizeof(int64_t)};
+    size_t offs[4];
+    int slot;
+    char* d = ar.stage(parts, sizes, 4, offs, s, &slot);
+    dim3 grid((unsigned)(tm * tn), (unsigned)nb.size());
+    launch_any<T>(opA, opB, false, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), M, N, tm,
+                  tn, s);
+    NC_HIP(hipGetLastError());
+    unsigned g = (unsigned)std::min<int64_t>(2048, (slice + 255) / 256);
+    hipLaunchKernelGGL(splitk_reduce_kernel<CT>, dim3(g, (unsigned)batches.size()), dim3(256), 0, s, W, slice, nsplit,
+                       (CT* const*)(d + offs[2]), (const int64_t*)(d + offs[3]), M, N, accumulate ? 1 : 0);
+    NC_HIP(hipGetLastError());
+    ar.release(slot, s);
+}
+
+}  // namespace
+
+GemmTuning& gemm_tuning() { return g_tuning; }
+
+void zgemm_run(GemmPlan const& plan, hipStream_t s) {
+    static_assert(sizeof(BatchT<double>) == sizeof(GemmBatch), "descriptor layout");
+    static_assert(sizeof(SegT<double>) == sizeof(GemmSeg), "descriptor layout");
+    std::vector<BatchT<double>> b(plan.batches.size());
+    std::vector<SegT<double>> sg(plan.segs.size());
+    std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
+    std::memcpy(sg.data(), plan.segs.data(), sg.size() * sizeof(GemmSeg));
+    run_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
+}
+
+void cgemm_run(CGemmPlan const& plan, hipStream_t s) {
+    std::vector<BatchT<float>> b;
+    std::vector<SegT<float>> sg;
+    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end});
+    for (auto const& x : plan.segs) sg.push_back(SegT<float>{x.A, x.B, x.lda, x.ldb, x.K, 0});
+    run_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
+}
+
+}  // namespace nc

@@ -32,6 +32,18 @@ int wanted_index(std::vector<zd> const& th, const char* which) {
     return best;
 }
 
+// indices of the Ritz values from most to least wanted
+std::vector<int> wanted_order(std::vector<zd> const& th, const char* which) {
+    std::vector<int> o(th.size());
+    for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) {
+        if (which[0] == 'S' && which[1] == 'R') return th[a].real() < th[b].real();
+        if (which[0] == 'L' && which[1] == 'R') return th[a].real() > th[b].real();
+        return std::abs(th[a]) > std::abs(th[b]);
+    });
+    return o;
+}
+
 // one explicit shifted QR step on the m x m Hessenberg H, accumulating Q
 void shifted_qr_step(int m, std::vector<zd>& H, zd mu, std::vector<zd>& Q) {
     auto h = [&](int i, int j) -> zd& { return H[(size_t)i + (size_t)j * m]; };
@@ -86,7 +98,7 @@ IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_re
     const double tol_eff = tol > 0 ? tol : eps;
 
     ZBuf V((size_t)n * (ncv + 1));   // columns 0..ncv-1: basis, column ncv: residual f
-    ZBuf W((size_t)n * 2);
+    ZBuf W((size_t)n * 2), Vk;
     double2* f = V.ptr + (size_t)n * ncv;
     double2* w = W.ptr;
     double2* tmp = W.ptr + n;
@@ -166,36 +178,57 @@ IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_re
             out.info = 0;
             return out;
         }
-        // ---- implicit restart: shifts = the ncv - 1 unwanted Ritz values
+        // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies the
+        // ncv - 1 unwanted Ritz values as exact shifts; a larger basis keeps half
+        // of its vectors (thick restart) and shifts out the least wanted half.
+        const int keep = ncv <= 3 ? 1 : ncv / 2;
+        std::vector<int> order = wanted_order(th, which);
         std::vector<zd> Q((size_t)ncv * ncv, zd(0, 0));
         for (int i = 0; i < ncv; ++i) Q[(size_t)i + (size_t)i * ncv] = 1.0;
-        for (int i = 0; i < ncv; ++i)
-            if (i != wi) shifted_qr_step(ncv, H, th[i], Q);
-        // new v_0 = V q_0 ; new f = V q_1 * H(1,0) + f * Q(ncv-1, 0)
-        std::vector<zd> q0(ncv), q1(ncv);
-        for (int i = 0; i < ncv; ++i) {
-            q0[i] = Q[(size_t)i];
-            q1[i] = Q[(size_t)i + (size_t)ncv];
+        for (int t = keep; t < ncv; ++t) shifted_qr_step(ncv, H, th[order[t]], Q);
+        // V_k <- V Q(:, 0:keep) ; f <- V q_keep H(keep, keep-1) + f Q(ncv-1, keep-1)
+        const zd hk = h(keep, keep - 1);
+        const zd qm = Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
+        zgemv_n(V.ptr, n, ncv, &Q[(size_t)keep * ncv], tmp, n, s);   // V q_keep
+        zaxpby(tmp, qm, f, hk, n, s);                                  // tmp = hk V q_keep + qm f
+        if (Vk.n < (size_t)n * keep) Vk.resize((size_t)n * keep);
+        if (keep == 1) {
+            zgemv_n(V.ptr, n, ncv, Q.data(), Vk.ptr, n, s);
+        } else {
+            ZBuf Qd((size_t)ncv * keep);
+            h2d(Qd.ptr, Q.data(), (size_t)ncv * keep * sizeof(double2));
+            GemmPlan g;
+            g.M = (int)n; g.N = keep; g.opA = OpN; g.opB = OpN;
+            g.begin_batch(Vk.ptr, n);
+            g.add_seg(V.ptr, n, Qd.ptr, ncv, ncv);
+            zgemm_run(g, s);
+            NC_HIP(hipStreamSynchronize(s));
         }
-        const zd h10 = h(1, 0);
-        const zd qm0 = Q[(size_t)(ncv - 1)];
-        zgemv_n(V.ptr, n, ncv, q1.data(), tmp, n, s);       // V q1
-        zaxpby(tmp, qm0, f, h10, n, s);                      // tmp = h10 * Vq1 + qm0 * f
-        zgemv_n(V.ptr, n, ncv, q0.data(), w, n, s);          // w = V q0
-        d2d(V.ptr, w, n * sizeof(double2));
+        d2d(V.ptr, Vk.ptr, (size_t)n * keep * sizeof(double2));
         d2d(f, tmp, n * sizeof(double2));
-        const zd h00 = h(0, 0);
+        std::vector<zd> Hk((size_t)keep * keep);
+        for (int c = 0; c < keep; ++c)
+            for (int r = 0; r < keep; ++r) Hk[(size_t)r + (size_t)c * keep] = (r <= c + 1) ? h(r, c) : zd(0, 0);
         std::fill(H.begin(), H.end(), zd(0, 0));
-        h(0, 0) = h00;
+        for (int c = 0; c < keep; ++c)
+            for (int r = 0; r < keep; ++r) h(r, c) = Hk[(size_t)r + (size_t)c * keep];
         beta = dznrm2(f, n, s);
-        // keep v_0 exactly unit norm against round-off drift
-        double n0 = dznrm2(V.ptr, n, s);
-        zscal(V.ptr, zd(1.0 / n0, 0), n, s);
-        k = 1;
+        if (keep == 1) {
+            // keep v_0 exactly unit norm against round-off drift
+            double n0 = dznrm2(V.ptr, n, s);
+            zscal(V.ptr, zd(1.0 / n0, 0), n, s);
+        }
+        k = keep;
         if (beta == 0.0) {
-            // the restart landed on an invariant subspace: v_0 is an eigenvector
-            d2d(result, V.ptr, n * sizeof(double2));
-            out.eigenvalue = h00;
+            // the restart landed on an invariant subspace: its Ritz pairs are exact
+            std::vector<zd> th2, Y2;
+            hessenberg_eig(keep, Hk, th2, Y2);
+            const int w2 = wanted_index(th2, which);
+            std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
+            zgemv_n(V.ptr, n, keep, yv.data(), result, n, s);
+            double rn = dznrm2(result, n, s);
+            zscal(result, zd(1.0 / rn, 0), n, s);
+            out.eigenvalue = th2[w2];
             out.restarts = iter;
             out.info = 0;
             return out;

@@ -1,8 +1,6 @@
 // gfx950 device kernels of the variational-MPS core (FP64 complex path).
 //
-//  * zgemm_f64: batched, K-segmented complex GEMM on v_mfma_f64_16x16x4_f64.
-//    Complex products use four real MFMAs per 16x16x4 step (re*re - im*im,
-//    re*im + im*re) into separate real / imaginary accumulators.
+//  (the complex GEMM lives in gemm.hip)
 //  * mix: the sparse MPO block mixing Q(., b, dst) = sum W_n(a,b) P(., a, src).
 //  * vector kernels: reductions, axpy and friends for the Arnoldi solver and
 //    the Householder projectors.
@@ -13,200 +11,6 @@
 namespace nc {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-// ----------------------------------------------------------------------------
-// Complex GEMM, 64 x 64 output tile per 256-thread workgroup (4 waves in 2 x 2,
-// each wave 32 x 32 = 2 x 2 MFMA tiles), K staged through LDS 16 at a time.
-// ----------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
-
-__device__ __forceinline__ double2 zload(const double2* __restrict__ p) { return *p; }
-__device__ __forceinline__ double2 zconj(double2 v) { return make_double2(v.x, -v.y); }
-
-template <int OPA, int OPB, bool ACC>
-__global__ __launch_bounds__(256) void zgemm_f64_kernel(const GemmBatch* __restrict__ batches,
-                                                        const GemmSeg* __restrict__ segs, int M,
-                                                        int N) {
-    __shared__ double2 As[GBK][GBM + GPAD];
-    __shared__ double2 Bs[GBK][GBN + GPAD];
-
-    const GemmBatch bt = batches[blockIdx.z];
-    const int m0 = blockIdx.x * GBM;
-    const int n0 = blockIdx.y * GBN;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = (wave & 1) * 32;
-    const int wn = (wave >> 1) * 32;
-    const int lrow = lane & 15;
-    const int lk = lane >> 4;
-
-    f64x4 acc_re[2][2], acc_im[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = f64x4{0, 0, 0, 0};
-            acc_im[i][j] = f64x4{0, 0, 0, 0};
-        }
-
-    for (int sidx = bt.seg_begin; sidx < bt.seg_end; ++sidx) {
-        const GemmSeg sg = segs[sidx];
-        const double2* __restrict__ A = sg.A;
-        const double2* __restrict__ B = sg.B;
-        const int K = sg.K;
-        for (int k0 = 0; k0 < K; k0 += GBK) {
-            // ---- A tile: op(A)(m, k), m in [m0, m0+64), k in [k0, k0+16)
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                const int id = tid + 256 * it;
-                int mm, kk;
-                if (OPA == OpN) { mm = id & 63; kk = id >> 6; }
-                else            { kk = id & 15; mm = id >> 4; }
-                const int gm = m0 + mm, gk = k0 + kk;
-                double2 v = make_double2(0.0, 0.0);
-                if (gm < M && gk < K) {
-                    if (OPA == OpN) v = zload(A + (int64_t)gm + (int64_t)gk * sg.lda);
-                    else {
-                        v = zload(A + (int64_t)gk + (int64_t)gm * sg.lda);
-                        if (OPA == OpC) v = zconj(v);
-                    }
-                }
-                As[kk][mm] = v;
-            }
-            // ---- B tile: op(B)(k, n)
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                const int id = tid + 256 * it;
-                int nn, kk;
-                if (OPB == OpN) { kk = id & 15; nn = id >> 4; }
-                else            { nn = id & 63; kk = id >> 6; }
-                const int gn = n0 + nn, gk = k0 + kk;
-                double2 v = make_double2(0.0, 0.0);
-                if (gn < N && gk < K) {
-                    if (OPB == OpN) v = zload(B + (int64_t)gk + (int64_t)gn * sg.ldb);
-                    else {
-                        v = zload(B + (int64_t)gn + (int64_t)gk * sg.ldb);
-                        if (OPB == OpC) v = zconj(v);
-                    }
-                }
-                Bs[kk][nn] = v;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kq = 0; kq < GBK / 4; ++kq) {
-                const int k = kq * 4 + lk;
-                double2 a[2], b[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) a[i] = As[k][wm + i * 16 + lrow];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) b[j] = Bs[k][wn + j * 16 + lrow];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const double are = a[i].x, aim = a[i].y, naim = -a[i].y;
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, b[j].x, acc_re[i][j], 0, 0, 0);
-                        acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(naim, b[j].y, acc_re[i][j], 0, 0, 0);
-                        acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, b[j].y, acc_im[i][j], 0, 0, 0);
-                        acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, b[j].x, acc_im[i][j], 0, 0, 0);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-
-    // ---- store: for the f64 16x16x4 MFMA a lane holds column (lane & 15) and
-    // the strided rows (lane >> 4) + 4 v, v = 0..3
-    double2* __restrict__ C = bt.C;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn + j * 16 + lrow;
-            if (col >= N) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm + i * 16 + lk + 4 * v;
-                if (row < M) {
-                    double2* p = C + (int64_t)row + (int64_t)col * bt.ldc;
-                    double2 o = make_double2(acc_re[i][j][v], acc_im[i][j][v]);
-                    if (ACC) {
-                        double2 c = *p;
-                        o.x += c.x;
-                        o.y += c.y;
-                    }
-                    *p = o;
-                }
-            }
-        }
-}
-
-template <int OPA, int OPB>
-static void launch_ops(bool acc, dim3 grid, GemmBatch const* b, GemmSeg const* s, int M, int N,
-                       hipStream_t st) {
-    if (acc)
-        hipLaunchKernelGGL((zgemm_f64_kernel<OPA, OPB, true>), grid, dim3(256), 0, st, b, s, M, N);
-    else
-        hipLaunchKernelGGL((zgemm_f64_kernel<OPA, OPB, false>), grid, dim3(256), 0, st, b, s, M, N);
-}
-
-void zgemm_launch(int opA, int opB, bool accumulate, int M, int N, GemmBatch const* d_batches,
-                  GemmSeg const* d_segs, int nbatch, int /*max_k_total*/, hipStream_t s) {
-    if (M <= 0 || N <= 0 || nbatch <= 0) return;
-    dim3 grid((M + GBM - 1) / GBM, (N + GBN - 1) / GBN, nbatch);
-#define NC_OPS(a, b) \
-    if (opA == a && opB == b) { launch_ops<a, b>(accumulate, grid, d_batches, d_segs, M, N, s); NC_HIP(hipGetLastError()); return; }
-    NC_OPS(OpN, OpN) NC_OPS(OpN, OpT) NC_OPS(OpN, OpC)
-    NC_OPS(OpT, OpN) NC_OPS(OpC, OpN) NC_OPS(OpC, OpC) NC_OPS(OpT, OpT)
-#undef NC_OPS
-    throw Error("zgemm_launch: unsupported operand combination");
-}
-
-// Descriptor arena: a ring of device slots, each written by a stream-ordered
-// copy from pinned host memory, so building a plan never synchronizes.
-namespace {
-struct DescArena {
-    static constexpr size_t kSlot = 64 * 1024;
-    static constexpr int kSlots = 64;
-    char* dev = nullptr;
-    char* host = nullptr;
-    hipEvent_t done[kSlots];
-    int next = 0;
-    DescArena() {
-        NC_HIP(hipMalloc(&dev, kSlot * kSlots));
-        NC_HIP(hipHostMalloc((void**)&host, kSlot * kSlots, hipHostMallocDefault));
-        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
-        for (int i = 0; i < kSlots; ++i) NC_HIP(hipEventRecord(done[i], Context::get().stream));
-    }
-};
-DescArena& arena() {
-    static DescArena* a = new DescArena();
-    return *a;
-}
-}  // namespace
-
-void zgemm_run(GemmPlan const& plan, hipStream_t s) {
-    if (plan.batches.empty()) return;
-    size_t bbytes = plan.batches.size() * sizeof(GemmBatch);
-    size_t sbytes = plan.segs.size() * sizeof(GemmSeg);
-    size_t soff = (bbytes + 255) & ~size_t(255);
-    if (soff + sbytes > DescArena::kSlot) throw Error("zgemm_run: too many GEMM descriptors");
-    DescArena& a = arena();
-    int slot = a.next;
-    a.next = (a.next + 1) % DescArena::kSlots;
-    // the host staging of this slot may still be read by an earlier copy
-    NC_HIP(hipEventSynchronize(a.done[slot]));
-    char* h = a.host + slot * DescArena::kSlot;
-    char* d = a.dev + slot * DescArena::kSlot;
-    std::memcpy(h, plan.batches.data(), bbytes);
-    std::memcpy(h + soff, plan.segs.data(), sbytes);
-    NC_HIP(hipMemcpyAsync(d, h, soff + sbytes, hipMemcpyHostToDevice, s));
-    zgemm_launch(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, (GemmBatch const*)d,
-                 (GemmSeg const*)(d + soff), (int)plan.batches.size(), plan.max_k(), s);
-    NC_HIP(hipEventRecord(a.done[slot], s));
-}
 
 // ----------------------------------------------------------------------------
 // Mix kernel.  Blocks are grouped by destination (CSR); each thread owns one

@@ -108,10 +108,13 @@ struct GemmBatch {
 
 enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
 
-// Launch with descriptors already resident on the device.
-void zgemm_launch(int opA, int opB, bool accumulate, int M, int N,
-                  GemmBatch const* d_batches, GemmSeg const* d_segs, int nbatch,
-                  int max_k_total, hipStream_t s);
+// Launch-time choices of the GEMM (see gemm.hip).
+struct GemmTuning {
+    bool use_3m = true;             // Gauss's three-multiplication complex product
+    bool split_k = true;            // split K when a launch has few output tiles
+    int64_t split_k_tiles = 1024;   // ... fewer than this many (tiles x batches)
+};
+GemmTuning& gemm_tuning();
 
 // Host-side builder: keeps descriptor lists and uploads them per launch into a
 // small reusable device arena.
@@ -133,6 +136,33 @@ struct GemmPlan {
 
 // Upload-and-launch (descriptor arena is stream ordered).
 void zgemm_run(GemmPlan const& plan, hipStream_t s);
+
+// The same for complex64 operands (FP32 MFMA), used by the mixed-precision form.
+struct CGemmSeg {
+    const float2* A;
+    const float2* B;
+    int64_t lda, ldb;
+    int32_t K, _pad;
+};
+struct CGemmBatch {
+    float2* C;
+    int64_t ldc;
+    int32_t seg_begin, seg_end;
+};
+struct CGemmPlan {
+    std::vector<CGemmBatch> batches;
+    std::vector<CGemmSeg> segs;
+    int M = 0, N = 0, opA = OpN, opB = OpN;
+    bool accumulate = false;
+    void begin_batch(float2* C, int64_t ldc) {
+        batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size()});
+    }
+    void add_seg(const float2* A, int64_t lda, const float2* B, int64_t ldb, int K) {
+        segs.push_back(CGemmSeg{A, B, lda, ldb, K, 0});
+        batches.back().seg_end = (int32_t)segs.size();
+    }
+};
+void cgemm_run(CGemmPlan const& plan, hipStream_t s);
 
 // ----------------------------------------------------------------------------
 // Sparse MPO block list, prepared for grouped contraction.

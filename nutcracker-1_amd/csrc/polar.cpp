@@ -49,25 +49,60 @@ __global__ void conj_hermitian_part_kernel(double2* P, int64_t m) {
     }
 }
 
+// X(i, j) += scale * u(i, j) with u uniform in [-1, 1) from a counter hash of
+// the element index (splitmix64), so the perturbation is deterministic
+__global__ void add_hash_noise_kernel(double2* X, int64_t total, double scale) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = (uint64_t)e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double ur = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        z = z * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double ui = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        double2 v = X[e];
+        X[e] = make_double2(v.x + scale * ur, v.y + scale * ui);
+    }
+}
+
 namespace {
 PolarStats g_polar_stats;
 
 // Scaled Newton-Schulz iteration for the unitary polar factor of a wide
 // matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X,  with the
-// equioscillation scaling alpha = sqrt(3 / (1 + l + l^2)) for an assumed lower
-// bound l of the smallest singular value (X is first divided by its Frobenius
-// norm, so every singular value lies in (0, 1]).  The scaled cubic maps
-// [l, 1] onto [p(l), 1] and grows singular values below l by at least 3/2 per
-// step, so it converges for any full-rank input; the Gram matrix of each step
-// gives ||X X^H - I||_F for free.  Only GEMMs of this library are used.
+// equioscillation scaling alpha = sqrt(3 / (1 + l + l^2)) for a lower bound l
+// of the smallest singular value (X is first divided by its Frobenius norm, so
+// every singular value lies in (0, 1]).  The scaled cubic maps [l, 1] onto
+// [p(l), 1] with p(l) ~ 2.6 l, so the number of steps is fixed by l alone.
+// A site after optimisation is often numerically rank deficient (Schmidt
+// values far below round-off); its polar factor is then not determined in
+// those directions, and the iteration would stall there.  A deterministic
+// perturbation of Frobenius norm 1e-12 ||A|| floors the singular values near
+// 1e-14 ||A||, the schedule starts from l = 1e-14 and converges in about 40
+// steps for any input; the perturbation moves the split between the two sites
+// by ~1e-12 relative and leaves the state itself unchanged (P = A U_p^H below).
+// The Gram matrix of each step gives ||X X^H - I||_F for free.  Only GEMMs of
+// this library are used.
 bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStream_t s, int* iters_out) {
     const double nrm = dznrm2(A, (int64_t)m * n, s);
     if (!(nrm > 0) || !std::isfinite(nrm)) return false;
     ZBuf X0((size_t)m * n), X1((size_t)m * n), G((size_t)m * m);
     d2d(X0.ptr, A, (size_t)m * n * sizeof(double2));
-    zscal(X0.ptr, zd(1.0 / nrm, 0), (int64_t)m * n, s);
-    double l = 1e-3;
-    const int max_iters = 100;
+    const int64_t total = (int64_t)m * n;
+    // uniform entries in [-1,1) + i[-1,1) have RMS sqrt(2/3)
+    const double pscale = 1e-12 * nrm / std::sqrt((double)total) * std::sqrt(1.5);
+    {
+        unsigned gsz = (unsigned)std::min<int64_t>(4096, (total + 255) / 256);
+        hipLaunchKernelGGL(add_hash_noise_kernel, dim3(gsz), dim3(256), 0, s, X0.ptr, total, pscale);
+        NC_HIP(hipGetLastError());
+    }
+    const double nrm2 = dznrm2(X0.ptr, total, s);
+    zscal(X0.ptr, zd(1.0 / nrm2, 0), total, s);
+    double l = 1e-14;
+    const int max_iters = 60;
     bool last = false;
     for (int it = 0; it < max_iters; ++it) {
         GemmPlan g;  // G = X X^H
