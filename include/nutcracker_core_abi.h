@@ -172,6 +172,14 @@ int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d
                   const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
                   double* phase_ms);
 
+/* The same matvec in complex64 on FP32 matrix cores (the mixed-precision form's
+   matvec): operands are narrowed on the device before timing and y is widened
+   back to complex128. */
+int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                      const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L, const nc_zcomplex* R,
+                      const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
+                      double* phase_ms);
+
 /* Site-move statistics since load: out[0] polar splits, out[1] Newton-Schulz
    iterations, out[2] splits that fell back to the SVD (rank-deficient input). */
 int nc_polar_stats(int64_t* out);
@@ -184,6 +192,13 @@ nc_chain* nc_chain_create(uint32_t n_sites, const uint32_t* d, const uint32_t* c
 void nc_chain_destroy(nc_chain* chain);
 int nc_chain_set_options(nc_chain* chain, double site_tol, double sweep_tol, double sanity,
                          uint32_t max_iterations, const char* which, uint32_t ncv, int rethrow);
+/* Mixed-precision form: H_eff matvecs (mixed != 0) and environment updates
+   (env32 != 0 as well) in complex64 on FP32 matrix cores; Krylov vectors,
+   orthogonalisation, the Rayleigh quotient and the site move in FP64.  The Krylov
+   iteration stops at tol32; the eigenvalue is the FP64 Rayleigh quotient of the
+   result, or (polish != 0) the FP64 iteration continues from it to the site
+   tolerance.  Call before nc_chain_reset for env32 to cover the first build. */
+int nc_chain_set_mixed(nc_chain* chain, int mixed, int env32, double tol32, int polish);
 /* raw (unnormalised) site tensor (br, bl, d); nc_chain_reset canonicalises */
 int nc_chain_set_raw_site(nc_chain* chain, uint32_t site, const nc_zcomplex* data);
 int nc_chain_get_site(nc_chain* chain, uint32_t site, nc_zcomplex* out);
@@ -194,8 +209,8 @@ int nc_chain_sweep(nc_chain* chain);
 int nc_chain_sweep_until_converged(nc_chain* chain);
 int nc_chain_energy(nc_chain* chain, double* energy, int* current_site);
 int nc_chain_expectation(nc_chain* chain, nc_zcomplex* out);
-/* counts: matvecs, site solves, moves, sweeps, failures;
-   times_ms: optimize, site move, environment update, reset */
+/* counts (6): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs;
+   times_ms (4): optimize, site move, environment update, reset */
 int nc_chain_stats(nc_chain* chain, int64_t* counts, double* times_ms);
 
 #ifdef __cplusplus

@@ -1,25 +1,58 @@
-// Timing entry point for the H_eff matvec with every operand resident in HBM.
+// Timing entry points for the H_eff matvec with every operand resident in HBM.
 // Timing uses HIP events recorded on the library's stream (the stream every
 // kernel of the matvec is launched on).
 #include "nc_internal.hpp"
 #include "../../include/nutcracker_core_abi.h"
 
+#include <type_traits>
+
 using namespace nc;
 
-extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
-                             const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
-                             const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
-                             uint32_t reps, double* total_ms, double* phase_ms) {
+namespace {
+
+template <typename CT>
+struct BenchPlan;
+template <>
+struct BenchPlan<double2> {
+    typedef GemmPlan type;
+    static void run(GemmPlan const& p, hipStream_t s) { zgemm_run(p, s); }
+};
+template <>
+struct BenchPlan<float2> {
+    typedef CGemmPlan type;
+    static void run(CGemmPlan const& p, hipStream_t s) { cgemm_run(p, s); }
+};
+
+// Operands arrive as complex128 on the host; CT = float2 narrows them on the
+// device before timing, and widens y back to complex128.
+template <typename CT>
+int bench_heff(const char* name, uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+               const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L, const nc_zcomplex* R,
+               const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
+               double* phase_ms) {
     try {
         auto& c = Context::get();
         hipStream_t s = c.stream;
         MpoSite op;
         op.build((int)d, (int)cl, (int)cr, (int)nmat, idx, reinterpret_cast<const zd*>(mats));
         const size_t nL = (size_t)bl * bl * cl, nR = (size_t)br * br * cr, nx = (size_t)br * bl * d;
-        ZBuf dL(nL), dR(nR), dx(nx), dy(nx);
-        h2d(dL.ptr, L, nL * sizeof(double2));
-        h2d(dR.ptr, R, nR * sizeof(double2));
-        h2d(dx.ptr, x, nx * sizeof(double2));
+        DevBuf<CT> dL(nL), dR(nR), dx(nx), dy(nx);
+        {
+            ZBuf tL(nL), tR(nR), tx(nx);
+            h2d(tL.ptr, L, nL * sizeof(double2));
+            h2d(tR.ptr, R, nR * sizeof(double2));
+            h2d(tx.ptr, x, nx * sizeof(double2));
+            if constexpr (std::is_same<CT, double2>::value) {
+                d2d(dL.ptr, tL.ptr, nL * sizeof(double2));
+                d2d(dR.ptr, tR.ptr, nR * sizeof(double2));
+                d2d(dx.ptr, tx.ptr, nx * sizeof(double2));
+            } else {
+                to_c64(tL.ptr, dL.ptr, (int64_t)nL, s);
+                to_c64(tR.ptr, dR.ptr, (int64_t)nR, s);
+                to_c64(tx.ptr, dx.ptr, (int64_t)nx, s);
+            }
+            NC_HIP(hipStreamSynchronize(s));
+        }
         Workspace ws;
         for (uint32_t i = 0; i < warmup; ++i) heff_matvec((int)bl, (int)br, op, dL.ptr, dR.ptr, dx.ptr, dy.ptr, ws, s);
         NC_HIP(hipStreamSynchronize(s));
@@ -40,26 +73,28 @@ extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr,
             for (auto& e : ev) NC_HIP(hipEventCreate(&e));
             const uint32_t preps = reps < 3 ? reps : 3;
             const int64_t rs = (int64_t)br * bl;
-            ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
+            ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size(), (CT*)nullptr);
+            CT* P = ws.p((CT*)nullptr);
+            CT* Q = ws.q((CT*)nullptr);
             for (uint32_t i = 0; i < preps; ++i) {
-                GemmPlan pa;
+                typename BenchPlan<CT>::type pa;
                 pa.M = (int)br; pa.N = (int)bl;
                 for (size_t fi = 0; fi < op.froms.size(); ++fi)
                     for (uint32_t a = 0; a < d; ++a) {
-                        pa.begin_batch(ws.P.ptr + ((int64_t)fi * d + a) * rs, br);
+                        pa.begin_batch(P + ((int64_t)fi * d + a) * rs, br);
                         pa.add_seg(dx.ptr + a * rs, br, dL.ptr + (int64_t)op.froms[fi] * bl * bl, bl, (int)bl);
                     }
-                GemmPlan pc;
+                typename BenchPlan<CT>::type pc;
                 pc.M = (int)br; pc.N = (int)(bl * d);
                 pc.begin_batch(dy.ptr, br);
                 for (size_t ti = 0; ti < op.tos.size(); ++ti)
-                    pc.add_seg(dR.ptr + (int64_t)op.tos[ti] * br * br, br, ws.Q.ptr + (int64_t)ti * d * rs, br, (int)br);
+                    pc.add_seg(dR.ptr + (int64_t)op.tos[ti] * br * br, br, Q + (int64_t)ti * d * rs, br, (int)br);
                 NC_HIP(hipEventRecord(ev[0], s));
-                zgemm_run(pa, s);
+                BenchPlan<CT>::run(pa, s);
                 NC_HIP(hipEventRecord(ev[1], s));
-                mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
+                mix_blocks(op, true, P, Q, rs, s);
                 NC_HIP(hipEventRecord(ev[2], s));
-                zgemm_run(pc, s);
+                BenchPlan<CT>::run(pc, s);
                 NC_HIP(hipEventRecord(ev[3], s));
                 NC_HIP(hipEventSynchronize(ev[3]));
                 float t;
@@ -71,13 +106,38 @@ extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr,
         }
         NC_HIP(hipEventDestroy(e0));
         NC_HIP(hipEventDestroy(e1));
-        d2h(y, dy.ptr, nx * sizeof(double2));
+        if constexpr (std::is_same<CT, double2>::value) {
+            d2h(y, dy.ptr, nx * sizeof(double2));
+        } else {
+            ZBuf ty(nx);
+            to_c128(dy.ptr, ty.ptr, (int64_t)nx, s);
+            NC_HIP(hipStreamSynchronize(s));
+            d2h(y, ty.ptr, nx * sizeof(double2));
+        }
         return 0;
     } catch (std::exception const& e) {
-        set_last_error(std::string("nc_bench_heff: ") + e.what());
-        fprintf(stderr, "[nutcracker-hip] nc_bench_heff: %s\n", e.what());
+        set_last_error(std::string(name) + ": " + e.what());
+        fprintf(stderr, "[nutcracker-hip] %s: %s\n", name, e.what());
         return -1;
     }
+}
+
+}  // namespace
+
+extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                             const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
+                             const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
+                             uint32_t reps, double* total_ms, double* phase_ms) {
+    return bench_heff<double2>("nc_bench_heff", bl, br, cl, cr, d, nmat, idx, mats, L, R, x, y, warmup, reps,
+                               total_ms, phase_ms);
+}
+
+extern "C" int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                                 const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
+                                 const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
+                                 uint32_t reps, double* total_ms, double* phase_ms) {
+    return bench_heff<float2>("nc_bench_heff_c64", bl, br, cl, cr, d, nmat, idx, mats, L, R, x, y, warmup, reps,
+                              total_ms, phase_ms);
 }
 
 // Site-move statistics: polar splits, Newton-Schulz iterations, SVD fallbacks.

@@ -71,6 +71,52 @@ void Chain::normalize_right_site(int i, hipStream_t s) {
     permute_102(Up.ptr, sites[i].ptr, bl, br, d, s);
 }
 
+namespace {
+void grow(DevBuf<float2>& b, size_t n) {
+    if (b.n < n) b.resize(n);
+}
+}  // namespace
+
+void Chain::update_left_env(int i, hipStream_t s) {
+    // absorb<Left>: L_{i+1} = contract_sos_left(L_i, site i)
+    MpoSite const& op = *ops[i];
+    const int bl = bonds[i], br = bonds[i + 1];
+    const size_t nout = (size_t)br * br * op.cr;
+    left_envs[i + 1].resize(nout);
+    if (!(opts.mixed && opts.env32)) {
+        env_left(bl, br, op, left_envs[i].ptr, sites[i].ptr, left_envs[i + 1].ptr, ws, s);
+        return;
+    }
+    const size_t nin = (size_t)bl * bl * op.cl, npsi = site_size(i);
+    grow(t32a, nin);
+    grow(t32b, npsi);
+    grow(t32c, nout);
+    to_c64(left_envs[i].ptr, t32a.ptr, nin, s);
+    to_c64(sites[i].ptr, t32b.ptr, npsi, s);
+    env_left(bl, br, op, t32a.ptr, t32b.ptr, t32c.ptr, ws, s);
+    to_c128(t32c.ptr, left_envs[i + 1].ptr, nout, s);
+}
+
+void Chain::update_right_env(int i, hipStream_t s) {
+    // absorb<Right>: R_i = contract_sos_right(R_{i+1}, site i)
+    MpoSite const& op = *ops[i];
+    const int bl = bonds[i], br = bonds[i + 1];
+    const size_t nout = (size_t)bl * bl * op.cl;
+    right_envs[i].resize(nout);
+    if (!(opts.mixed && opts.env32)) {
+        env_right(bl, br, op, right_envs[i + 1].ptr, sites[i].ptr, right_envs[i].ptr, ws, s);
+        return;
+    }
+    const size_t nin = (size_t)br * br * op.cr, npsi = site_size(i);
+    grow(t32a, nin);
+    grow(t32b, npsi);
+    grow(t32c, nout);
+    to_c64(right_envs[i + 1].ptr, t32a.ptr, nin, s);
+    to_c64(sites[i].ptr, t32b.ptr, npsi, s);
+    env_right(bl, br, op, t32a.ptr, t32b.ptr, t32c.ptr, ws, s);
+    to_c128(t32c.ptr, right_envs[i].ptr, nout, s);
+}
+
 void Chain::reset() {
     hipStream_t s = Context::get().stream;
     double t0 = now_ms();
@@ -83,9 +129,7 @@ void Chain::reset() {
     d2d(right_envs[N].ptr, trivial.ptr, sizeof(double2));
     for (int i = N - 1; i >= 1; --i) {
         normalize_right_site(i, s);
-        MpoSite const& op = *ops[i];
-        right_envs[i].resize((size_t)bonds[i] * bonds[i] * op.cl);
-        env_right(bonds[i], bonds[i + 1], op, right_envs[i + 1].ptr, sites[i].ptr, right_envs[i].ptr, ws, s);
+        update_right_env(i, s);
     }
     // first site: divided by its norm (core.f95:2661-2671)
     double nrm = dznrm2(sites[0].ptr, site_size(0), s);
@@ -95,7 +139,8 @@ void Chain::reset() {
     current = 0;
     optimized = false;
     zd e = site_expectation();
-    if (std::abs(e.imag()) / std::abs(e) > 1e-10) throw Error("InitialChainEnergyNotRealError", -62);
+    const double imag_tol = opts.mixed && opts.env32 ? 1e-6 : 1e-10;  // complex64 environments: FP32 round-off
+    if (std::abs(e.imag()) / std::abs(e) > imag_tol) throw Error("InitialChainEnergyNotRealError", -62);
     energy = e.real();
     stats.t_reset_ms += now_ms() - t0;
 }
@@ -122,10 +167,7 @@ void Chain::move_right() {
     d2d(sites[i + 1].ptr, dn.ptr, site_size(i + 1) * sizeof(double2));
     NC_HIP(hipStreamSynchronize(s));
     double t1 = now_ms();
-    // absorb<Left>: L_{i+1} = contract_sos_left(L_i, site i)
-    MpoSite const& op = *ops[i];
-    left_envs[i + 1].resize((size_t)bm * bm * op.cr);
-    env_left(bl, bm, op, left_envs[i].ptr, sites[i].ptr, left_envs[i + 1].ptr, ws, s);
+    update_left_env(i, s);
     NC_HIP(hipStreamSynchronize(s));
     double t2 = now_ms();
     stats.t_move_ms += t1 - t0;
@@ -151,10 +193,7 @@ void Chain::move_left() {
     d2d(sites[i].ptr, nr.ptr, site_size(i) * sizeof(double2));
     NC_HIP(hipStreamSynchronize(s));
     double t1 = now_ms();
-    // absorb<Right>: R_i = contract_sos_right(R_{i+1}, site i)
-    MpoSite const& op = *ops[i];
-    right_envs[i].resize((size_t)bm * bm * op.cl);
-    env_right(bm, br, op, right_envs[i + 1].ptr, sites[i].ptr, right_envs[i].ptr, ws, s);
+    update_right_env(i, s);
     NC_HIP(hipStreamSynchronize(s));
     double t2 = now_ms();
     stats.t_move_ms += t1 - t0;
@@ -176,10 +215,28 @@ int Chain::optimize_site() {
     none.orth = (int)site_size(i);
     OptimizeArgs a{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, &none,
                    opts.which, opts.site_tol, opts.max_iterations, sites[i].ptr, result.ptr, opts.ncv};
+    DevBuf<float2> L32, R32;
+    if (opts.mixed) {
+        const size_t nl = (size_t)bl * bl * op.cl, nr = (size_t)br * br * op.cr;
+        L32.resize(nl);
+        R32.resize(nr);
+        to_c64(left_envs[i].ptr, L32.ptr, nl, s);
+        to_c64(right_envs[i + 1].ptr, R32.ptr, nr, s);
+        a.L32 = L32.ptr;
+        a.R32 = R32.ptr;
+        a.mixed = true;
+        a.tol32 = opts.tol32;
+        a.polish = opts.polish;
+    }
     OptimizeOut o = nc::optimize_site(a, ws, s);
     stats.matvecs += o.matvecs;
+    stats.matvecs32 += o.matvecs32;
     ++stats.site_solves;
     int status = o.info;
+    // complex64 environments are Hermitian only to FP32 round-off and carry its
+    // noise into the energy: the complex-eigenvalue and regression checks and the
+    // sweep convergence test use that floor
+    const double floor_tol = opts.mixed && opts.env32 ? std::max(opts.sanity, opts.env32_floor) : opts.sanity;
     if (status == 0) {
         // optimizeStateSite post-checks (optimizer.cpp:265-300)
         zd ev = o.eigenvalue;
@@ -187,14 +244,14 @@ int Chain::optimize_site() {
         const double san = opts.sanity;
         if ((std::abs(ev) + std::abs(ex)) / 2 > san && std::abs(ev - ex) / (std::abs(ev) + std::abs(ex) + san) > san)
             status = 101;  // OptimizerObtainedEigenvalueDifferentFromExpectationValue
-        else if (std::abs(ev) > san && std::abs(ev.imag()) / std::abs(ev) > san)
+        else if (std::abs(ev) > san && std::abs(ev.imag()) / std::abs(ev) > floor_tol)
             status = 102;  // OptimizerObtainedComplexEigenvalue
         else if (o.normal < 1 - san)
             status = 103;  // OptimizerObtainedVanishingEigenvector
     }
     if (status == 0) {
         const double e = o.eigenvalue.real();
-        if (regression(opts.which, energy, e, opts.sanity)) {
+        if (regression(opts.which, energy, e, floor_tol)) {
             status = 104;  // OptimizerObtainedRegressiveEigenvalue
         } else {
             if ((energy >= 0 && e >= 0) || (energy <= 0 && e <= 0) ||
@@ -232,7 +289,8 @@ void Chain::sweep_until_converged() {
     double prev = energy;
     sweep();
     double cur = energy;
-    while (outside_tolerance(prev, cur, opts.sweep_tol)) {
+    const double tol = opts.mixed && opts.env32 ? std::max(opts.sweep_tol, opts.env32_floor) : opts.sweep_tol;
+    while (outside_tolerance(prev, cur, tol)) {
         sweep();
         prev = cur;
         cur = energy;

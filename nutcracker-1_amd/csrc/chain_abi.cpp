@@ -79,6 +79,17 @@ int nc_chain_set_options(nc_chain* c, double site_tol, double sweep_tol, double 
     });
 }
 
+int nc_chain_set_mixed(nc_chain* c, int mixed, int env32, double tol32, int polish) {
+    return guarded_chain("nc_chain_set_mixed", [&] {
+        auto& o = c->chain->opts;
+        o.mixed = mixed != 0;
+        o.env32 = env32 != 0;
+        o.tol32 = tol32;
+        o.polish = polish != 0;
+        return 0;
+    });
+}
+
 int nc_chain_set_raw_site(nc_chain* c, uint32_t i, const nc_zcomplex* data) {
     return guarded_chain("nc_chain_set_raw_site", [&] {
         if ((int)i >= c->chain->N) throw Error("site index out of range", -69);
@@ -144,6 +155,7 @@ int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
         counts[2] = s.moves;
         counts[3] = s.sweeps;
         counts[4] = s.failures;
+        counts[5] = s.matvecs32;
         times_ms[0] = s.t_opt_ms;
         times_ms[1] = s.t_move_ms;
         times_ms[2] = s.t_env_ms;

@@ -1,4 +1,5 @@
-// Grouped sparse-MPO contractions on the device (FP64 complex).
+// Grouped sparse-MPO contractions on the device (complex128 on FP64 matrix
+// cores, or complex64 on FP32 ones for the mixed-precision form).
 //
 // The reference builds a dense stage-1 tensor I(bl,d,cr,bl,d) and runs two
 // zgemms per matvec (core.f95:532-621).  Here the block list is contracted as
@@ -52,90 +53,145 @@ void MpoSite::build(int d_, int cl_, int cr_, int nmat_, const uint32_t* idx1, c
             }
             ptr[dst + 1] = (int)src.size();
         }
+        std::vector<float2> w32(w.size());
+        for (size_t i = 0; i < w.size(); ++i) w32[i] = make_float2((float)w[i].real(), (float)w[i].imag());
         mix_ptr[dir].resize(ptr.size());
         mix_src[dir].resize(src.size());
         mix_W[dir].resize(w.size());
+        mix_W32[dir].resize(w.size());
         h2d(mix_ptr[dir].ptr, ptr.data(), ptr.size() * sizeof(int));
         h2d(mix_src[dir].ptr, src.data(), src.size() * sizeof(int));
         h2d(mix_W[dir].ptr, w.data(), w.size() * sizeof(zd));
+        h2d(mix_W32[dir].ptr, w32.data(), w32.size() * sizeof(float2));
     }
 }
 
+namespace {
+
+template <typename CT>
+struct Plan;
+template <>
+struct Plan<double2> {
+    typedef GemmPlan type;
+    static void run(GemmPlan const& p, hipStream_t s) { zgemm_run(p, s); }
+};
+template <>
+struct Plan<float2> {
+    typedef CGemmPlan type;
+    static void run(CGemmPlan const& p, hipStream_t s) { cgemm_run(p, s); }
+};
+
 // Step A of the matvec / left environment: P_{f,a} = X_a L_f
-static void step_a_left(int bl, int br, MpoSite const& op, const double2* L, const double2* X,
-                        double2* P, hipStream_t s) {
+template <typename CT>
+void step_a_left(int bl, int br, MpoSite const& op, const CT* L, const CT* X, CT* P, hipStream_t s) {
     const int64_t rs = (int64_t)br * bl;
-    GemmPlan plan;
+    typename Plan<CT>::type plan;
     plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
     for (size_t fi = 0; fi < op.froms.size(); ++fi)
         for (int a = 0; a < op.d; ++a) {
             plan.begin_batch(P + ((int64_t)fi * op.d + a) * rs, br);
             plan.add_seg(X + a * rs, br, L + (int64_t)op.froms[fi] * bl * bl, bl, bl);
         }
-    zgemm_run(plan, s);
+    Plan<CT>::run(plan, s);
 }
 
-void heff_matvec(int bl, int br, MpoSite const& op, const double2* L, const double2* R,
-                 const double2* x, double2* y, Workspace& ws, hipStream_t s) {
+template <typename CT>
+void heff_matvec_t(int bl, int br, MpoSite const& op, const CT* L, const CT* R, const CT* x, CT* y, Workspace& ws,
+                   hipStream_t s) {
     const int64_t rs = (int64_t)br * bl;
     const int d = op.d;
-    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
-    step_a_left(bl, br, op, L, x, ws.P.ptr, s);
-    mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
-    GemmPlan plan;
+    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size(), (CT*)nullptr);
+    CT* P = ws.p((CT*)nullptr);
+    CT* Q = ws.q((CT*)nullptr);
+    step_a_left<CT>(bl, br, op, L, x, P, s);
+    mix_blocks(op, true, P, Q, rs, s);
+    typename Plan<CT>::type plan;
     plan.M = br; plan.N = bl * d; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
     plan.begin_batch(y, br);
     for (size_t ti = 0; ti < op.tos.size(); ++ti)
-        plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, ws.Q.ptr + (int64_t)ti * d * rs, br, br);
-    zgemm_run(plan, s);
+        plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, Q + (int64_t)ti * d * rs, br, br);
+    Plan<CT>::run(plan, s);
 }
 
-void env_left(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
-              double2* newL, Workspace& ws, hipStream_t s) {
+template <typename CT>
+void env_left_t(int bl, int br, MpoSite const& op, const CT* L, const CT* psi, CT* newL, Workspace& ws,
+                hipStream_t s) {
     const int64_t rs = (int64_t)br * bl;
     const int d = op.d;
-    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
-    step_a_left(bl, br, op, L, psi, ws.P.ptr, s);
-    mix_blocks(op, true, ws.P.ptr, ws.Q.ptr, rs, s);
-    if ((int)op.tos.size() < op.cr) dzero(newL, (size_t)br * br * op.cr * sizeof(double2));
-    GemmPlan plan;
+    ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size(), (CT*)nullptr);
+    CT* P = ws.p((CT*)nullptr);
+    CT* Q = ws.q((CT*)nullptr);
+    step_a_left<CT>(bl, br, op, L, psi, P, s);
+    mix_blocks(op, true, P, Q, rs, s);
+    if ((int)op.tos.size() < op.cr) dzero(newL, (size_t)br * br * op.cr * sizeof(CT));
+    typename Plan<CT>::type plan;
     plan.M = br; plan.N = br; plan.opA = OpN; plan.opB = OpC; plan.accumulate = false;
     for (size_t ti = 0; ti < op.tos.size(); ++ti) {
         plan.begin_batch(newL + (int64_t)op.tos[ti] * br * br, br);
-        plan.add_seg(ws.Q.ptr + (int64_t)ti * d * rs, br, psi, br, bl * d);
+        plan.add_seg(Q + (int64_t)ti * d * rs, br, psi, br, bl * d);
     }
-    zgemm_run(plan, s);
+    Plan<CT>::run(plan, s);
 }
 
-void env_right(int bl, int br, MpoSite const& op, const double2* R, const double2* psi,
-               double2* newR, Workspace& ws, hipStream_t s) {
+template <typename CT>
+void env_right_t(int bl, int br, MpoSite const& op, const CT* R, const CT* psi, CT* newR, Workspace& ws,
+                 hipStream_t s) {
     const int64_t rs = (int64_t)br * bl;
     const int d = op.d;
-    ws.ensure(rs * d * op.tos.size(), rs * d * op.froms.size());
+    ws.ensure(rs * d * op.tos.size(), rs * d * op.froms.size(), (CT*)nullptr);
+    CT* P = ws.p((CT*)nullptr);
+    CT* Q = ws.q((CT*)nullptr);
     {   // P'_{t,a} = R_t Psi_a
-        GemmPlan plan;
+        typename Plan<CT>::type plan;
         plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
         for (size_t ti = 0; ti < op.tos.size(); ++ti)
             for (int a = 0; a < d; ++a) {
-                plan.begin_batch(ws.P.ptr + ((int64_t)ti * d + a) * rs, br);
+                plan.begin_batch(P + ((int64_t)ti * d + a) * rs, br);
                 plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, psi + a * rs, br, br);
             }
-        zgemm_run(plan, s);
+        Plan<CT>::run(plan, s);
     }
-    mix_blocks(op, false, ws.P.ptr, ws.Q.ptr, rs, s);
-    if ((int)op.froms.size() < op.cl) dzero(newR, (size_t)bl * bl * op.cl * sizeof(double2));
-    GemmPlan plan;  // N_f = sum_b Psi_b^H Q'_{f,b}
+    mix_blocks(op, false, P, Q, rs, s);
+    if ((int)op.froms.size() < op.cl) dzero(newR, (size_t)bl * bl * op.cl * sizeof(CT));
+    typename Plan<CT>::type plan;  // N_f = sum_b Psi_b^H Q'_{f,b}
     plan.M = bl; plan.N = bl; plan.opA = OpC; plan.opB = OpN; plan.accumulate = false;
     for (size_t fi = 0; fi < op.froms.size(); ++fi) {
         plan.begin_batch(newR + (int64_t)op.froms[fi] * bl * bl, bl);
         for (int b = 0; b < d; ++b)
-            plan.add_seg(psi + b * rs, br, ws.Q.ptr + ((int64_t)fi * d + b) * rs, br, br);
+            plan.add_seg(psi + b * rs, br, Q + ((int64_t)fi * d + b) * rs, br, br);
     }
-    zgemm_run(plan, s);
+    Plan<CT>::run(plan, s);
 }
 
-zd expectation(int bl, int br, MpoSite const& op, const double2* L, const double2* psi,
-               const double2* R, Workspace& ws, hipStream_t s) {
+}  // namespace
+
+void heff_matvec(int bl, int br, MpoSite const& op, const double2* L, const double2* R, const double2* x, double2* y,
+                 Workspace& ws, hipStream_t s) {
+    heff_matvec_t<double2>(bl, br, op, L, R, x, y, ws, s);
+}
+void heff_matvec(int bl, int br, MpoSite const& op, const float2* L, const float2* R, const float2* x, float2* y,
+                 Workspace& ws, hipStream_t s) {
+    heff_matvec_t<float2>(bl, br, op, L, R, x, y, ws, s);
+}
+void env_left(int bl, int br, MpoSite const& op, const double2* L, const double2* psi, double2* newL, Workspace& ws,
+              hipStream_t s) {
+    env_left_t<double2>(bl, br, op, L, psi, newL, ws, s);
+}
+void env_left(int bl, int br, MpoSite const& op, const float2* L, const float2* psi, float2* newL, Workspace& ws,
+              hipStream_t s) {
+    env_left_t<float2>(bl, br, op, L, psi, newL, ws, s);
+}
+void env_right(int bl, int br, MpoSite const& op, const double2* R, const double2* psi, double2* newR, Workspace& ws,
+               hipStream_t s) {
+    env_right_t<double2>(bl, br, op, R, psi, newR, ws, s);
+}
+void env_right(int bl, int br, MpoSite const& op, const float2* R, const float2* psi, float2* newR, Workspace& ws,
+               hipStream_t s) {
+    env_right_t<float2>(bl, br, op, R, psi, newR, ws, s);
+}
+
+zd expectation(int bl, int br, MpoSite const& op, const double2* L, const double2* psi, const double2* R,
+               Workspace& ws, hipStream_t s) {
     ZBuf N((size_t)bl * bl * op.cl);
     env_right(bl, br, op, R, psi, N.ptr, ws, s);
     return trace_pair(N.ptr, L, bl, op.cl, s);

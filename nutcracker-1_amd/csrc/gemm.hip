@@ -388,7 +388,7 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
     if (g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
         while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
     }
-    const bool m3 = g_tuning.use_3m;
+    const bool m3 = sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32;
     Arena& ar = arena();
     if (nsplit == 1) {
         const void* parts[2] = {batches.data(), segs.data()};

@@ -21,23 +21,24 @@ struct MixDesc {
     int64_t rs;
 };
 
-template <int D>
-__global__ __launch_bounds__(256) void mix_kernel(const double2* __restrict__ P, double2* __restrict__ Q,
+template <typename CT, int D>
+__global__ __launch_bounds__(256) void mix_kernel(const CT* __restrict__ P, CT* __restrict__ Q,
                                                   const int* __restrict__ dst_ptr,
                                                   const int* __restrict__ blk_src,
-                                                  const double2* __restrict__ W, MixDesc md) {
+                                                  const CT* __restrict__ W, MixDesc md) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int dst = blockIdx.y;
     if (e >= md.rs) return;
     const int d = (D > 0) ? D : md.d;
-    double2 q[4] = {};
+    CT q[4];
+    for (int b = 0; b < 4; ++b) { q[b].x = 0; q[b].y = 0; }
     for (int t = dst_ptr[dst]; t < dst_ptr[dst + 1]; ++t) {
         const int src = blk_src[t];
-        const double2* Wn = W + (int64_t)t * d * d;
+        const CT* Wn = W + (int64_t)t * d * d;
         for (int a = 0; a < d; ++a) {
-            const double2 p = P[((int64_t)src * d + a) * md.rs + e];
+            const CT p = P[((int64_t)src * d + a) * md.rs + e];
             for (int b = 0; b < d; ++b) {
-                const double2 w = Wn[a + b * d];
+                const CT w = Wn[a + b * d];
                 q[b].x += w.x * p.x - w.y * p.y;
                 q[b].y += w.x * p.y + w.y * p.x;
             }
@@ -46,8 +47,9 @@ __global__ __launch_bounds__(256) void mix_kernel(const double2* __restrict__ P,
     for (int b = 0; b < d; ++b) Q[((int64_t)dst * d + b) * md.rs + e] = q[b];
 }
 
-void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q, int64_t rs,
-                hipStream_t s) {
+template <typename CT>
+static void mix_blocks_t(MpoSite const& op, bool src_is_from, const CT* P, CT* Q, int64_t rs, const CT* W,
+                         hipStream_t s) {
     if (op.d > 4) throw Error("mix_blocks: physical dimension above 4 is not supported");
     const int dir = src_is_from ? 0 : 1;
     const int ndst = (int)(src_is_from ? op.tos.size() : op.froms.size());
@@ -56,12 +58,39 @@ void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* 
     dim3 grid((unsigned)((rs + 255) / 256), (unsigned)ndst);
     const int* ptr = op.mix_ptr[dir].ptr;
     const int* srcs = op.mix_src[dir].ptr;
-    const double2* W = op.mix_W[dir].ptr;
     switch (op.d) {
-        case 2: hipLaunchKernelGGL(mix_kernel<2>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
-        case 3: hipLaunchKernelGGL(mix_kernel<3>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
-        default: hipLaunchKernelGGL(mix_kernel<0>, grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        case 2: hipLaunchKernelGGL((mix_kernel<CT, 2>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        case 3: hipLaunchKernelGGL((mix_kernel<CT, 3>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        default: hipLaunchKernelGGL((mix_kernel<CT, 0>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
     }
+    NC_HIP(hipGetLastError());
+}
+
+void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q, int64_t rs, hipStream_t s) {
+    mix_blocks_t<double2>(op, src_is_from, P, Q, rs, op.mix_W[src_is_from ? 0 : 1].ptr, s);
+}
+void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q, int64_t rs, hipStream_t s) {
+    mix_blocks_t<float2>(op, src_is_from, P, Q, rs, op.mix_W32[src_is_from ? 0 : 1].ptr, s);
+}
+
+__global__ void to_c64_kernel(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+__global__ void to_c128_kernel(const float2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+void to_c64(const double2* src, float2* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    unsigned g = (unsigned)std::min<int64_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(to_c64_kernel, dim3(g), dim3(256), 0, s, src, dst, n);
+    NC_HIP(hipGetLastError());
+}
+void to_c128(const float2* src, double2* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    unsigned g = (unsigned)std::min<int64_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(to_c128_kernel, dim3(g), dim3(256), 0, s, src, dst, n);
     NC_HIP(hipGetLastError());
 }
 

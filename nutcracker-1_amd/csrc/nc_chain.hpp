@@ -14,10 +14,18 @@ struct ChainOptions {
     char which[3] = {'S', 'R', 0};
     int ncv = 3;
     bool rethrow = false;  // raise on optimizer failures instead of skipping the site
+    // mixed-precision form: matvecs in complex64 on FP32 matrix cores (and, with
+    // env32, the environment updates too; environments stay complex128 in HBM),
+    // Krylov vectors, orthogonalisation, Rayleigh quotient and site move in FP64
+    bool mixed = false;
+    bool env32 = false;
+    double env32_floor = 1e-6;  // relative energy noise floor of complex64 environments
+    double tol32 = 1e-7;
+    bool polish = false;
 };
 
 struct ChainStats {
-    int64_t matvecs = 0, site_solves = 0, moves = 0, sweeps = 0, failures = 0;
+    int64_t matvecs = 0, site_solves = 0, moves = 0, sweeps = 0, failures = 0, matvecs32 = 0;
     double t_opt_ms = 0, t_move_ms = 0, t_env_ms = 0, t_reset_ms = 0;
 };
 
@@ -41,6 +49,10 @@ struct Chain {
     void set_raw_site(int i, const double2* host);
     void get_site(int i, double2* host) const;
     void normalize_right_site(int i, hipStream_t s);
+    // environment updates (FP64, or complex64 arithmetic in the mixed form)
+    void update_left_env(int i, hipStream_t s);
+    void update_right_env(int i, hipStream_t s);
+    DevBuf<float2> t32a, t32b, t32c;  // complex64 staging for the mixed form
     void reset();
     zd site_expectation();
     void move_right();

@@ -110,7 +110,10 @@ enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
 
 // Launch-time choices of the GEMM (see gemm.hip).
 struct GemmTuning {
-    bool use_3m = true;             // Gauss's three-multiplication complex product
+    // Gauss's three-multiplication complex product: faster for FP32 (matrix-core
+    // bound), slower for FP64 (measured on the MI355X), hence per precision
+    bool use_3m_f64 = false;
+    bool use_3m_f32 = true;
     bool split_k = true;            // split K when a launch has few output tiles
     int64_t split_k_tiles = 1024;   // ... fewer than this many (tiles x batches)
 };
@@ -178,6 +181,7 @@ struct MpoSite {
     // (dir 0: source = from, destination = to; dir 1: source = to, destination = from)
     DevBuf<int> mix_ptr[2], mix_src[2];
     DevBuf<double2> mix_W[2];
+    DevBuf<float2> mix_W32[2];  // the same blocks rounded to complex64
     void build(int d, int cl, int cr, int nmat, const uint32_t* idx1, const zd* mats);
 };
 
@@ -185,17 +189,42 @@ struct MpoSite {
 // where e runs over an element slab of size rs.  P has d * nsrc slabs, Q d * ndst.
 void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q,
                 int64_t rs, hipStream_t s);
+void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q,
+                int64_t rs, hipStream_t s);
+
+// complex128 <-> complex64 conversion
+void to_c64(const double2* src, float2* dst, int64_t n, hipStream_t s);
+void to_c128(const float2* src, double2* dst, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------------------
 // Contractions (FP64, grouped).  All pointers are device pointers.
 // ----------------------------------------------------------------------------
 struct Workspace {
     ZBuf P, Q;
+    DevBuf<float2> P32, Q32;
     void ensure(size_t nP, size_t nQ) {
         if (P.n < nP) P.resize(nP);
         if (Q.n < nQ) Q.resize(nQ);
     }
+    void ensure(size_t nP, size_t nQ, double2*) { ensure(nP, nQ); }
+    void ensure(size_t nP, size_t nQ, float2*) {
+        if (P32.n < nP) P32.resize(nP);
+        if (Q32.n < nQ) Q32.resize(nQ);
+    }
+    double2* p(double2*) { return P.ptr; }
+    double2* q(double2*) { return Q.ptr; }
+    float2* p(float2*) { return P32.ptr; }
+    float2* q(float2*) { return Q32.ptr; }
 };
+
+// The grouped contractions in complex64 on FP32 matrix cores (mixed precision
+// form): the same three steps as the FP64 functions below.
+void heff_matvec(int bl, int br, MpoSite const& op, const float2* L, const float2* R,
+                 const float2* x, float2* y, Workspace& ws, hipStream_t s);
+void env_left(int bl, int br, MpoSite const& op, const float2* L, const float2* psi,
+              float2* newL, Workspace& ws, hipStream_t s);
+void env_right(int bl, int br, MpoSite const& op, const float2* R, const float2* psi,
+               float2* newR, Workspace& ws, hipStream_t s);
 
 // y = H_eff x (no projectors), the product of core.f95's iteration_stage_1/2/3.
 void heff_matvec(int bl, int br, MpoSite const& op, const double2* L, const double2* R,
@@ -298,12 +327,22 @@ struct OptimizeArgs {
     const double2* guess;  // device, full space
     double2* result;       // device, full space
     int ncv = 3;
+    // mixed-precision form: the Krylov iteration applies H_eff in complex64 on
+    // FP32 matrix cores with these environment copies, to tolerance tol32; the
+    // eigenvalue is then the FP64 Rayleigh quotient of the result, and with
+    // `polish` the FP64 iteration continues from it to `tol`
+    const float2* L32 = nullptr;
+    const float2* R32 = nullptr;
+    bool mixed = false;
+    double tol32 = 1e-7;
+    bool polish = false;
 };
 struct OptimizeOut {
     int info = 0;
     int iterations = 0;
     int strategy = 0;
-    int64_t matvecs = 0;
+    int64_t matvecs = 0;    // FP64 operator applications
+    int64_t matvecs32 = 0;  // complex64 operator applications (mixed form)
     zd eigenvalue{0, 0};
     double normal = 0;
 };

@@ -185,6 +185,42 @@ struct HeffOperator : Operator {
         a.proj->project(full_out.ptr, y, tmp.ptr, s);
     }
 };
+
+// The same operator with the contraction in complex64 on FP32 matrix cores;
+// the vectors handed to the eigen-solver (and the projectors) stay FP64.
+struct Heff32Operator : Operator {
+    OptimizeArgs const& a;
+    Workspace& ws;
+    ZBuf full_in, full_out, tmp;
+    DevBuf<float2> x32, y32;
+    Heff32Operator(OptimizeArgs const& a_, Workspace& ws_) : a(a_), ws(ws_) {
+        const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+        n = a.proj->active() ? a.proj->orth : nfull;
+        x32.resize(nfull);
+        y32.resize(nfull);
+        if (a.proj->active()) {
+            full_in.resize(nfull);
+            full_out.resize(nfull);
+            tmp.resize(nfull);
+        }
+    }
+    void apply(const double2* x, double2* y, hipStream_t s) override {
+        const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+        const double2* xin = x;
+        if (a.proj->active()) {
+            a.proj->unproject(x, full_in.ptr, s);
+            xin = full_in.ptr;
+        }
+        to_c64(xin, x32.ptr, nfull, s);
+        heff_matvec(a.bl, a.br, *a.op, a.L32, a.R32, x32.ptr, y32.ptr, ws, s);
+        if (a.proj->active()) {
+            to_c128(y32.ptr, full_out.ptr, nfull, s);
+            a.proj->project(full_out.ptr, y, tmp.ptr, s);
+        } else {
+            to_c128(y32.ptr, y, nfull, s);
+        }
+    }
+};
 }  // namespace
 
 // strategy 1 (core.f95:2049-2129): dense projected H_eff and a Hermitian eigen-solve
@@ -204,6 +240,7 @@ static void strategy_1(OptimizeArgs const& a, HeffOperator& op, OptimizeOut& out
     }
     std::vector<double> w;
     std::vector<zd> V;
+    const std::vector<zd> Hfull = H;
     hermitian_eig(m, H, w, V);
     int k;
     const char* which = a.which;
@@ -215,7 +252,17 @@ static void strategy_1(OptimizeArgs const& a, HeffOperator& op, OptimizeOut& out
     for (int i = 0; i < m; ++i) hv[i] = make_double2(V[(size_t)i + (size_t)k * m].real(), V[(size_t)i + (size_t)k * m].imag());
     h2d(pv.ptr, hv.data(), (size_t)m * sizeof(double2));
     a.proj->unproject(pv.ptr, a.result, s);
-    out.eigenvalue = zd(w[k], 0.0);
+    // the eigen-solve reads one triangle; the Rayleigh quotient of the chosen
+    // vector against the whole matrix equals w[k] to round-off when H_eff is
+    // Hermitian, and stays consistent with the expectation value when it is only
+    // Hermitian to FP32 round-off (complex64 environments)
+    zd rq = 0;
+    for (int i = 0; i < m; ++i) {
+        zd row = 0;
+        for (int j = 0; j < m; ++j) row += Hfull[(size_t)i + (size_t)j * m] * V[(size_t)j + (size_t)k * m];
+        rq += std::conj(V[(size_t)i + (size_t)k * m]) * row;
+    }
+    out.eigenvalue = rq;
     out.info = 0;
     out.iterations = 0;
 }
@@ -242,10 +289,23 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
         out.strategy = (a.bl * a.br < a.cl * a.cr) ? 2 : 3;
         ZBuf pguess((size_t)orth), pres((size_t)orth), tmp((size_t)nfull);
         a.proj->project(a.guess, pguess.ptr, tmp.ptr, s);
+        const bool mixed = a.mixed && a.L32 && a.R32;
+        Heff32Operator* op32 = nullptr;
+        std::unique_ptr<Heff32Operator> op32_owner;
+        if (mixed) {
+            op32_owner.reset(new Heff32Operator(a, ws));
+            op32 = op32_owner.get();
+        }
+        Operator& kop = mixed ? static_cast<Operator&>(*op32) : static_cast<Operator&>(op);
+        const double ktol = mixed ? a.tol32 : a.tol;
+        auto count = [&](IramResult const& rr) {
+            if (mixed) out.matvecs32 += rr.matvecs;
+            else out.matvecs += rr.matvecs;
+        };
         int ncv = a.ncv;
         int iterations = a.max_iterations;
-        IramResult r = iram(op, ncv, a.which, a.tol, iterations, pguess.ptr, pres.ptr, s);
-        out.matvecs += r.matvecs;
+        IramResult r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
+        count(r);
         iterations = r.restarts;
         bool done = false;
         if (r.info == -14) {
@@ -255,11 +315,30 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
             } else {
                 while (ncv <= iterations) {
                     ncv = ncv * 3 / 2;
-                    r = iram(op, ncv, a.which, a.tol, iterations, pguess.ptr, pres.ptr, s);
-                    out.matvecs += r.matvecs;
+                    r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
+                    count(r);
                     iterations = r.restarts;
                     if (r.info != -14) break;
                 }
+            }
+        }
+        if (!done && mixed && r.info == 0) {
+            if (a.polish) {
+                // continue in FP64 from the complex64 solution
+                ZBuf start((size_t)orth);
+                d2d(start.ptr, pres.ptr, (size_t)orth * sizeof(double2));
+                IramResult r2 = iram(op, a.ncv, a.which, a.tol, a.max_iterations, start.ptr, pres.ptr, s);
+                out.matvecs += r2.matvecs;
+                if (r2.info == 0) r = r2;
+            } else {
+                // FP64 Rayleigh quotient of the complex64 solution (one FP64 matvec)
+                ZBuf y((size_t)orth);
+                op.apply(pres.ptr, y.ptr, s);
+                ++out.matvecs;
+                zd num;
+                zdotc_multi(pres.ptr, orth, 1, y.ptr, orth, &num, s);
+                const double nn = dznrm2(pres.ptr, orth, s);
+                r.eigenvalue = num / (nn * nn);
             }
         }
         if (!done) {

@@ -9,6 +9,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -72,18 +73,23 @@ namespace {
 PolarStats g_polar_stats;
 
 // Scaled Newton-Schulz iteration for the unitary polar factor of a wide
-// matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X,  with the
-// equioscillation scaling alpha = sqrt(3 / (1 + l + l^2)) for a lower bound l
-// of the smallest singular value (X is first divided by its Frobenius norm, so
-// every singular value lies in (0, 1]).  The scaled cubic maps [l, 1] onto
-// [p(l), 1] with p(l) ~ 2.6 l, so the number of steps is fixed by l alone.
+// matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X  for a lower
+// bound l of the smallest singular value (X is first divided by its Frobenius
+// norm, so every singular value lies in (0, 1]).  The cubic
+// p(x) = (alpha/2) x (3 - alpha^2 x^2) peaks at p(1/alpha) = 1 and maps [l, 1]
+// onto [min(p(l), p(1)), 1].  The equioscillation scaling
+// alpha = sqrt(3 / (1 + l + l^2)) makes p(l) = p(1) ~ 2.6 l, which for small l
+// sends the largest singular values down to the round-off level and loses
+// their singular vectors; alpha is therefore capped so that p(1) >= 0.1
+// (alpha = 1.698, growth 2.55 per step for small singular values instead of
+// 2.60), which keeps every direction at least a tenth of ||X|| in size.
 // A site after optimisation is often numerically rank deficient (Schmidt
 // values far below round-off); its polar factor is then not determined in
 // those directions, and the iteration would stall there.  A deterministic
-// perturbation of Frobenius norm 1e-12 ||A|| floors the singular values near
-// 1e-14 ||A||, the schedule starts from l = 1e-14 and converges in about 40
-// steps for any input; the perturbation moves the split between the two sites
-// by ~1e-12 relative and leaves the state itself unchanged (P = A U_p^H below).
+// perturbation of Frobenius norm 1e-14 ||A|| floors the singular values near
+// 1e-16 ||A||, the schedule starts from l = 1e-16 and converges in about 45
+// steps for any input; the two-site state changes by ~1e-14 relative
+// (P = A U_p^H below, kept Hermitian).
 // The Gram matrix of each step gives ||X X^H - I||_F for free.  Only GEMMs of
 // this library are used.
 bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStream_t s, int* iters_out) {
@@ -93,7 +99,7 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
     d2d(X0.ptr, A, (size_t)m * n * sizeof(double2));
     const int64_t total = (int64_t)m * n;
     // uniform entries in [-1,1) + i[-1,1) have RMS sqrt(2/3)
-    const double pscale = 1e-12 * nrm / std::sqrt((double)total) * std::sqrt(1.5);
+    const double pscale = 1e-14 * nrm / std::sqrt((double)total) * std::sqrt(1.5);
     {
         unsigned gsz = (unsigned)std::min<int64_t>(4096, (total + 255) / 256);
         hipLaunchKernelGGL(add_hash_noise_kernel, dim3(gsz), dim3(256), 0, s, X0.ptr, total, pscale);
@@ -101,8 +107,18 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
     }
     const double nrm2 = dznrm2(X0.ptr, total, s);
     zscal(X0.ptr, zd(1.0 / nrm2, 0), total, s);
-    double l = 1e-14;
-    const int max_iters = 60;
+    double l = 1e-16;
+    const int max_iters = 64;
+    // alpha_cap solves (a/2)(3 - a^2) = 0.1 on [1, sqrt(3)]
+    static const double alpha_cap = [] {
+        double lo = 1.0, hi = std::sqrt(3.0);
+        for (int k = 0; k < 80; ++k) {
+            const double mid = 0.5 * (lo + hi);
+            if (0.5 * mid * (3.0 - mid * mid) > 0.1) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }();
     bool last = false;
     for (int it = 0; it < max_iters; ++it) {
         GemmPlan g;  // G = X X^H
@@ -119,7 +135,7 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
             return true;
         }
         double alpha = 1.0;
-        if (l < 1.0 - 1e-3) alpha = std::sqrt(3.0 / (1.0 + l + l * l));
+        if (l < 1.0 - 1e-3) alpha = std::min(alpha_cap, std::sqrt(3.0 / (1.0 + l + l * l)));
         // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
         zscal(G.ptr, zd(-0.5 * alpha * alpha * alpha, 0), (int64_t)m * m, s);
         add_identity(G.ptr, m, zd(0.5 * alpha * (3.0 - alpha * alpha), 0), s);
@@ -129,7 +145,8 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
         u.add_seg(G.ptr, m, X0.ptr, m, m);
         zgemm_run(u, s);
         std::swap(X0.ptr, X1.ptr);
-        l = std::min(1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l));
+        const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
+        l = std::min({1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l), p1});
         // cubic convergence: once every |sigma^2 - 1| < 1e-7 one more step reaches round-off
         if (err < 1e-7) last = true;
     }

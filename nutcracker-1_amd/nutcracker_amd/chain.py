@@ -34,6 +34,7 @@ _CHAIN_SIGS = {
     "nc_chain_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "nc_chain_expectation": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "nc_chain_set_mixed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
 }
 
 
@@ -57,12 +58,17 @@ class Chain:
     """A chain of operator sites solved at a fixed bond dimension ``chi``.
 
     Parameters follow ChainOptions (chain_options.cpp:21-29); ``seed`` selects the
-    raw initial state (nutcracker_amd.fixtures), or pass ``raw_sites``."""
+    raw initial state (nutcracker_amd.fixtures), or pass ``raw_sites``.  ``mixed``
+    selects the mixed-precision form (complex64 matvecs, and with ``env32``
+    complex64 environment updates; FP64 Krylov vectors, Rayleigh quotient and site
+    moves; the Krylov iteration stops at ``tol32``, ``polish`` continues it in
+    FP64 to ``site_tol``)."""
 
     def __init__(self, ops: Sequence[OperatorSite], chi: int, seed: int = 0,
                  raw_sites: Optional[List[np.ndarray]] = None, site_tol: float = 1e-12,
                  sweep_tol: float = 1e-12, sanity: float = 1e-12, max_iterations: int = 10000,
-                 which: str = "SR", ncv: int = 3, rethrow: bool = False):
+                 which: str = "SR", ncv: int = 3, rethrow: bool = False, mixed: bool = False,
+                 env32: bool = False, tol32: float = 1e-7, polish: bool = False):
         self.ops = list(ops)
         self.n = len(self.ops)
         dims = [o.d for o in self.ops]
@@ -83,6 +89,7 @@ class Chain:
             raise RuntimeError(core.last_error())
         self._check(lib.nc_chain_set_options(self._h, site_tol, sweep_tol, sanity, max_iterations, which.encode(),
                                              ncv, 1 if rethrow else 0))
+        self.set_mixed(mixed, env32, tol32, polish)
         if raw_sites is None:
             raw_sites = raw_state(seed, dims, self.bonds)
         for i, a in enumerate(raw_sites):
@@ -112,6 +119,10 @@ class Chain:
             self.close()
         except Exception:
             pass
+
+    def set_mixed(self, mixed: bool, env32: bool = False, tol32: float = 1e-7, polish: bool = False):
+        self._check(_lib().nc_chain_set_mixed(self._h, 1 if mixed else 0, 1 if env32 else 0, tol32,
+                                              1 if polish else 0))
 
     # -- Chain methods -----------------------------------------------------
     def reset(self):
@@ -155,9 +166,9 @@ class Chain:
         return out
 
     def stats(self) -> dict:
-        counts = (C.c_int64 * 5)()
+        counts = (C.c_int64 * 6)()
         times = (C.c_double * 4)()
         self._check(_lib().nc_chain_stats(self._h, counts, times))
         return {"matvecs": counts[0], "site_solves": counts[1], "moves": counts[2], "sweeps": counts[3],
-                "failures": counts[4], "t_optimize_ms": times[0], "t_move_ms": times[1], "t_env_ms": times[2],
+                "failures": counts[4], "matvecs32": counts[5], "t_optimize_ms": times[0], "t_move_ms": times[1], "t_env_ms": times[2],
                 "t_reset_ms": times[3]}
