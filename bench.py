@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
-"""Throughput of the variational-MPS inner loop on one MI355X.
+"""Throughput of the variational-MPS inner loop on one MI355X, plain FP64 and
+mixed-precision forms.
 
-Workload: the bulk-site effective-Hamiltonian matvec of the XXZ chain
-(J = 1, Delta = 0.8, N = 100, d = 2, MPO bond 5) at bond dimension chi = 1024 --
-the operation ARPACK's znaupd loop applies inside every site optimisation
-(core.f95:2505-2513, 2529-2556).  One step = one H_eff application; inputs are
-resident in HBM before the timed region and the timing uses HIP events on the
-stream the kernels run on.
+Workload: the XXZ chain (J = 1, Delta = 0.8, N = 100, d = 2, MPO bond 5) at bond
+dimension chi = 1024.
 
-    python bench.py [--steps K] [--warmup W] [--chi 1024] [--cpu-matvecs 3] [--no-cpu]
+* Matvec: the bulk-site effective-Hamiltonian matvec that ARPACK's znaupd loop
+  applies inside every site optimisation (core.f95:2505-2513, 2529-2556).  One
+  step = one H_eff application in FP64 (the drop-in); the same matvec in
+  complex64 on FP32 matrix cores (the mixed form) is timed beside it.  Inputs are
+  resident in HBM before the timed region; timing uses HIP events on the stream
+  the kernels run on.
+* Sweep: two chains on the same seeded right-canonical initial state, one FP64
+  and one mixed (complex64 matvecs, FP64 environments, Krylov vectors, Rayleigh
+  quotient and site moves).  Each is walked to the middle of the chain and then
+  optimises a window of bulk sites (optimise + move right), timed on the host
+  around synchronised library calls.  Sweep seconds are EXTRAPOLATED as
+  2 (N - 1) x the mean bulk site step; |dE| is the difference of the two chains'
+  energies after each site solve of the window.
 
-Prints one JSON line.  The roofline object describes the dominant kernel (the
-grouped complex FP64 GEMM, two launches per matvec); cpu_baseline times the
-oracle's restatement of the reference's strategy-3 matvec (dense stage-1 tensor
-+ two zgemms through OpenBLAS) on a bounded sample on the host cores.
+    python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-cpu]
+
+Prints one JSON line.  The roofline object describes the dominant kernel of the
+FP64 path (the grouped complex FP64 GEMM, two launches per matvec);
+cpu_baseline times the oracle's restatement of the reference's strategy-3 matvec
+(dense stage-1 tensor + two zgemms through OpenBLAS) on a bounded sample on the
+host cores.
 """
 
 from __future__ import annotations
@@ -22,6 +34,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -32,7 +45,13 @@ sys.path.insert(0, os.path.join(ROOT, "nutcracker-1_amd"))
 
 METRIC = "matvecs/s + sweep sec at chi=1024, mixed vs FP64 on 1 GPU; |dE| vs FP64"
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cgemm_heff_traffic_chi1024.md")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def hermitian_env(rng, b, c):
@@ -46,6 +65,72 @@ def flops_matvec(bl, br, d, n_from, n_to, nblk):
     return 8 * d * bl * br * (n_from * bl + n_to * br) + 8 * nblk * bl * br * d * d
 
 
+def measured_traffic():
+    """HBM bytes per GEMM launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if present."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def bind(lib):
+    for name in ("nc_bench_heff", "nc_bench_heff_c64"):
+        f = getattr(lib, name)
+        f.restype = C.c_int
+        f.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)] + [C.c_void_p] * 5 + \
+            [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.nc_bench_copy.restype = C.c_int
+    lib.nc_bench_copy.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
+
+
+def time_matvec(core, lib, name, op, L, R, x, warmup, steps):
+    bl, br, d = L.shape[0], R.shape[0], op.d
+    y = np.zeros_like(x)
+    idx = np.ascontiguousarray(np.asarray(op.idx, np.uint32).reshape(-1, order="F"))
+    mats = np.ascontiguousarray(np.asarray(op.mats).reshape(-1, order="F"))
+    total = C.c_double(0)
+    phases = (C.c_double * 3)()
+    rc = getattr(lib, name)(bl, br, op.cl, op.cr, d, op.nmat, idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            mats.ctypes.data, L.ctypes.data, R.ctypes.data, x.ctypes.data, y.ctypes.data, warmup,
+                            steps, C.byref(total), phases)
+    if rc != 0:
+        raise RuntimeError(core.last_error())
+    return total.value / steps, list(phases), y
+
+
+def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
+    name = "mixed" if mixed else "fp64"
+    t0 = time.perf_counter()
+    ch = chain_mod.Chain(ops, chi, seed=seed, ncv=ncv, rethrow=True, mixed=mixed)
+    t_reset = time.perf_counter() - t0
+    log(name, "reset %.2f s" % t_reset)
+    t0 = time.perf_counter()
+    for _ in range(start):
+        ch.move(1)
+    t_walk = time.perf_counter() - t0
+    log(name, "walked to site %d in %.2f s" % (start, t_walk))
+    sites = []
+    for w in range(window):
+        s0 = ch.stats()
+        t0 = time.perf_counter()
+        ch.optimize_site()
+        t1 = time.perf_counter()
+        ch.move(1)
+        t2 = time.perf_counter()
+        s1 = ch.stats()
+        rec = {"site": start + w, "optimize_s": t1 - t0, "move_s": t2 - t1,
+               "matvecs_fp64": s1["matvecs"] - s0["matvecs"], "matvecs_c64": s1["matvecs32"] - s0["matvecs32"],
+               "energy": ch.energy}
+        sites.append(rec)
+        log(name, "site %d: optimize %.3f s, move %.3f s, matvecs fp64 %d c64 %d, E %.15f"
+            % (rec["site"], rec["optimize_s"], rec["move_s"], rec["matvecs_fp64"], rec["matvecs_c64"], rec["energy"]))
+    ch.close()
+    return {"reset_s": t_reset, "sites": sites}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,8 +138,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--delta", type=float, default=0.8)
+    ap.add_argument("--sites", type=int, default=100)
+    ap.add_argument("--window", type=int, default=3)
+    ap.add_argument("--window-start", type=int, default=48)
+    ap.add_argument("--ncv", type=int, default=16)
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy calibration kernel")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
 
@@ -64,72 +155,123 @@ def main():
         sys.exit(2)
 
     from nutcracker_amd import core
+    from nutcracker_amd import chain as chain_mod
     from nutcracker_amd.operators import xxz_operator
 
     lib = core.lib()
-    lib.nc_bench_heff.restype = C.c_int
-    lib.nc_bench_heff.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)] + [C.c_void_p] * 5 + \
-        [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-
+    bind(lib)
     dev = core.device_info()
+    log("device", dev)
     chi = args.chi
     d = 2
     op = xxz_operator(3, d, 1.0, args.delta)[1]  # a bulk site: w = 5, 8 blocks
     bl = br = chi
-    cl, cr = op.cl, op.cr
     rng = np.random.default_rng(args.seed)
-    L = hermitian_env(rng, bl, cl)
-    R = hermitian_env(rng, br, cr)
+    L = hermitian_env(rng, bl, op.cl)
+    R = hermitian_env(rng, br, op.cr)
     x = np.asfortranarray((rng.standard_normal((br, bl, d)) + 1j * rng.standard_normal((br, bl, d))) * np.sqrt(0.5))
     x /= np.linalg.norm(x)
-    y = np.zeros_like(x)
-    idx = np.ascontiguousarray(np.asarray(op.idx, np.uint32).reshape(-1, order="F"))
-    mats = np.ascontiguousarray(np.asarray(op.mats).reshape(-1, order="F"))
 
-    total = C.c_double(0)
-    phases = (C.c_double * 3)()
-    rc = lib.nc_bench_heff(bl, br, cl, cr, d, op.nmat, idx.ctypes.data_as(C.POINTER(C.c_uint32)), mats.ctypes.data,
-                           L.ctypes.data, R.ctypes.data, x.ctypes.data, y.ctypes.data, args.warmup, args.steps,
-                           C.byref(total), phases)
-    if rc != 0:
+    # ---- matvecs ---------------------------------------------------------
+    ms64, ph64, y64 = time_matvec(core, lib, "nc_bench_heff", op, L, R, x, args.warmup, args.steps)
+    log("fp64 matvec %.4f ms" % ms64, "phases", ph64)
+    ms32, ph32, y32 = time_matvec(core, lib, "nc_bench_heff_c64", op, L, R, x, args.warmup, args.steps)
+    log("complex64 matvec %.4f ms" % ms32, "phases", ph32)
+    err32 = float(np.linalg.norm((y32 - y64).ravel()) / np.linalg.norm(y64.ravel()))
+
+    copy_ms = C.c_double(0)
+    copy_bytes = args.copy_mib << 20
+    if lib.nc_bench_copy(copy_bytes, 10, C.byref(copy_ms)) != 0:
         raise RuntimeError(core.last_error())
-    ms_per_step = total.value / args.steps
-    value = 1000.0 / ms_per_step
+    copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
+    log("copy %d MiB: %.4f ms, %.0f GB/s" % (args.copy_mib, copy_ms.value, copy_gbs))
 
     n_from = len(set(int(v) for v in op.idx[0]))
     n_to = len(set(int(v) for v in op.idx[1]))
     f_mv = flops_matvec(bl, br, d, n_from, n_to, op.nmat)
     f_gemm_a = 8 * br * bl * bl * d * n_from
     f_gemm_c = 8 * br * br * bl * d * n_to
-    t_a, t_mix, t_c = phases[0], phases[1], phases[2]
-    gemm_avg_ms = (t_a + t_c) / 2
-    achieved = (f_gemm_a + f_gemm_c) / 2 / (gemm_avg_ms * 1e-3) / 1e12
+    f_gemm_launch = (f_gemm_a + f_gemm_c) / 2
+
+    def roof(ph, peak):
+        avg_ms = (ph[0] + ph[2]) / 2
+        ach = f_gemm_launch / (avg_ms * 1e-3) / 1e12
+        return avg_ms, ach, ach / peak
+
+    g64_ms, g64_tf, g64_frac = roof(ph64, FP64_MFMA_PEAK_TFLOPS)
+    g32_ms, g32_tf, g32_frac = roof(ph32, FP32_MFMA_PEAK_TFLOPS)
+    traffic = measured_traffic()
 
     result = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(1000.0 / ms64, 3),
         "unit": "matvecs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms64, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "XXZ chain J=1 Delta=%g N=100 d=2 MPO bond 5: bulk-site H_eff matvec at chi=%d"
-                               % (args.delta, chi),
-                   "chi": chi, "d": d, "mpo_bond": cl, "mpo_blocks": op.nmat, "form": "fp64 complex (plain drop-in)"},
-        "matvec_tflops": round(f_mv / (ms_per_step * 1e-3) / 1e12, 3),
-        "phase_ms": {"gemm_step_a": round(t_a, 4), "mix": round(t_mix, 4), "gemm_step_c": round(t_c, 4)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "zgemm_f64_kernel", "flops_per_launch": (f_gemm_a + f_gemm_c) // 2,
-                     "avg_launch_ms": round(gemm_avg_ms, 4)},
+        "config": {"workload": "XXZ chain J=1 Delta=%g N=%d d=2 MPO bond 5 at chi=%d: bulk-site H_eff matvec "
+                               "(value) and a bulk sweep window" % (args.delta, args.sites, chi),
+                   "chi": chi, "d": d, "mpo_bond": op.cl, "mpo_blocks": op.nmat,
+                   "form": "value = plain FP64 complex drop-in; 'mixed' = complex64 matvec on FP32 MFMA"},
+        "matvec_tflops": round(f_mv / (ms64 * 1e-3) / 1e12, 3),
+        "phase_ms": {"gemm_step_a": round(ph64[0], 4), "mix": round(ph64[1], 4), "gemm_step_c": round(ph64[2], 4)},
+        "roofline": {"bound": "mfma", "achieved": round(g64_tf, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(g64_frac, 4),
+                     "traffic": traffic, "kernel": "cgemm_kernel (complex128, FP64 MFMA)",
+                     "flops_per_launch": int(f_gemm_launch), "avg_launch_ms": round(g64_ms, 4)},
+        "mixed": {
+            "matvecs_per_s": round(1000.0 / ms32, 3), "ms_per_matvec": round(ms32, 4),
+            "matvec_speedup_vs_fp64": round(ms64 / ms32, 3),
+            "matvec_rel_err_vs_fp64": err32,
+            "phase_ms": {"gemm_step_a": round(ph32[0], 4), "mix": round(ph32[1], 4),
+                         "gemm_step_c": round(ph32[2], 4)},
+            "roofline": {"bound": "mfma", "achieved": round(g32_tf, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(g32_frac, 4),
+                         "kernel": "cgemm_kernel (complex64, FP32 MFMA)", "avg_launch_ms": round(g32_ms, 4)},
+        },
+        "copy_calibration": {"bytes": copy_bytes, "avg_ms": round(copy_ms.value, 4),
+                             "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},
         "device": dev,
     }
 
+    # ---- sweep window ----------------------------------------------------
+    if not args.no_sweep:
+        ops = xxz_operator(args.sites, d, 1.0, args.delta)
+        start = min(args.window_start, args.sites - 1 - args.window)
+        w64 = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=False)
+        wmx = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=True)
+
+        def step_s(w):
+            return float(np.mean([s["optimize_s"] + s["move_s"] for s in w["sites"]]))
+
+        n_steps = 2 * (args.sites - 1)
+        s64, smx = step_s(w64), step_s(wmx)
+        de = [abs(a["energy"] - b["energy"]) for a, b in zip(w64["sites"], wmx["sites"])]
+        result["sweep"] = {
+            "fp64_sweep_s": round(s64 * n_steps, 2),
+            "mixed_sweep_s": round(smx * n_steps, 2),
+            "mixed_speedup": round(s64 / smx, 3),
+            "max_abs_dE_per_site_solve": max(de),
+            "abs_dE_per_site_solve": de,
+            "max_abs_dE_energy_per_site": max(de) / args.sites,
+            "dE_note": "after each site solve of the window the two chains' total energies differ by dE; the "
+                       "first solve sees the same state in both forms, later ones follow slightly different "
+                       "trajectories (complex64 Krylov vectors converge to 1e-7)",
+            "extrapolated": True,
+            "how": "2(N-1)=%d site steps x mean (optimize + move) over sites %d-%d of the first sweep from the "
+                   "seeded right-canonical random state; ncv=%d, FP64 tol 1e-12, mixed Krylov tol 1e-7 + FP64 "
+                   "Rayleigh quotient" % (n_steps, start, start + args.window - 1, args.ncv),
+            "fp64": {"step_s": round(s64, 4), "reset_s": round(w64["reset_s"], 2), "sites": w64["sites"]},
+            "mixed": {"step_s": round(smx, 4), "reset_s": round(wmx["reset_s"], 2), "sites": wmx["sites"]},
+        }
+
+    # ---- CPU baseline ----------------------------------------------------
     if not args.no_cpu:
         sys.path.insert(0, ROOT)
         import oracle.core as oc  # CPU baseline only
@@ -140,22 +282,23 @@ def main():
         except Exception:
             threads = os.cpu_count()
         t0 = time.perf_counter()
-        stage1 = oc.iteration_stage_1(bl, cl, cr, d, L, op.idx, op.mats)
+        stage1 = oc.iteration_stage_1(bl, op.cl, op.cr, d, L, op.idx, op.mats)
         t_stage1 = time.perf_counter() - t0
         times = []
         yc = None
         for _ in range(max(1, args.cpu_matvecs)):
             t0 = time.perf_counter()
-            yc = oc.matvec(bl, br, cl, cr, d, L, op.idx, op.mats, R, x, stage1)
+            yc = oc.matvec(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, R, x, stage1)
             times.append(time.perf_counter() - t0)
         cpu_ms = float(np.median(times)) * 1e3
-        err = float(np.linalg.norm((y - yc).ravel()) / np.linalg.norm(yc.ravel()))
+        log("cpu oracle matvec %.1f ms (%d threads)" % (cpu_ms, threads))
+        err = float(np.linalg.norm((y64 - yc).ravel()) / np.linalg.norm(yc.ravel()))
         result["cpu_baseline"] = {
             "value": round(1000.0 / cpu_ms, 4), "unit": "matvecs/s", "cores": int(threads), "kind": "port",
             "sample": "%d oracle strategy-3 matvecs (dense stage-1 tensor built once in %.1f s, then 2 zgemms each, "
                       "OpenBLAS) at the same chi=%d bulk site; median %.1f ms" % (len(times), t_stage1, chi, cpu_ms),
         }
-        result["gpu_vs_cpu_matvec_speedup"] = round(cpu_ms / ms_per_step, 2)
+        result["gpu_vs_cpu_matvec_speedup"] = round(cpu_ms / ms64, 2)
         result["max_rel_err_vs_oracle"] = err
     print(json.dumps(result), flush=True)
 

@@ -180,6 +180,10 @@ int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32
                       const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
                       double* phase_ms);
 
+/* Calibration kernel for HBM byte counters: copies `bytes` (a multiple of 16)
+   between two device buffers `reps` times; *avg_ms is the mean launch time. */
+int nc_bench_copy(uint64_t bytes, uint32_t reps, double* avg_ms);
+
 /* Site-move statistics since load: out[0] polar splits, out[1] Newton-Schulz
    iterations, out[2] splits that fell back to the SVD (rank-deficient input). */
 int nc_polar_stats(int64_t* out);

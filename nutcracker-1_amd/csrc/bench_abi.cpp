@@ -4,6 +4,7 @@
 #include "nc_internal.hpp"
 #include "../../include/nutcracker_core_abi.h"
 
+#include <algorithm>
 #include <type_traits>
 
 using namespace nc;
@@ -122,7 +123,48 @@ int bench_heff(const char* name, uint32_t bl, uint32_t br, uint32_t cl, uint32_t
     }
 }
 
+// Plain 16-byte-per-lane copy: the calibration kernel for HBM byte counters
+// (reads and writes exactly `n16` * 16 bytes).
+__global__ __launch_bounds__(256) void nc_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+extern "C" int nc_bench_copy(uint64_t bytes, uint32_t reps, double* avg_ms) {
+    try {
+        if (bytes % 16 != 0 || bytes == 0) throw Error("copy size must be a positive multiple of 16 bytes", -2);
+        auto& c = Context::get();
+        hipStream_t s = c.stream;
+        const int64_t n16 = (int64_t)(bytes / 16);
+        DevBuf<uint4> a((size_t)n16), b((size_t)n16);
+        NC_HIP(hipMemsetAsync(a.ptr, 1, bytes, s));
+        const unsigned grid = (unsigned)std::min<int64_t>(8192, (n16 + 255) / 256);
+        hipLaunchKernelGGL(nc_copy_kernel, dim3(grid), dim3(256), 0, s, a.ptr, b.ptr, n16);  // warm-up
+        NC_HIP(hipGetLastError());
+        hipEvent_t e0, e1;
+        NC_HIP(hipEventCreate(&e0));
+        NC_HIP(hipEventCreate(&e1));
+        NC_HIP(hipEventRecord(e0, s));
+        for (uint32_t r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(nc_copy_kernel, dim3(grid), dim3(256), 0, s, a.ptr, b.ptr, n16);
+        NC_HIP(hipGetLastError());
+        NC_HIP(hipEventRecord(e1, s));
+        NC_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = reps ? ms / reps : 0.0;
+        NC_HIP(hipEventDestroy(e0));
+        NC_HIP(hipEventDestroy(e1));
+        return 0;
+    } catch (std::exception const& e) {
+        set_last_error(std::string("nc_bench_copy: ") + e.what());
+        fprintf(stderr, "[nutcracker-hip] nc_bench_copy: %s\n", e.what());
+        return -1;
+    }
+}
 
 extern "C" int nc_bench_heff(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
                              const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
