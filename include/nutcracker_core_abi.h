@@ -137,6 +137,34 @@ void contract_vs_right_(const uint32_t* b_left_old, const uint32_t* b_right_old,
                         const nc_zcomplex* normalized_projector_site_tensor,
                         const nc_zcomplex* normalized_state_site_tensor, nc_zcomplex* new_right_environment);
 
+/* core.cpp:517-525; core.f95:954-1005.  overlap_vector(b_right_new, b_left_new, d)
+   from left_environment(b_left_new, b_left_old), right_environment(b_right_old,
+   b_right_new) and the unnormalised overlap site (b_left_old, d, b_right_old). */
+void form_overlap_vector_(const uint32_t* b_left_old, const uint32_t* b_right_old, const uint32_t* b_left_new,
+                          const uint32_t* b_right_new, const uint32_t* d, const nc_zcomplex* left_environment,
+                          const nc_zcomplex* right_environment, const nc_zcomplex* unnormalized_projector_site_tensor,
+                          nc_zcomplex* overlap_vector);
+
+/* core.cpp:371-378; core.f95:1341-1435.  n = vector length, m = number of
+   vectors; vectors(n, m) is overwritten by its column-pivoted Householder QR,
+   coefficients and swaps have min(n, m) entries, *rank = number of non-null
+   reflectors. */
+void convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
+                                    nc_zcomplex* coefficients, uint32_t* swaps);
+
+/* core.cpp:498-504; core.f95:3299-3313.  overlap(bl, d, br) = conj(state(br, bl, d)). */
+void form_overlap_site_tensor_(const uint32_t* br, const uint32_t* bl, const uint32_t* d,
+                               const nc_zcomplex* state_site_tensor, nc_zcomplex* overlap_site_tensor);
+
+/* core.cpp:463-472; core.f95:3315-3373.  Left-normalises state 1 (bm, bl, dl) into
+   state 2 (br, bm, dr) and forms the three overlap sites.  (The reference's C++
+   declaration marks the four outputs const; they are written.) */
+void form_norm_overlap_tensors_(const uint32_t* bl, const uint32_t* bm, const uint32_t* br, const uint32_t* dl,
+                                const uint32_t* dr, const nc_zcomplex* unnormalized_state_tensor_1,
+                                const nc_zcomplex* right_norm_state_tensor_2, nc_zcomplex* left_norm_overlap_tensor_1,
+                                nc_zcomplex* unnormalized_overlap_tensor_1, nc_zcomplex* unnormalized_state_tensor_2,
+                                nc_zcomplex* right_norm_overlap_tensor_2);
+
 /* ---- entry points with no counterpart in the reference --------------------- */
 
 /* Message of the last failed call on this thread (the reference's C API keeps
@@ -213,6 +241,19 @@ int nc_chain_sweep(nc_chain* chain);
 int nc_chain_sweep_until_converged(nc_chain* chain);
 int nc_chain_energy(nc_chain* chain, double* energy, int* current_site);
 int nc_chain_expectation(nc_chain* chain, nc_zcomplex* out);
+/* Excited levels (chain.cpp:60-73, 240-263, 301-310 of the reference).  The
+   chain must be at its first site.  add_projector keeps the current state as a
+   lower level; after nc_chain_reset every site solve works in the space
+   orthogonal to the lower levels.  projector_overlap is the norm of the current
+   site's overlap with them.  randomize draws new raw site tensors from `seed`
+   (call nc_chain_reset afterwards).  solve_for_multiple_levels converges
+   `levels` levels in turn at the chain's bond dimension (new raw states from
+   seed + 1, seed + 2, ...) and writes their energies. */
+int nc_chain_add_projector(nc_chain* chain);
+int nc_chain_clear_projectors(nc_chain* chain);
+int nc_chain_projector_overlap(nc_chain* chain, double* overlap);
+int nc_chain_randomize(nc_chain* chain, uint64_t seed);
+int nc_chain_solve_for_multiple_levels(nc_chain* chain, uint32_t levels, uint64_t seed, double* energies);
 /* counts (6): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs;
    times_ms (4): optimize, site move, environment update, reset */
 int nc_chain_stats(nc_chain* chain, int64_t* counts, double* times_ms);

@@ -32,6 +32,7 @@ SOURCES = [
     "iram.cpp",
     "optimize.cpp",
     "polar.cpp",
+    "overlap.cpp",
     "chain.cpp",
     "abi.cpp",
     "chain_abi.cpp",

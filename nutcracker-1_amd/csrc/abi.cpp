@@ -461,17 +461,8 @@ void contract_vs_left_(const uint32_t* b_left_old, const uint32_t* b_right_old, 
         HostIn L(left_environment, (size_t)(bln * blo));
         HostIn P(normalized_projector_site_tensor, (size_t)(blo * dd * bro));
         HostIn S(normalized_state_site_tensor, (size_t)(brn * bln * dd));
-        ZBuf inter((size_t)(bln * dd * bro)), out(nout);
-        GemmPlan p1;  // inter(bln, d*bro) = L(bln, blo) P(blo, d*bro)
-        p1.M = (int)bln; p1.N = (int)(dd * bro);
-        p1.begin_batch(inter.ptr, bln);
-        p1.add_seg(L.ptr(), bln, P.ptr(), blo, (int)blo);
-        zgemm_run(p1, s);
-        GemmPlan p2;  // out(brn, bro) = S(brn, bln*d) inter(bln*d, bro)
-        p2.M = (int)brn; p2.N = (int)bro;
-        p2.begin_batch(out.ptr, brn);
-        p2.add_seg(S.ptr(), brn, inter.ptr, bln * dd, (int)(bln * dd));
-        zgemm_run(p2, s);
+        ZBuf out(nout);
+        vs_left(blo, bro, bln, brn, dd, L.ptr(), P.ptr(), S.ptr(), out.ptr, s);
         d2h(new_left_environment, out.ptr, nout * sizeof(double2));
         return 0;
     });
@@ -489,24 +480,99 @@ void contract_vs_right_(const uint32_t* b_left_old, const uint32_t* b_right_old,
         HostIn R(right_environment, (size_t)(bro * brn));
         HostIn P(normalized_projector_site_tensor, (size_t)(blo * dd * bro));
         HostIn S(normalized_state_site_tensor, (size_t)(brn * bln * dd));
-        ZBuf inter((size_t)(blo * dd * brn)), st((size_t)(dd * brn * bln)), out(nout);
-        GemmPlan p1;  // inter(blo*d, brn) = P(blo*d, bro) R(bro, brn)
-        p1.M = (int)(blo * dd); p1.N = (int)brn;
-        p1.begin_batch(inter.ptr, blo * dd);
-        p1.add_seg(P.ptr(), blo * dd, R.ptr(), bro, (int)bro);
-        zgemm_run(p1, s);
-        // st(d, brn, bln) = S(brn, bln, d) reshaped with order (2,3,1): st(s, r, l) = S(r, l, s),
-        // i.e. the transpose of S viewed as a (brn*bln) x d matrix
-        permute_102(S.ptr(), st.ptr, brn * bln, dd, 1, s);
-        GemmPlan p2;  // out(blo, bln) = inter(blo, d*brn) st(d*brn, bln)
-        p2.M = (int)blo; p2.N = (int)bln;
-        p2.begin_batch(out.ptr, blo);
-        p2.add_seg(inter.ptr, blo, st.ptr, dd * brn, (int)(dd * brn));
-        zgemm_run(p2, s);
+        ZBuf out(nout);
+        vs_right(blo, bro, bln, brn, dd, R.ptr(), P.ptr(), S.ptr(), out.ptr, s);
         d2h(new_right_environment, out.ptr, nout * sizeof(double2));
         return 0;
     });
     if (rc) fill_zero(new_right_environment, nout * sizeof(nc_zcomplex));
+}
+
+void form_overlap_vector_(const uint32_t* b_left_old, const uint32_t* b_right_old, const uint32_t* b_left_new,
+                          const uint32_t* b_right_new, const uint32_t* d, const nc_zcomplex* left_environment,
+                          const nc_zcomplex* right_environment, const nc_zcomplex* unnormalized_projector_site_tensor,
+                          nc_zcomplex* overlap_vector) {
+    const int64_t blo = *b_left_old, bro = *b_right_old, bln = *b_left_new, brn = *b_right_new, dd = *d;
+    const size_t nout = (size_t)(brn * bln * dd);
+    int rc = guarded("form_overlap_vector_", [&] {
+        auto s = Context::get().stream;
+        HostIn L(left_environment, (size_t)(bln * blo));
+        HostIn R(right_environment, (size_t)(bro * brn));
+        HostIn P(unnormalized_projector_site_tensor, (size_t)(blo * dd * bro));
+        ZBuf out(nout);
+        nc::overlap_vector(blo, bro, bln, brn, dd, L.ptr(), R.ptr(), P.ptr(), out.ptr, s);
+        d2h(overlap_vector, out.ptr, nout * sizeof(double2));
+        return 0;
+    });
+    if (rc) fill_zero(overlap_vector, nout * sizeof(nc_zcomplex));
+}
+
+void form_overlap_site_tensor_(const uint32_t* br, const uint32_t* bl, const uint32_t* d,
+                               const nc_zcomplex* state_site_tensor, nc_zcomplex* overlap_site_tensor) {
+    const size_t n = (size_t)*br * *bl * *d;
+    int rc = guarded("form_overlap_site_tensor_", [&] {
+        auto s = Context::get().stream;
+        HostIn S(state_site_tensor, n);
+        ZBuf out(std::max<size_t>(1, n));
+        overlap_site(*br, *bl, *d, S.ptr(), out.ptr, s);
+        d2h(overlap_site_tensor, out.ptr, n * sizeof(double2));
+        return 0;
+    });
+    if (rc) fill_zero(overlap_site_tensor, n * sizeof(nc_zcomplex));
+}
+
+void form_norm_overlap_tensors_(const uint32_t* bl, const uint32_t* bm, const uint32_t* br, const uint32_t* dl,
+                                const uint32_t* dr, const nc_zcomplex* unnormalized_state_tensor_1,
+                                const nc_zcomplex* right_norm_state_tensor_2, nc_zcomplex* left_norm_overlap_tensor_1,
+                                nc_zcomplex* unnormalized_overlap_tensor_1, nc_zcomplex* unnormalized_state_tensor_2,
+                                nc_zcomplex* right_norm_overlap_tensor_2) {
+    const size_t n1 = (size_t)*bm * *bl * *dl, n2 = (size_t)*br * *bm * *dr;
+    int rc = guarded("form_norm_overlap_tensors_", [&] {
+        auto s = Context::get().stream;
+        HostIn U1(unnormalized_state_tensor_1, n1);
+        HostIn R2(right_norm_state_tensor_2, n2);
+        ZBuf lo1(n1), uo1(n1), u2(n2), ro2(n2);
+        int info = norm_overlap_tensors((int)*bl, (int)*bm, (int)*br, (int)*dl, (int)*dr, U1.ptr(), R2.ptr(), lo1.ptr,
+                                        uo1.ptr, u2.ptr, ro2.ptr, s);
+        // the Fortran routine stops the process here ("Unable to normalize tensor.")
+        if (info != 0) throw Error("Unable to normalize tensor.", -55);
+        d2h(left_norm_overlap_tensor_1, lo1.ptr, n1 * sizeof(double2));
+        d2h(unnormalized_overlap_tensor_1, uo1.ptr, n1 * sizeof(double2));
+        d2h(unnormalized_state_tensor_2, u2.ptr, n2 * sizeof(double2));
+        d2h(right_norm_overlap_tensor_2, ro2.ptr, n2 * sizeof(double2));
+        return 0;
+    });
+    if (rc) {
+        fill_zero(left_norm_overlap_tensor_1, n1 * sizeof(nc_zcomplex));
+        fill_zero(unnormalized_overlap_tensor_1, n1 * sizeof(nc_zcomplex));
+        fill_zero(unnormalized_state_tensor_2, n2 * sizeof(nc_zcomplex));
+        fill_zero(right_norm_overlap_tensor_2, n2 * sizeof(nc_zcomplex));
+    }
+}
+
+void convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
+                                    nc_zcomplex* coefficients, uint32_t* swaps) {
+    // n: vector length (rows), m: number of vectors (the Fortran routine's m, n)
+    const int64_t rows = *n;
+    const int cols = (int)*m;
+    const int k = (int)std::min<int64_t>(rows, cols);
+    int rc = guarded("convert_vectors_to_reflectors_", [&] {
+        if (cols == 0) { *rank = 0; return 0; }
+        auto s = Context::get().stream;
+        HostIn V(vectors, (size_t)(rows * cols));
+        std::vector<zd> tau(k);
+        std::vector<uint32_t> sw(k);
+        int r = reflectors_from_vectors(rows, cols, V.ptr(), tau.data(), sw.data(), s);
+        d2h(vectors, V.ptr(), (size_t)(rows * cols) * sizeof(double2));
+        for (int i = 0; i < k; ++i) {
+            coefficients[i].re = tau[i].real();
+            coefficients[i].im = tau[i].imag();
+            swaps[i] = sw[i];
+        }
+        *rank = (uint32_t)r;
+        return 0;
+    });
+    if (rc) *rank = 0;
 }
 
 }  // extern "C"

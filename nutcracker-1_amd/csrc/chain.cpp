@@ -16,6 +16,7 @@
 #include "nc_internal.hpp"
 #include "nc_chain.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -127,9 +128,23 @@ void Chain::reset() {
     left_envs.resize(N + 1);
     right_envs[N].resize(1);
     d2d(right_envs[N].ptr, trivial.ptr, sizeof(double2));
+    const size_t np = projectors.size();
+    left_ovl.clear();
+    right_ovl.clear();
+    left_ovl.resize(np);
+    right_ovl.resize(np);
+    for (size_t p = 0; p < np; ++p) {
+        left_ovl[p].resize(N + 1);
+        right_ovl[p].resize(N + 1);
+        left_ovl[p][0].resize(1);
+        d2d(left_ovl[p][0].ptr, trivial.ptr, sizeof(double2));
+        right_ovl[p][N].resize(1);
+        d2d(right_ovl[p][N].ptr, trivial.ptr, sizeof(double2));
+    }
     for (int i = N - 1; i >= 1; --i) {
         normalize_right_site(i, s);
         update_right_env(i, s);
+        update_right_overlaps(i, s);
     }
     // first site: divided by its norm (core.f95:2661-2671)
     double nrm = dznrm2(sites[0].ptr, site_size(0), s);
@@ -138,6 +153,25 @@ void Chain::reset() {
     d2d(left_envs[0].ptr, trivial.ptr, sizeof(double2));
     current = 0;
     optimized = false;
+    reset_projector_matrix();
+    if (np > 0) {
+        // chain.cpp:218-225: find a site with room outside the lower levels,
+        // filter the state there, and come back to the first site
+        while (site_projector.orth == 0) {
+            if (current + 1 >= N) throw Error("no site has room outside the lower levels at this bond dimension", -70);
+            move_right();
+        }
+        const int64_t n = (int64_t)site_size(current);
+        ZBuf proj_x((size_t)site_projector.orth), tmp((size_t)n), full((size_t)n);
+        site_projector.project(sites[current].ptr, proj_x.ptr, tmp.ptr, s);
+        site_projector.unproject(proj_x.ptr, full.ptr, s);
+        const double fn = dznrm2(full.ptr, n, s);
+        if (!(fn > 1e-7)) throw Error("the initial state lies in the space of the lower levels", -71);
+        zscal(full.ptr, zd(1.0 / fn, 0), n, s);
+        d2d(sites[current].ptr, full.ptr, (size_t)n * sizeof(double2));
+        while (current > 0) move_left();
+        optimized = false;
+    }
     zd e = site_expectation();
     const double imag_tol = opts.mixed && opts.env32 ? 1e-6 : 1e-10;  // complex64 environments: FP32 round-off
     if (std::abs(e.imag()) / std::abs(e) > imag_tol) throw Error("InitialChainEnergyNotRealError", -62);
@@ -168,6 +202,7 @@ void Chain::move_right() {
     NC_HIP(hipStreamSynchronize(s));
     double t1 = now_ms();
     update_left_env(i, s);
+    update_left_overlaps(i, s);
     NC_HIP(hipStreamSynchronize(s));
     double t2 = now_ms();
     stats.t_move_ms += t1 - t0;
@@ -175,6 +210,7 @@ void Chain::move_right() {
     ++stats.moves;
     current = i + 1;
     optimized = false;
+    reset_projector_matrix();
 }
 
 void Chain::move_left() {
@@ -194,6 +230,7 @@ void Chain::move_left() {
     NC_HIP(hipStreamSynchronize(s));
     double t1 = now_ms();
     update_right_env(i, s);
+    update_right_overlaps(i, s);
     NC_HIP(hipStreamSynchronize(s));
     double t2 = now_ms();
     stats.t_move_ms += t1 - t0;
@@ -201,6 +238,7 @@ void Chain::move_left() {
     ++stats.moves;
     current = i - 1;
     optimized = false;
+    reset_projector_matrix();
 }
 
 int Chain::optimize_site() {
@@ -213,7 +251,8 @@ int Chain::optimize_site() {
     Projector none;
     none.n = (int)site_size(i);
     none.orth = (int)site_size(i);
-    OptimizeArgs a{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, &none,
+    Projector const* proj = projectors.empty() ? &none : &site_projector;
+    OptimizeArgs a{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, proj,
                    opts.which, opts.site_tol, opts.max_iterations, sites[i].ptr, result.ptr, opts.ncv};
     DevBuf<float2> L32, R32;
     if (opts.mixed) {
@@ -248,6 +287,11 @@ int Chain::optimize_site() {
             status = 102;  // OptimizerObtainedComplexEigenvalue
         else if (o.normal < 1 - san)
             status = 103;  // OptimizerObtainedVanishingEigenvector
+        else if (proj->active()) {
+            ZBuf tmp(site_size(i));
+            if (proj->overlap(result.ptr, tmp.ptr, s) > san)
+                status = 105;  // OptimizerObtainedEigenvectorInProjectorSpace
+        }
     }
     if (status == 0) {
         const double e = o.eigenvalue.real();
@@ -267,6 +311,146 @@ int Chain::optimize_site() {
     if (status != 0) ++stats.failures;
     last_status = status;
     return status;
+}
+
+// ---------------------------------------------------------------------------
+// Excited levels (chain.cpp:60-73, 240-263, 301-310; projectors.hpp:324-485)
+// ---------------------------------------------------------------------------
+void Chain::update_left_overlaps(int i, hipStream_t s) {
+    // absorb<Left> (chain.hpp:248-279): the lower level's left-canonical site
+    for (size_t p = 0; p < projectors.size(); ++p) {
+        LevelProjector const& pr = projectors[p];
+        const int blo = pr.bonds[i], bro = pr.bonds[i + 1], bln = bonds[i], brn = bonds[i + 1];
+        left_ovl[p][i + 1].resize((size_t)brn * bro);
+        vs_left(blo, bro, bln, brn, ops[i]->d, left_ovl[p][i].ptr, pr.left[i].ptr, sites[i].ptr,
+                left_ovl[p][i + 1].ptr, s);
+    }
+}
+
+void Chain::update_right_overlaps(int i, hipStream_t s) {
+    // absorb<Right>: the lower level's right-canonical site
+    for (size_t p = 0; p < projectors.size(); ++p) {
+        LevelProjector const& pr = projectors[p];
+        const int blo = pr.bonds[i], bro = pr.bonds[i + 1], bln = bonds[i], brn = bonds[i + 1];
+        right_ovl[p][i].resize((size_t)blo * bln);
+        vs_right(blo, bro, bln, brn, ops[i]->d, right_ovl[p][i + 1].ptr, pr.right[i].ptr, sites[i].ptr,
+                 right_ovl[p][i].ptr, s);
+    }
+}
+
+void Chain::reset_projector_matrix() {
+    // formProjectorMatrix (projectors.hpp:402-485): one overlap vector per lower
+    // level at the current site, turned into Householder reflectors
+    const size_t np = projectors.size();
+    if (np == 0) return;
+    hipStream_t s = Context::get().stream;
+    const int i = current;
+    const int64_t n = (int64_t)site_size(i);
+    ZBuf V((size_t)n * np);
+    for (size_t p = 0; p < np; ++p) {
+        LevelProjector const& pr = projectors[p];
+        overlap_vector(pr.bonds[i], pr.bonds[i + 1], bonds[i], bonds[i + 1], ops[i]->d, left_ovl[p][i].ptr,
+                       right_ovl[p][i + 1].ptr, pr.middle[i].ptr, V.ptr + (size_t)p * n, s);
+    }
+    const int k = (int)std::min<int64_t>(n, (int64_t)np);
+    std::vector<zd> tau(k);
+    std::vector<uint32_t> sw(k);
+    const int rank = reflectors_from_vectors(n, (int)np, V.ptr, tau.data(), sw.data(), s);
+    site_projector.set_device((int)n, (int)np, k, (int)(n - rank), V.ptr, tau.data(), sw.data());
+}
+
+void Chain::add_projector_from_state() {
+    // constructAndAddProjectorFromState (chain.cpp:63-73) via
+    // computeProjectorFromStateSites (projectors.hpp:324-359)
+    if (current != 0) throw Error("ChainNotAtFirstSiteError", -72);
+    hipStream_t s = Context::get().stream;
+    LevelProjector pr;
+    pr.bonds = bonds;
+    pr.left.resize(N);
+    pr.middle.resize(N);
+    pr.right.resize(N);
+    ZBuf cur(site_size(0));
+    d2d(cur.ptr, sites[0].ptr, site_size(0) * sizeof(double2));
+    for (int i = 0; i + 1 < N; ++i) {
+        const int bl = bonds[i], bm = bonds[i + 1], br = bonds[i + 2];
+        pr.left[i].resize(site_size(i));
+        pr.middle[i].resize(site_size(i));
+        pr.right[i + 1].resize(site_size(i + 1));
+        ZBuf next(site_size(i + 1));
+        int info = norm_overlap_tensors(bl, bm, br, ops[i]->d, ops[i + 1]->d, cur.ptr, sites[i + 1].ptr,
+                                        pr.left[i].ptr, pr.middle[i].ptr, next.ptr, pr.right[i + 1].ptr, s);
+        if (info != 0) throw Error("NormalizationError", info);
+        cur = std::move(next);
+    }
+    pr.middle[N - 1].resize(site_size(N - 1));
+    overlap_site(bonds[N], bonds[N - 1], ops[N - 1]->d, cur.ptr, pr.middle[N - 1].ptr, s);
+    NC_HIP(hipStreamSynchronize(s));
+    projectors.push_back(std::move(pr));
+    optimized = false;
+}
+
+void Chain::clear_projectors() {
+    projectors.clear();
+    left_ovl.clear();
+    right_ovl.clear();
+}
+
+double Chain::projector_overlap() {
+    // computeProjectorOverlapAtCurrentSite (chain.cpp:60-62)
+    if (projectors.empty()) return 0.0;
+    hipStream_t s = Context::get().stream;
+    ZBuf tmp(site_size(current));
+    return site_projector.overlap(sites[current].ptr, tmp.ptr, s);
+}
+
+namespace {
+// uniform entries in [-1, 1) + i [-1, 1) from a counter-based hash
+__global__ void random_site_kernel(double2* x, int64_t n, uint64_t key) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = key ^ ((uint64_t)e * 0x9E3779B97F4A7C15ull);
+        double v[2];
+        for (int h = 0; h < 2; ++h) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t t = z;
+            t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+            t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+            t ^= t >> 31;
+            v[h] = (double)(t >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        }
+        x[e] = make_double2(v[0], v[1]);
+    }
+}
+}  // namespace
+
+void Chain::randomize(uint64_t seed) {
+    hipStream_t s = Context::get().stream;
+    for (int i = 0; i < N; ++i) {
+        const int64_t n = (int64_t)site_size(i);
+        const uint64_t key = (seed + 1) * 0xD1B54A32D192ED03ull ^ ((uint64_t)(i + 1) * 0x8CB92BA72F3D8DD7ull);
+        const unsigned g = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+        hipLaunchKernelGGL(random_site_kernel, dim3(g), dim3(256), 0, s, sites[i].ptr, n, key);
+        NC_HIP(hipGetLastError());
+    }
+    NC_HIP(hipStreamSynchronize(s));
+}
+
+std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed) {
+    // solveForMultipleLevels (chain.cpp:301-310) at the chain's fixed bond
+    // dimension: converge, keep the state as a projector, restart from a new
+    // random state orthogonal to the lower levels
+    std::vector<double> energies;
+    for (int k = 0; k + 1 < levels; ++k) {
+        sweep_until_converged();
+        energies.push_back(energy);
+        add_projector_from_state();
+        randomize(seed + (uint64_t)k + 1);
+        reset();
+    }
+    if (levels > 0) {
+        sweep_until_converged();
+        energies.push_back(energy);
+    }
+    return energies;
 }
 
 void Chain::sweep() {

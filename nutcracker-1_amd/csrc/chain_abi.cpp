@@ -147,6 +147,30 @@ int nc_chain_expectation(nc_chain* c, nc_zcomplex* out) {
     });
 }
 
+int nc_chain_add_projector(nc_chain* c) {
+    return guarded_chain("nc_chain_add_projector", [&] { c->chain->add_projector_from_state(); return 0; });
+}
+
+int nc_chain_clear_projectors(nc_chain* c) {
+    return guarded_chain("nc_chain_clear_projectors", [&] { c->chain->clear_projectors(); return 0; });
+}
+
+int nc_chain_projector_overlap(nc_chain* c, double* overlap) {
+    return guarded_chain("nc_chain_projector_overlap", [&] { *overlap = c->chain->projector_overlap(); return 0; });
+}
+
+int nc_chain_randomize(nc_chain* c, uint64_t seed) {
+    return guarded_chain("nc_chain_randomize", [&] { c->chain->randomize(seed); return 0; });
+}
+
+int nc_chain_solve_for_multiple_levels(nc_chain* c, uint32_t levels, uint64_t seed, double* energies) {
+    return guarded_chain("nc_chain_solve_for_multiple_levels", [&] {
+        std::vector<double> e = c->chain->solve_for_multiple_levels((int)levels, seed);
+        for (size_t k = 0; k < e.size(); ++k) energies[k] = e[k];
+        return 0;
+    });
+}
+
 int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
     return guarded_chain("nc_chain_stats", [&] {
         auto const& s = c->chain->stats;

@@ -29,6 +29,15 @@ struct ChainStats {
     double t_opt_ms = 0, t_move_ms = 0, t_env_ms = 0, t_reset_ms = 0;
 };
 
+// A lower level kept for excited-state solves (projectors.hpp:324-359): the
+// overlap sites (conjugated, (bl, d, br)) of the level's state in left-canonical
+// (left[i], i < N-1), centred (middle[i]) and right-canonical (right[i], i > 0)
+// form.
+struct LevelProjector {
+    std::vector<ZBuf> left, middle, right;
+    std::vector<int> bonds;
+};
+
 struct Chain {
     int N;
     std::vector<std::unique_ptr<MpoSite>> ops;
@@ -53,6 +62,20 @@ struct Chain {
     void update_left_env(int i, hipStream_t s);
     void update_right_env(int i, hipStream_t s);
     DevBuf<float2> t32a, t32b, t32c;  // complex64 staging for the mixed form
+    // excited levels: lower levels, their overlap boundaries with the current
+    // state (left_ovl[p][i]: sites < i, right_ovl[p][i]: sites >= i) and the
+    // projector matrix of the current site (chain.cpp:240-263)
+    std::vector<LevelProjector> projectors;
+    std::vector<std::vector<ZBuf>> left_ovl, right_ovl;
+    Projector site_projector;
+    void update_left_overlaps(int i, hipStream_t s);
+    void update_right_overlaps(int i, hipStream_t s);
+    void reset_projector_matrix();
+    void add_projector_from_state();
+    void clear_projectors();
+    double projector_overlap();
+    void randomize(uint64_t seed);
+    std::vector<double> solve_for_multiple_levels(int levels, uint64_t seed);
     void reset();
     zd site_expectation();
     void move_right();

@@ -272,19 +272,40 @@ zd trace_pair(const double2* A, const double2* B, int64_t b, int64_t c, hipStrea
 // ----------------------------------------------------------------------------
 // Projectors: Householder reflectors (LAPACK storage) applied to vectors.
 // ----------------------------------------------------------------------------
+struct ProjectorDevice;
 struct Projector {
     int n = 0, nproj = 0, nrefl = 0, orth = 0;
     std::vector<uint32_t> swaps;  // nrefl, 1-based
     std::vector<zd> coeffs;       // nrefl
     DevBuf<double2> d_refl;       // n * nproj
+    std::shared_ptr<ProjectorDevice> dev;  // explicit reflector vectors
     bool active() const { return nproj > 0; }
+    // reflectors from the host (refl) or already on the device (d_refl_in)
     void set(int n, int nproj, int nrefl, int orth, const zd* refl, const zd* coeffs,
              const uint32_t* swaps);
+    void set_device(int n, int nproj, int nrefl, int orth, const double2* d_refl_in, const zd* coeffs,
+                    const uint32_t* swaps);
     // full (n) -> orthogonal (orth), and back
     void project(const double2* full, double2* out, double2* tmp, hipStream_t s) const;
     void unproject(const double2* in, double2* full, hipStream_t s) const;
     double overlap(const double2* v, double2* tmp, hipStream_t s) const;
+
+  private:
+    void finish(const std::vector<zd>& rtop, hipStream_t s);
 };
+
+// Overlap contractions and projector formation (overlap.cpp)
+void vs_left(int64_t blo, int64_t bro, int64_t bln, int64_t brn, int64_t d, const double2* L, const double2* P,
+             const double2* S, double2* out, hipStream_t s);
+void vs_right(int64_t blo, int64_t bro, int64_t bln, int64_t brn, int64_t d, const double2* R, const double2* P,
+              const double2* S, double2* out, hipStream_t s);
+void overlap_vector(int64_t blo, int64_t bro, int64_t bln, int64_t brn, int64_t d, const double2* L,
+                    const double2* R, const double2* P, double2* out, hipStream_t s);
+void overlap_site(int64_t br, int64_t bl, int64_t d, const double2* S, double2* out, hipStream_t s);
+int norm_overlap_tensors(int bl, int bm, int br, int dl, int dr, const double2* unnorm1, const double2* rightnorm2,
+                         double2* left_ov1, double2* unnorm_ov1, double2* unnorm2, double2* right_ov2,
+                         hipStream_t s);
+int reflectors_from_vectors(int64_t m, int n, double2* V, zd* tau, uint32_t* swaps, hipStream_t s);
 
 // ----------------------------------------------------------------------------
 // Eigen-solver (implicitly restarted Arnoldi, nev = 1) with device vectors.

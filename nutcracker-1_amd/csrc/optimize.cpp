@@ -43,20 +43,15 @@ __global__ void build_reflectors_kernel(const double2* refl, double2* V, double2
     }
 }
 
-namespace {
 struct ProjectorDevice {
     ZBuf V, Vc;           // nrefl explicit reflectors
     DevBuf<uint32_t> swaps;
     std::vector<zd> Rtop;  // nrefl x nrefl upper block of the stored reflectors
 };
-}  // namespace
-
-static thread_local std::vector<std::pair<Projector const*, std::shared_ptr<ProjectorDevice>>> g_proj_cache;
 
 static ProjectorDevice& proj_dev(Projector const& p) {
-    for (auto& kv : g_proj_cache)
-        if (kv.first == &p) return *kv.second;
-    throw Error("projector used before set()", -30);
+    if (!p.dev) throw Error("projector used before set()", -30);
+    return *p.dev;
 }
 
 void Projector::set(int n_, int nproj_, int nrefl_, int orth_, const zd* refl, const zd* coeffs_,
@@ -65,28 +60,47 @@ void Projector::set(int n_, int nproj_, int nrefl_, int orth_, const zd* refl, c
     if (nrefl > nproj || nrefl > n) throw Error("projector: more reflectors than vectors", -31);
     swaps.assign(swaps_, swaps_ + nrefl);
     coeffs.assign(coeffs_, coeffs_ + nrefl);
-    for (auto s : swaps)
-        if (s < 1 || (int)s > nrefl) throw Error("projector: swap index out of range", -32);
+    for (auto sw : swaps)
+        if (sw < 1 || (int)sw > nrefl) throw Error("projector: swap index out of range", -32);
     d_refl.resize((size_t)n * std::max(1, nproj));
     if (nproj) h2d(d_refl.ptr, refl, (size_t)n * nproj * sizeof(double2));
-    auto dev = std::make_shared<ProjectorDevice>();
-    if (nrefl) {
-        dev->V.resize((size_t)n * nrefl);
-        dev->Vc.resize((size_t)n * nrefl);
-        hipStream_t s = Context::get().stream;
-        unsigned g = (unsigned)std::min<int64_t>(4096, ((int64_t)n * nrefl + 255) / 256);
-        hipLaunchKernelGGL(build_reflectors_kernel, dim3(g), dim3(256), 0, s, d_refl.ptr, dev->V.ptr, dev->Vc.ptr,
-                           (int64_t)n, nrefl);
-        NC_HIP(hipGetLastError());
-        dev->swaps.resize(nrefl);
-        h2d(dev->swaps.ptr, swaps.data(), nrefl * sizeof(uint32_t));
-        dev->Rtop.resize((size_t)nrefl * nrefl);
-        for (int j = 0; j < nrefl; ++j)
-            for (int i = 0; i < nrefl; ++i) dev->Rtop[(size_t)i + (size_t)j * nrefl] = refl[(size_t)i + (size_t)j * n];
+    std::vector<zd> rtop((size_t)nrefl * nrefl);
+    for (int j = 0; j < nrefl; ++j)
+        for (int i = 0; i < nrefl; ++i) rtop[(size_t)i + (size_t)j * nrefl] = refl[(size_t)i + (size_t)j * n];
+    finish(rtop, Context::get().stream);
+}
+
+void Projector::set_device(int n_, int nproj_, int nrefl_, int orth_, const double2* d_refl_in, const zd* coeffs_,
+                           const uint32_t* swaps_) {
+    n = n_; nproj = nproj_; nrefl = nrefl_; orth = orth_;
+    if (nrefl > nproj || nrefl > n) throw Error("projector: more reflectors than vectors", -31);
+    swaps.assign(swaps_, swaps_ + nrefl);
+    coeffs.assign(coeffs_, coeffs_ + nrefl);
+    for (auto sw : swaps)
+        if (sw < 1 || (int)sw > nrefl) throw Error("projector: swap index out of range", -32);
+    d_refl.resize((size_t)n * std::max(1, nproj));
+    if (nproj) d2d(d_refl.ptr, d_refl_in, (size_t)n * nproj * sizeof(double2));
+    std::vector<zd> rtop((size_t)nrefl * nrefl);
+    for (int j = 0; j < nrefl; ++j) {
+        std::vector<double2> col(nrefl);
+        d2h(col.data(), d_refl.ptr + (size_t)j * n, (size_t)nrefl * sizeof(double2));
+        for (int i = 0; i < nrefl; ++i) rtop[(size_t)i + (size_t)j * nrefl] = zd(col[i].x, col[i].y);
     }
-    for (auto it = g_proj_cache.begin(); it != g_proj_cache.end(); ++it)
-        if (it->first == this) { g_proj_cache.erase(it); break; }
-    g_proj_cache.push_back({this, dev});
+    finish(rtop, Context::get().stream);
+}
+
+void Projector::finish(const std::vector<zd>& rtop, hipStream_t s) {
+    dev = std::make_shared<ProjectorDevice>();
+    if (!nrefl) return;
+    dev->V.resize((size_t)n * nrefl);
+    dev->Vc.resize((size_t)n * nrefl);
+    unsigned g = (unsigned)std::min<int64_t>(4096, ((int64_t)n * nrefl + 255) / 256);
+    hipLaunchKernelGGL(build_reflectors_kernel, dim3(g), dim3(256), 0, s, d_refl.ptr, dev->V.ptr, dev->Vc.ptr,
+                       (int64_t)n, nrefl);
+    NC_HIP(hipGetLastError());
+    dev->swaps.resize(nrefl);
+    h2d(dev->swaps.ptr, swaps.data(), nrefl * sizeof(uint32_t));
+    dev->Rtop = rtop;
 }
 
 // x <- x H_0 H_1 ... (forward, zunmqr 'R','N') or x <- x H_{k-1}^H ... H_0^H ('R','C')

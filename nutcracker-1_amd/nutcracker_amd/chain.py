@@ -35,6 +35,11 @@ _CHAIN_SIGS = {
     "nc_chain_expectation": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "nc_chain_set_mixed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
+    "nc_chain_add_projector": (C.c_int, [C.c_void_p]),
+    "nc_chain_clear_projectors": (C.c_int, [C.c_void_p]),
+    "nc_chain_projector_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "nc_chain_randomize": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "nc_chain_solve_for_multiple_levels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_double)]),
 }
 
 
@@ -164,6 +169,32 @@ class Chain:
         out = np.zeros((self.bonds[i + 1], self.bonds[i], d), CPLX, order="F")
         self._check(_lib().nc_chain_get_site(self._h, i, out.ctypes.data))
         return out
+
+    # -- excited levels (chain.cpp:60-73, 301-310) ----------------------------
+    def construct_and_add_projector_from_state(self):
+        """Keep the current state (chain at its first site) as a lower level."""
+        self._check(_lib().nc_chain_add_projector(self._h))
+
+    def clear_projectors(self):
+        self._check(_lib().nc_chain_clear_projectors(self._h))
+
+    def compute_projector_overlap_at_current_site(self) -> float:
+        v = C.c_double(0)
+        self._check(_lib().nc_chain_projector_overlap(self._h, C.byref(v)))
+        return v.value
+
+    def randomize(self, seed: int):
+        """New raw site tensors from ``seed``; call reset() afterwards."""
+        self._check(_lib().nc_chain_randomize(self._h, seed))
+
+    def solve_for_multiple_levels(self, levels: int, seed: int = 1) -> List[float]:
+        """Converge ``levels`` levels in turn at the chain's bond dimension, each
+        orthogonal to the ones before; returns their energies."""
+        out = (C.c_double * max(1, levels))()
+        self._check(_lib().nc_chain_solve_for_multiple_levels(self._h, levels, seed, out))
+        return [out[i] for i in range(levels)]
+
+    solve_for_eigenvalues = solve_for_multiple_levels
 
     def stats(self) -> dict:
         counts = (C.c_int64 * 6)()
