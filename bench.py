@@ -17,7 +17,8 @@ dimension chi = 1024.
   optimises a window of bulk sites (optimise + move right), timed on the host
   around synchronised library calls.  Sweep seconds are EXTRAPOLATED as
   2 (N - 1) x the mean bulk site step; |dE| is the difference of the two chains'
-  energies after each site solve of the window.
+  energies after each site solve of the window, and the state infidelity
+  1 - |<psi_mixed|psi_FP64>|^2 is taken after the window.
 
     python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-cpu]
 
@@ -127,8 +128,7 @@ def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
         sites.append(rec)
         log(name, "site %d: optimize %.3f s, move %.3f s, matvecs fp64 %d c64 %d, E %.15f"
             % (rec["site"], rec["optimize_s"], rec["move_s"], rec["matvecs_fp64"], rec["matvecs_c64"], rec["energy"]))
-    ch.close()
-    return {"reset_s": t_reset, "sites": sites}
+    return ch, {"reset_s": t_reset, "sites": sites}
 
 
 def main():
@@ -244,8 +244,12 @@ def main():
     if not args.no_sweep:
         ops = xxz_operator(args.sites, d, 1.0, args.delta)
         start = min(args.window_start, args.sites - 1 - args.window)
-        w64 = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=False)
-        wmx = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=True)
+        c64, w64 = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=False)
+        cmx, wmx = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=True)
+        infidelity = 1.0 - abs(c64.compute_state_overlap(cmx)) ** 2
+        log("state infidelity after the window %.3e" % infidelity)
+        c64.close()
+        cmx.close()
 
         def step_s(w):
             return float(np.mean([s["optimize_s"] + s["move_s"] for s in w["sites"]]))
@@ -260,6 +264,7 @@ def main():
             "max_abs_dE_per_site_solve": max(de),
             "abs_dE_per_site_solve": de,
             "max_abs_dE_energy_per_site": max(de) / args.sites,
+            "state_infidelity": infidelity,
             "dE_note": "after each site solve of the window the two chains' total energies differ by dE; the "
                        "first solve sees the same state in both forms, later ones follow slightly different "
                        "trajectories (complex64 Krylov vectors converge to 1e-7)",

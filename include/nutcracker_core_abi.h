@@ -254,6 +254,9 @@ int nc_chain_clear_projectors(nc_chain* chain);
 int nc_chain_projector_overlap(nc_chain* chain, double* overlap);
 int nc_chain_randomize(nc_chain* chain, uint64_t seed);
 int nc_chain_solve_for_multiple_levels(nc_chain* chain, uint32_t levels, uint64_t seed, double* energies);
+/* <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382): the
+   chains need the same length and physical dimensions, not the same bonds. */
+int nc_chain_state_overlap(nc_chain* a, nc_chain* b, nc_zcomplex* overlap);
 /* counts (6): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs;
    times_ms (4): optimize, site move, environment update, reset */
 int nc_chain_stats(nc_chain* chain, int64_t* counts, double* times_ms);

@@ -453,6 +453,30 @@ std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed) 
     return energies;
 }
 
+zd state_overlap(Chain const& a, Chain const& b) {
+    // computeStateOverlap (states.hpp:349-382): <a|b>, contracted from the left
+    // with the overlap sites (conjugates) of a; every site of either chain holds
+    // its part of the state whatever the position of its middle site
+    if (a.N != b.N) throw Error("state overlap: chains of different lengths", -73);
+    hipStream_t s = Context::get().stream;
+    ZBuf L(1), ov, next;
+    double2 one = make_double2(1, 0);
+    h2d(L.ptr, &one, sizeof(double2));
+    for (int i = 0; i < a.N; ++i) {
+        const int d = a.ops[i]->d;
+        if (b.ops[i]->d != d) throw Error("state overlap: physical dimensions differ", -74);
+        const int blo = a.bonds[i], bro = a.bonds[i + 1], bln = b.bonds[i], brn = b.bonds[i + 1];
+        ov.resize((size_t)blo * d * bro);
+        overlap_site(bro, blo, d, a.sites[i].ptr, ov.ptr, s);
+        next.resize((size_t)brn * bro);
+        vs_left(blo, bro, bln, brn, d, L.ptr, ov.ptr, b.sites[i].ptr, next.ptr, s);
+        std::swap(L, next);
+    }
+    double2 r;
+    d2h(&r, L.ptr, sizeof(double2));
+    return zd(r.x, r.y);
+}
+
 void Chain::sweep() {
     // chain.cpp:147-163
     const int start = current;

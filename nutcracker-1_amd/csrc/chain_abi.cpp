@@ -171,6 +171,15 @@ int nc_chain_solve_for_multiple_levels(nc_chain* c, uint32_t levels, uint64_t se
     });
 }
 
+int nc_chain_state_overlap(nc_chain* a, nc_chain* b, nc_zcomplex* out) {
+    return guarded_chain("nc_chain_state_overlap", [&] {
+        zd v = state_overlap(*a->chain, *b->chain);
+        out->re = v.real();
+        out->im = v.imag();
+        return 0;
+    });
+}
+
 int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
     return guarded_chain("nc_chain_stats", [&] {
         auto const& s = c->chain->stats;

@@ -85,4 +85,7 @@ struct Chain {
     void sweep_until_converged();
 };
 
+// <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382)
+zd state_overlap(Chain const& a, Chain const& b);
+
 }  // namespace nc

@@ -36,6 +36,7 @@ _CHAIN_SIGS = {
     "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "nc_chain_set_mixed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
     "nc_chain_add_projector": (C.c_int, [C.c_void_p]),
+    "nc_chain_state_overlap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nc_chain_clear_projectors": (C.c_int, [C.c_void_p]),
     "nc_chain_projector_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "nc_chain_randomize": (C.c_int, [C.c_void_p, C.c_uint64]),
@@ -195,6 +196,12 @@ class Chain:
         return [out[i] for i in range(levels)]
 
     solve_for_eigenvalues = solve_for_multiple_levels
+
+    def compute_state_overlap(self, other: "Chain") -> complex:
+        """<self|other> (computeStateOverlap, states.hpp:349-382)."""
+        out = np.zeros(1, CPLX)
+        self._check(_lib().nc_chain_state_overlap(self._h, other._h, out.ctypes.data))
+        return complex(out[0])
 
     def stats(self) -> dict:
         counts = (C.c_int64 * 6)()

@@ -217,3 +217,43 @@ def test_xxz_levels_match_exact_diagonalisation(mods):
     got = ch.solve_for_multiple_levels(4, seed=21)
     print("levels", got, "exact", exact, flush=True)
     assert np.allclose(got, exact, atol=1e-9, rtol=0)
+
+
+def _dense_state(ch):
+    """The chain's state as a dense vector (site tensors (br, bl, d))."""
+    v = np.ones((1, 1), complex)  # (configurations, right bond)
+    for i in range(ch.n):
+        A = ch.site(i)  # (br, bl, d)
+        v = np.einsum("cl,rls->csr", v, A).reshape(-1, A.shape[0])
+    return v[:, 0]
+
+
+def test_state_overlap_brute_force(mods):
+    """computeStateOverlap (states.hpp:349-382) against the dense states; <a|a> = 1."""
+    chain, ops_mod = mods
+    ops = ops_mod.xxz_operator(6, 2, 1.0, 0.8)
+    a = chain.Chain(ops, 4, seed=1, rethrow=True)
+    b = chain.Chain(ops, 3, seed=2, rethrow=True)
+    b.perform_optimization_sweep()
+    va, vb = _dense_state(a), _dense_state(b)
+    got = a.compute_state_overlap(b)
+    assert abs(got - np.vdot(va, vb)) < 1e-12
+    assert abs(a.compute_state_overlap(a) - 1) < 1e-12
+    assert abs(b.compute_state_overlap(b) - 1) < 1e-12
+
+
+CONFIG1_LEVELS = [-20.400217867027, -20.400216436515, -19.373376103501, -19.373374672990]
+
+
+def test_config1_four_levels(mods):
+    """Config 1: H = -sum ZZ - 0.5 sum X, N = 20, chi = 16, site tolerance 1e-12,
+    four levels through projectors, against the free-fermion closed form
+    (pinned in test_oracle.py).  chi = 16 truncates the chain; the levels agree to
+    6e-11 (measured), the bound is 1e-9 absolute."""
+    chain, ops_mod = mods
+    ops = ops_mod.tfim_zz_x_operator(20, 0.5)
+    ch = chain.Chain(ops, 16, seed=20, rethrow=True, ncv=8)
+    got = ch.solve_for_multiple_levels(4, seed=40)
+    err = np.abs(np.array(got) - CONFIG1_LEVELS)
+    print("config 1 levels", got, "abs err", err, "stats", ch.stats(), flush=True)
+    assert err.max() < 1e-9
