@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy calibration kernel")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
@@ -239,6 +240,30 @@ def main():
                              "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},
         "device": dev,
     }
+
+    # ---- the matvec at the other configurations' bulk sites ---------------
+    if not args.no_configs:
+        others = {}
+        for name, (cop, cchi, reps) in {
+            "config2_xxz_delta1_chi256": (xxz_operator(3, 2, 1.0, 1.0)[1], 256, 50),
+            "config5_spin1_heisenberg_chi2048": (xxz_operator(3, 3, 1.0, 1.0)[1], 2048, 5),
+        }.items():
+            crng = np.random.default_rng(args.seed + cchi)
+            cL = hermitian_env(crng, cchi, cop.cl)
+            cR = hermitian_env(crng, cchi, cop.cr)
+            cx = np.asfortranarray((crng.standard_normal((cchi, cchi, cop.d)) +
+                                    1j * crng.standard_normal((cchi, cchi, cop.d))) * np.sqrt(0.5))
+            nf = len(set(int(v) for v in cop.idx[0]))
+            nt = len(set(int(v) for v in cop.idx[1]))
+            fl = flops_matvec(cchi, cchi, cop.d, nf, nt, cop.nmat)
+            m64, _, _ = time_matvec(core, lib, "nc_bench_heff", cop, cL, cR, cx, 2, reps)
+            m32, _, _ = time_matvec(core, lib, "nc_bench_heff_c64", cop, cL, cR, cx, 2, reps)
+            others[name] = {"chi": cchi, "d": cop.d, "flops_per_matvec": fl,
+                            "fp64_ms": round(m64, 4), "fp64_tflops": round(fl / (m64 * 1e-3) / 1e12, 2),
+                            "c64_ms": round(m32, 4), "c64_tflops": round(fl / (m32 * 1e-3) / 1e12, 2)}
+            log(name, others[name])
+            del cL, cR, cx
+        result["other_configs_matvec"] = others
 
     # ---- sweep window ----------------------------------------------------
     if not args.no_sweep:

@@ -241,6 +241,15 @@ int nc_chain_sweep(nc_chain* chain);
 int nc_chain_sweep_until_converged(nc_chain* chain);
 int nc_chain_energy(nc_chain* chain, double* energy, int* current_site);
 int nc_chain_expectation(nc_chain* chain, nc_zcomplex* out);
+/* Bond dimension growth.  set_growth: optimizeChain (base_chain.cpp:64-79)
+   grows chi by one after each convergence until the energy changes by less than
+   chain_tol (grow = 0: fixed chi).  increase_bandwidth: increaseBandwidthDimension
+   (chain.cpp:75-121) at the first site, the state unchanged.  get_bonds writes
+   the N + 1 current bond dimensions. */
+int nc_chain_set_growth(nc_chain* chain, int grow, double chain_tol);
+int nc_chain_increase_bandwidth(nc_chain* chain, uint32_t new_chi);
+int nc_chain_optimize_chain(nc_chain* chain);
+int nc_chain_get_bonds(nc_chain* chain, uint32_t* bonds);
 /* Excited levels (chain.cpp:60-73, 240-263, 301-310 of the reference).  The
    chain must be at its first site.  add_projector keeps the current state as a
    lower level; after nc_chain_reset every site solve works in the space

@@ -41,9 +41,62 @@ bool regression(const char* which, double from, double to, double tol) {
 }
 }  // namespace
 
+std::vector<int> bandwidth_sequence(int requested, std::vector<int> const& dims) {
+    // computeBandwidthDimensionSequence (utilities.cpp:32-91): 1, d1, d1 d2, ...
+    // capped at `requested`, from both ends
+    if (requested <= 0) throw Error("requested bandwidth must be greater than zero", -75);
+    const int n = (int)dims.size();
+    const int middle = (n + 1) / 2;
+    auto side = [&](bool reverse) {
+        std::vector<int> out{1};
+        uint64_t p = 1;
+        for (int k = 0; k < middle; ++k) {
+            p *= (uint64_t)dims[reverse ? n - 1 - k : k];
+            if (p >= (uint64_t)requested) break;
+            out.push_back((int)p);
+        }
+        return out;
+    };
+    std::vector<int> fwd = side(false), rev = side(true);
+    if ((int)fwd.size() == middle + 1 || (int)rev.size() == middle + 1)
+        throw Error("RequestedBandwidthDimensionTooLargeError", -76);
+    const int fill = n + 1 - (int)fwd.size() - (int)rev.size();
+    for (int k = 0; k < fill; ++k) fwd.push_back(requested);
+    for (int k = (int)rev.size() - 1; k >= 0; --k) fwd.push_back(rev[k]);
+    return fwd;
+}
+
+int maximum_bandwidth(std::vector<int> const& dims) {
+    // maximumBandwidthDimension (utilities.hpp:264-283)
+    const int n = (int)dims.size();
+    if (n == 1) return 1;
+    const int middle = (n + 1) / 2;
+    uint64_t a = 1, b = 1;
+    const uint64_t cap = 0xFFFFFFFFull;
+    for (int k = 0; k < middle; ++k) {
+        a = std::min(cap, a * (uint64_t)dims[k]);
+        b = std::min(cap, b * (uint64_t)dims[n - 1 - k]);
+    }
+    return (int)std::min<uint64_t>({a, b, (uint64_t)0x7FFFFFFF});
+}
+
+int minimum_bandwidth_for_projectors(std::vector<int> const& dims, int nproj) {
+    // minimumBandwidthDimensionForProjectorCount (projectors.cpp:153-172)
+    if (nproj == 0) return 1;
+    const int maxb = maximum_bandwidth(dims);
+    for (int b = 1; b <= maxb; ++b) {
+        std::vector<int> seq = bandwidth_sequence(b, dims);
+        for (size_t i = 0; i < dims.size(); ++i)
+            if ((int64_t)dims[i] * seq[i] * seq[i + 1] > nproj) return b;
+    }
+    return maxb;
+}
+
 Chain::Chain(int n_sites, std::vector<std::unique_ptr<MpoSite>> ops_, std::vector<int> bonds_)
     : N(n_sites), ops(std::move(ops_)), bonds(std::move(bonds_)) {
     if ((int)ops.size() != N || (int)bonds.size() != N + 1) throw Error("chain: inconsistent sizes", -60);
+    chi = *std::max_element(bonds.begin(), bonds.end());
+    initial_chi = chi;
     sites.resize(N);
     for (int i = 0; i < N; ++i) sites[i].resize(site_size(i));
     trivial.resize(1);
@@ -403,51 +456,136 @@ double Chain::projector_overlap() {
     return site_projector.overlap(sites[current].ptr, tmp.ptr, s);
 }
 
-namespace {
-// uniform entries in [-1, 1) + i [-1, 1) from a counter-based hash
-__global__ void random_site_kernel(double2* x, int64_t n, uint64_t key) {
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t z = key ^ ((uint64_t)e * 0x9E3779B97F4A7C15ull);
-        double v[2];
-        for (int h = 0; h < 2; ++h) {
-            z += 0x9E3779B97F4A7C15ull;
-            uint64_t t = z;
-            t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
-            t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
-            t ^= t >> 31;
-            v[h] = (double)(t >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        }
-        x[e] = make_double2(v[0], v[1]);
-    }
-}
-}  // namespace
-
 void Chain::randomize(uint64_t seed) {
     hipStream_t s = Context::get().stream;
     for (int i = 0; i < N; ++i) {
         const int64_t n = (int64_t)site_size(i);
         const uint64_t key = (seed + 1) * 0xD1B54A32D192ED03ull ^ ((uint64_t)(i + 1) * 0x8CB92BA72F3D8DD7ull);
-        const unsigned g = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
-        hipLaunchKernelGGL(random_site_kernel, dim3(g), dim3(256), 0, s, sites[i].ptr, n, key);
-        NC_HIP(hipGetLastError());
+        fill_random(sites[i].ptr, n, key, s);
     }
     NC_HIP(hipStreamSynchronize(s));
 }
 
+std::vector<int> Chain::dims() const {
+    std::vector<int> d(N);
+    for (int i = 0; i < N; ++i) d[i] = ops[i]->d;
+    return d;
+}
+
+void Chain::set_bandwidth(int new_chi) {
+    chi = new_chi;
+    bonds = bandwidth_sequence(new_chi, dims());
+    for (int i = 0; i < N; ++i) sites[i].resize(site_size(i));
+}
+
+void Chain::increase_bandwidth(int new_chi) {
+    // increaseBandwidthDimension (chain.cpp:75-121) with increase_bandwidth_between
+    // (core.f95:3090-3144): from the right end, each bond that grows from b to b'
+    // is widened by a random b' x b isometry M -- the left site absorbs M, the
+    // right site M^H -- and the right site is re-normalised by a polar move
+    // whose factor the left site absorbs.  The state is unchanged.
+    optimized = false;
+    if (new_chi == chi) return;
+    if (new_chi < chi) throw Error("the bandwidth dimension can only grow", -77);
+    if (current != 0) throw Error("ChainNotAtFirstSiteError", -72);
+    hipStream_t s = Context::get().stream;
+    std::vector<int> nb = bandwidth_sequence(new_chi, dims());
+    std::vector<int> cur = bonds;  // bonds of the tensors as they are being widened
+    for (int i = N - 1; i >= 1; --i) {
+        const int old = cur[i], nw = nb[i];
+        const int dl = ops[i - 1]->d, dr = ops[i]->d;
+        const int bll = cur[i - 1], br = cur[i + 1];
+        ZBuf A1, B1;
+        const double2* a_in = sites[i - 1].ptr;
+        const double2* b_in = sites[i].ptr;
+        if (nw != old) {
+            // M = transpose of the polar factor of a random old x nw matrix
+            ZBuf W((size_t)old * nw), Up((size_t)old * nw), Pc((size_t)old * old), M((size_t)nw * old);
+            fill_random(W.ptr, (int64_t)old * nw, 0xA24BAED4963EE407ull ^ ((uint64_t)new_chi << 32) ^ (uint64_t)i, s);
+            int info = polar_split(old, nw, W.ptr, Up.ptr, Pc.ptr, s);
+            if (info != 0) throw Error("create_bandwidth_increase_matrix: bad rank", -78);
+            permute_102(Up.ptr, M.ptr, old, nw, 1, s);
+            // left site (old, bll, dl) -> (nw, bll, dl): M A
+            A1.resize((size_t)nw * bll * dl);
+            GemmPlan g1;
+            g1.M = nw; g1.N = bll * dl;
+            g1.begin_batch(A1.ptr, nw);
+            g1.add_seg(M.ptr, nw, sites[i - 1].ptr, old, old);
+            zgemm_run(g1, s);
+            // right site (br, old, dr) -> (br, nw, dr): B_s M^H for each s
+            B1.resize((size_t)br * nw * dr);
+            GemmPlan g2;
+            g2.M = br; g2.N = nw; g2.opA = OpN; g2.opB = OpC;
+            for (int a = 0; a < dr; ++a) {
+                g2.begin_batch(B1.ptr + (size_t)a * br * nw, br);
+                g2.add_seg(sites[i].ptr + (size_t)a * br * old, br, M.ptr, nw, old);
+            }
+            zgemm_run(g2, s);
+            a_in = A1.ptr;
+            b_in = B1.ptr;
+        }
+        // right-normalise the right site (it may hold the factor of the bond to
+        // its right), the left one absorbs the factor
+        ZBuf L2((size_t)nw * bll * dl), R2((size_t)br * nw * dr);
+        int info = norm_denorm_left(bll, nw, br, dl, dr, a_in, b_in, L2.ptr, R2.ptr, s);
+        if (info != 0) throw Error("NormalizationError", info);
+        sites[i - 1] = std::move(L2);
+        sites[i] = std::move(R2);
+        cur[i] = nw;
+        // absorb<Right>: environment and overlap boundaries of site i
+        bonds = cur;
+        update_right_env(i, s);
+        update_right_overlaps(i, s);
+    }
+    bonds = nb;
+    chi = new_chi;
+    NC_HIP(hipStreamSynchronize(s));
+    reset_projector_matrix();
+}
+
+void Chain::optimize_chain() {
+    // optimizeChain (base_chain.cpp:64-79); with fixed chi only the first step
+    sweep_until_converged();
+    if (!opts.grow) return;
+    const int maxb = maximum_bandwidth(dims());
+    double prev = energy;
+    increase_bandwidth(std::min(chi + 1, maxb));
+    sweep_until_converged();
+    double cur = energy;
+    while (outside_tolerance(prev, cur, opts.chain_tol) && chi < maxb) {
+        increase_bandwidth(std::min(chi + 1, maxb));
+        sweep_until_converged();
+        prev = cur;
+        cur = energy;
+    }
+}
+
+void Chain::restart_level(uint64_t seed) {
+    // reset (chain.cpp:175-191): the initial bond dimension, raised to the
+    // minimum that leaves room outside the lower levels, then a new random state
+    if (opts.grow || chi < minimum_bandwidth_for_projectors(dims(), (int)projectors.size())) {
+        const int maxb = maximum_bandwidth(dims());
+        const int want = std::min(maxb, std::max(opts.grow ? initial_chi : chi,
+                                                 minimum_bandwidth_for_projectors(dims(), (int)projectors.size())));
+        if (want != chi || opts.grow) set_bandwidth(want);
+    }
+    randomize(seed);
+    reset();
+}
+
 std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed) {
-    // solveForMultipleLevels (chain.cpp:301-310) at the chain's fixed bond
-    // dimension: converge, keep the state as a projector, restart from a new
-    // random state orthogonal to the lower levels
+    // solveForMultipleLevels (chain.cpp:301-310): converge (optimizeChain), keep
+    // the state as a projector, restart from a new random state orthogonal to
+    // the lower levels
     std::vector<double> energies;
     for (int k = 0; k + 1 < levels; ++k) {
-        sweep_until_converged();
+        optimize_chain();
         energies.push_back(energy);
         add_projector_from_state();
-        randomize(seed + (uint64_t)k + 1);
-        reset();
+        restart_level(seed + (uint64_t)k + 1);
     }
     if (levels > 0) {
-        sweep_until_converged();
+        optimize_chain();
         energies.push_back(energy);
     }
     return energies;

@@ -147,6 +147,29 @@ int nc_chain_expectation(nc_chain* c, nc_zcomplex* out) {
     });
 }
 
+int nc_chain_set_growth(nc_chain* c, int grow, double chain_tol) {
+    return guarded_chain("nc_chain_set_growth", [&] {
+        c->chain->opts.grow = grow != 0;
+        c->chain->opts.chain_tol = chain_tol;
+        return 0;
+    });
+}
+
+int nc_chain_increase_bandwidth(nc_chain* c, uint32_t new_chi) {
+    return guarded_chain("nc_chain_increase_bandwidth", [&] { c->chain->increase_bandwidth((int)new_chi); return 0; });
+}
+
+int nc_chain_optimize_chain(nc_chain* c) {
+    return guarded_chain("nc_chain_optimize_chain", [&] { c->chain->optimize_chain(); return 0; });
+}
+
+int nc_chain_get_bonds(nc_chain* c, uint32_t* bonds) {
+    return guarded_chain("nc_chain_get_bonds", [&] {
+        for (int i = 0; i <= c->chain->N; ++i) bonds[i] = (uint32_t)c->chain->bonds[i];
+        return 0;
+    });
+}
+
 int nc_chain_add_projector(nc_chain* c) {
     return guarded_chain("nc_chain_add_projector", [&] { c->chain->add_projector_from_state(); return 0; });
 }

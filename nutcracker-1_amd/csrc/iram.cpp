@@ -2,12 +2,22 @@
 // the Krylov basis resident on the device.  This is the algorithm of ARPACK's
 // znaupd/zneupd as the reference drives it (core.f95:2465-2527: bmat = 'I',
 // mode 1, iparam(1) = 1, info = 1 so the start vector is the caller's guess):
-//   * Arnoldi steps with classical Gram-Schmidt and one DGKS correction
-//     (accept when ||w|| > 0.717 ||A v||, as znaitr does);
+//   * Arnoldi steps with classical Gram-Schmidt applied twice (CGS2; znaitr
+//     applies the second pass when the first loses more than 0.717 of the
+//     norm -- here it always runs, which is never less orthogonal);
 //   * Ritz pairs of the small Hessenberg H on the host; convergence when the
 //     Ritz estimate |beta e_m^T s| <= tol max(eps^(2/3), |theta|) (znconv);
-//   * restart by ncv - 1 shifted QR steps with the unwanted Ritz values.
+//   * restart by ncv - 1 shifted QR steps with the unwanted Ritz values
+//     (ncv <= 3, as ARPACK with nev = 1), or keeping ncv / 2 Ritz vectors
+//     (thick restart) for larger ncv.
 // The restart count returned plays the role of ARPACK's iparam(3).
+//
+// The Arnoldi extension from k to ncv vectors runs without a host round trip:
+// each step's Gram-Schmidt coefficients, norm and Hessenberg column stay on the
+// device (two-pass deterministic reductions), the new basis vector is written
+// in place, and the host reads the Hessenberg matrices once per extension.
+// Several independent problems (one per chain) extend in lockstep, so one
+// kernel launch serves all of them.
 #include "nc_internal.hpp"
 
 #include <algorithm>
@@ -17,6 +27,169 @@
 namespace nc {
 
 namespace {
+
+constexpr int KT = 256;  // threads per block
+constexpr int KC = 8;    // basis columns per register group
+
+struct KDesc {
+    double2* V;   // n x (ncv + 1), column-major: basis columns 0..ncv-1, residual f in column ncv
+    int64_t n;
+    int kstart;   // first step of the current extension; ncv = idle
+    int _pad;
+};
+
+__device__ __forceinline__ double kwarp_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// partial dots of columns 0..j+1 of V against w = column j+1 (the last one is
+// ||w||^2): partial[((p * ldp) + c) * nblk + b]
+__global__ __launch_bounds__(KT) void kdot_kernel(const KDesc* __restrict__ desc, int j, double2* __restrict__ partial,
+                                                  int ldp) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    const int m = j + 2;
+    const int64_t n = d.n;
+    const double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    __shared__ double sh[KT / 64][2 * KC];
+    const int nb = gridDim.x;
+    for (int c0 = 0; c0 < m; c0 += KC) {
+        double re[KC], im[KC];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) { re[q] = 0; im[q] = 0; }
+        for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+            const double2 wv = w[i];
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                if (c0 + q < m) {
+                    const double2 a = d.V[i + (int64_t)(c0 + q) * n];
+                    re[q] += a.x * wv.x + a.y * wv.y;
+                    im[q] += a.x * wv.y - a.y * wv.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+            re[q] = kwarp_sum(re[q]);
+            im[q] = kwarp_sum(im[q]);
+        }
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                sh[threadIdx.x >> 6][2 * q] = re[q];
+                sh[threadIdx.x >> 6][2 * q + 1] = im[q];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < KC && c0 + (int)threadIdx.x < m) {
+            const int q = threadIdx.x;
+            double tr = 0, ti = 0;
+            for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2 * q]; ti += sh[wv][2 * q + 1]; }
+            partial[((int64_t)p * ldp + c0 + q) * nb + blockIdx.x] = make_double2(tr, ti);
+        }
+        __syncthreads();
+    }
+}
+
+// out[p * ldp + c] = sum_b partial[((p * ldp) + c) * nblk + b], c = 0..j+1
+__global__ __launch_bounds__(KT) void kreduce_kernel(const KDesc* __restrict__ desc, int j,
+                                                     const double2* __restrict__ partial, int ldp, int nblk,
+                                                     double2* __restrict__ out) {
+    const int p = blockIdx.y, c = blockIdx.x;
+    if (j < desc[p].kstart || c > j + 1) return;
+    __shared__ double sre[KT / 64], sim[KT / 64];
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nblk; b += KT) {
+        const double2 v = partial[((int64_t)p * ldp + c) * nblk + b];
+        re += v.x;
+        im += v.y;
+    }
+    re = kwarp_sum(re);
+    im = kwarp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < KT / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+        out[(int64_t)p * ldp + c] = make_double2(tr, ti);
+    }
+}
+
+// w -= sum_{c <= j} V_c coef_c, and partial ||w||^2 per block
+__global__ __launch_bounds__(KT) void kupdate_kernel(const KDesc* __restrict__ desc, int j,
+                                                     const double2* __restrict__ coef, int ldp,
+                                                     double* __restrict__ pnorm) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    __shared__ double2 sc[64];
+    __shared__ double sn[KT / 64];
+    const int m = j + 1;
+    for (int c = threadIdx.x; c < m; c += KT) sc[c] = coef[(int64_t)p * ldp + c];
+    __syncthreads();
+    const int64_t n = d.n;
+    double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)gridDim.x * KT) {
+        double2 v = w[i];
+        for (int c = 0; c < m; ++c) {
+            const double2 a = d.V[i + (int64_t)c * n];
+            const double2 h = sc[c];
+            v.x -= a.x * h.x - a.y * h.y;
+            v.y -= a.x * h.y + a.y * h.x;
+        }
+        w[i] = v;
+        acc += v.x * v.x + v.y * v.y;
+    }
+    acc = kwarp_sum(acc);
+    if ((threadIdx.x & 63) == 0) sn[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < KT / 64; ++k) t += sn[k];
+        pnorm[(int64_t)p * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// Hessenberg column j = coef1 + coef2, beta_j = ||w||; beta = 0 when the new
+// vector is round-off of an invariant subspace; scale[p] = 1 / beta (or 0)
+__global__ void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv, const double2* __restrict__ coef1,
+                               const double2* __restrict__ coef2, int ldp, const double* __restrict__ pnorm, int nblk,
+                               double2* __restrict__ H, double* __restrict__ betas, double* __restrict__ scale) {
+    const int p = blockIdx.x;
+    if (j < desc[p].kstart || threadIdx.x != 0) return;
+    double b2 = 0;
+    for (int b = 0; b < nblk; ++b) b2 += pnorm[(int64_t)p * nblk + b];
+    double beta = sqrt(b2);
+    const double wn = sqrt(fmax(0.0, coef1[(int64_t)p * ldp + j + 1].x));
+    if (!(beta >= 1e-14 * wn)) beta = 0.0;
+    double2* Hp = H + (int64_t)p * (ncv + 1) * ncv;
+    for (int r = 0; r <= j; ++r) {
+        const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
+        Hp[r + (int64_t)j * (ncv + 1)] = make_double2(a.x + c.x, a.y + c.y);
+    }
+    Hp[(j + 1) + (int64_t)j * (ncv + 1)] = make_double2(beta, 0.0);
+    betas[(int64_t)p * ncv + j] = beta;
+    scale[p] = beta > 0 ? 1.0 / beta : 0.0;
+}
+
+// V(:, j+1) *= scale (the next basis vector), unless it is the residual f
+__global__ __launch_bounds__(KT) void kscale_kernel(const KDesc* __restrict__ desc, int j, int ncv,
+                                                    const double* __restrict__ scale) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart || j + 1 >= ncv) return;
+    const double sc = scale[p];
+    double2* w = d.V + (int64_t)(j + 1) * d.n;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < d.n; i += (int64_t)gridDim.x * KT) {
+        double2 v = w[i];
+        v.x *= sc;
+        v.y *= sc;
+        w[i] = v;
+    }
+}
 
 int wanted_index(std::vector<zd> const& th, const char* which) {
     int best = 0;
@@ -85,155 +258,277 @@ void shifted_qr_step(int m, std::vector<zd>& H, zd mu, std::vector<zd>& Q) {
         for (int i = j + 2; i < m; ++i) h(i, j) = 0.0;
 }
 
+// per-problem solver state
+struct KState {
+    IramTask* task;
+    int64_t n = 0;
+    ZBuf V, Vk;
+    std::vector<zd> H;  // ncv x ncv, host copy
+    int k = 0;
+    double beta = 0.0;
+    bool done = false;
+    int iter = 0;       // restarts performed (ARPACK iparam(3))
+    int fresh = 0;      // random vectors drawn so far
+};
+
+// ARPACK's getv0 after an invariant subspace (znaitr step 1, znaupd): the
+// factorization continues from a random vector orthogonal to the first k basis
+// vectors, with a zero subdiagonal H(k, k-1).  False when the basis already
+// spans the space.
+bool continue_with_random(KState& x, int p, int k, hipStream_t s) {
+    if (k >= x.n) return false;
+    double2* v = x.V.ptr + (size_t)k * x.n;
+    const uint64_t key = 0x51ED270B27D3A6C1ull ^ ((uint64_t)(p + 1) * 0x9E3779B97F4A7C15ull) ^
+                         ((uint64_t)(++x.fresh) * 0xC2B2AE3D27D4EB4Full);
+    fill_random(v, x.n, key, s);
+    std::vector<zd> c(k);
+    for (int pass = 0; pass < 2; ++pass) {
+        zdotc_multi(x.V.ptr, x.n, k, v, x.n, c.data(), s);
+        zgemv_sub(x.V.ptr, x.n, k, c.data(), v, x.n, s);
+    }
+    const double nr = dznrm2(v, x.n, s);
+    if (!(nr > 0)) return false;
+    zscal(v, zd(1.0 / nr, 0), x.n, s);
+    return true;
+}
+
+// the Ritz vector of the leading m x m block with coefficients y
+void finish_with(KState& st, int m, std::vector<zd> const& y, zd theta, int iter, hipStream_t s) {
+    IramTask& t = *st.task;
+    zgemv_n(st.V.ptr, st.n, m, y.data(), t.result, st.n, s);
+    const double rn = dznrm2(t.result, st.n, s);
+    zscal(t.result, zd(1.0 / rn, 0), st.n, s);
+    t.out.eigenvalue = theta;
+    t.out.restarts = iter;
+    t.out.info = 0;
+    st.done = true;
+}
+
 }  // namespace
+
+void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s) {
+    const int P = (int)tasks.size();
+    if (P == 0) return;
+    const int ncv = tasks[0].ncv;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double eps23 = std::pow(eps, 2.0 / 3.0);
+    if (ncv + 2 > 64) throw Error("iram: ncv above 62 is not supported", -8);
+    std::vector<KState> st(P);
+    int64_t nmax = 1;
+    for (int p = 0; p < P; ++p) {
+        IramTask& t = tasks[p];
+        if (t.ncv != ncv) throw Error("iram_batch: every problem of a batch needs the same ncv", -8);
+        t.out = IramResult();
+        KState& x = st[p];
+        x.task = &t;
+        x.n = t.op->n;
+        if (ncv > x.n || ncv < 2 || x.n < 1) { t.out.info = -3; x.done = true; continue; }
+        if (t.max_restarts < 1) t.max_restarts = 1;
+        x.V.resize((size_t)x.n * (ncv + 1));
+        x.H.assign((size_t)ncv * ncv, zd(0, 0));
+        d2d(x.V.ptr, t.guess, (size_t)x.n * sizeof(double2));
+        const double nrm = dznrm2(x.V.ptr, x.n, s);
+        if (!(nrm > 0) || !std::isfinite(nrm)) { t.out.info = -9; x.done = true; continue; }
+        zscal(x.V.ptr, zd(1.0 / nrm, 0), x.n, s);
+        nmax = std::max(nmax, x.n);
+    }
+    // reduction geometry: about 2048 blocks over the batch, deterministic per (P, nmax)
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((nmax + 4 * KT - 1) / (4 * KT),
+                                                                   std::max(1, 2048 / P)));
+    const int ldp = ncv + 2;
+    ZBuf partial((size_t)P * ldp * nblk), coef1((size_t)P * ldp), coef2((size_t)P * ldp);
+    ZBuf Hd((size_t)P * (ncv + 1) * ncv);
+    DevBuf<double> pnorm((size_t)P * nblk), betas((size_t)P * ncv), scale(P);
+    DevBuf<KDesc> ddesc(P);
+    std::vector<KDesc> hdesc(P);
+    std::vector<double2> Hh((size_t)P * (ncv + 1) * ncv);
+    std::vector<double> bh((size_t)P * ncv);
+
+    for (;;) {
+        int jmin = ncv;
+        for (int p = 0; p < P; ++p) {
+            hdesc[p].V = st[p].V.ptr;
+            hdesc[p].n = st[p].n;
+            hdesc[p].kstart = st[p].done ? ncv : st[p].k;
+            hdesc[p]._pad = 0;
+            if (!st[p].done) jmin = std::min(jmin, st[p].k);
+        }
+        if (jmin >= ncv) break;  // every problem finished
+        h2d(ddesc.ptr, hdesc.data(), (size_t)P * sizeof(KDesc));
+        // ---- extend every active factorization from its k to ncv
+        std::vector<int> who;
+        std::vector<const double2*> xs;
+        std::vector<double2*> ys;
+        for (int j = jmin; j < ncv; ++j) {
+            who.clear(); xs.clear(); ys.clear();
+            for (int p = 0; p < P; ++p) {
+                if (st[p].done || st[p].k > j) continue;
+                who.push_back(p);
+                xs.push_back(st[p].V.ptr + (size_t)j * st[p].n);
+                ys.push_back(st[p].V.ptr + (size_t)(j + 1) * st[p].n);
+                ++tasks[p].out.matvecs;
+            }
+            apply(who, xs, ys, s);
+            const dim3 g(nblk, P), gr(ldp, P);
+            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+            hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr);
+            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
+            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+            hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr);
+            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr);
+            hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr, ldp,
+                               pnorm.ptr, nblk, Hd.ptr, betas.ptr, scale.ptr);
+            hipLaunchKernelGGL(kscale_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, ncv, scale.ptr);
+            NC_HIP(hipGetLastError());
+        }
+        d2h(Hh.data(), Hd.ptr, Hh.size() * sizeof(double2));
+        d2h(bh.data(), betas.ptr, bh.size() * sizeof(double));
+        // ---- per problem: Ritz values, convergence, restart
+        for (int p = 0; p < P; ++p) {
+            KState& x = st[p];
+            if (x.done) continue;
+            IramTask& t = *x.task;
+            const int iter = ++x.iter;
+            auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
+            const double2* Hp = Hh.data() + (size_t)p * (ncv + 1) * ncv;
+            int stop = -1;  // first step whose new vector vanished (invariant subspace)
+            for (int c = x.k; c < ncv; ++c) {
+                for (int r = 0; r <= c; ++r) h(r, c) = zd(Hp[r + (size_t)c * (ncv + 1)].x, Hp[r + (size_t)c * (ncv + 1)].y);
+                const double bc = bh[(size_t)p * ncv + c];
+                if (c + 1 < ncv) h(c + 1, c) = bc;
+                if (bc == 0.0 && c + 1 < ncv && stop < 0) stop = c;
+                if (stop >= 0) break;
+            }
+            if (stop >= 0) {
+                // invariant subspace found early: continue from a random vector
+                // orthogonal to it, as ARPACK does (the extension does not count
+                // as a restart)
+                if (continue_with_random(x, p, stop + 1, s)) {
+                    h(stop + 1, stop) = 0.0;
+                    for (int c = stop + 1; c < ncv; ++c)
+                        for (int r = 0; r < ncv; ++r) h(r, c) = 0.0;
+                    x.k = stop + 1;
+                    --x.iter;
+                    continue;
+                }
+                // the basis spans the whole space: its Ritz values are exact
+                const int m = stop + 1;
+                std::vector<zd> Hs((size_t)m * m);
+                for (int c = 0; c < m; ++c)
+                    for (int r = 0; r < m; ++r) Hs[(size_t)r + (size_t)c * m] = h(r, c);
+                std::vector<zd> th, Y;
+                hessenberg_eig(m, Hs, th, Y);
+                const int wi = wanted_index(th, t.which);
+                std::vector<zd> yv(Y.begin() + (size_t)wi * m, Y.begin() + (size_t)(wi + 1) * m);
+                finish_with(x, m, yv, th[wi], iter, s);
+                continue;
+            }
+            x.beta = bh[(size_t)p * ncv + ncv - 1];
+            std::vector<zd> th, Y;
+            hessenberg_eig(ncv, x.H, th, Y);
+            const int wi = wanted_index(th, t.which);
+            const double est = x.beta * std::abs(Y[(size_t)(ncv - 1) + (size_t)wi * ncv]);
+            const double tol_eff = t.tol > 0 ? t.tol : eps;
+            const bool converged = est <= tol_eff * std::max(eps23, std::abs(th[wi]));
+            if (converged || iter >= t.max_restarts) {
+                if (!converged) {
+                    t.out.restarts = iter;
+                    t.out.info = -14;
+                    x.done = true;
+                    continue;
+                }
+                std::vector<zd> yv(Y.begin() + (size_t)wi * ncv, Y.begin() + (size_t)(wi + 1) * ncv);
+                finish_with(x, ncv, yv, th[wi], iter, s);
+                continue;
+            }
+            // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies
+            // the ncv - 1 unwanted Ritz values as exact shifts; a larger basis
+            // keeps half of its vectors (thick restart) and shifts out the rest.
+            const int keep = ncv <= 3 ? 1 : ncv / 2;
+            std::vector<int> order = wanted_order(th, t.which);
+            std::vector<zd> Q((size_t)ncv * ncv, zd(0, 0));
+            for (int i = 0; i < ncv; ++i) Q[(size_t)i + (size_t)i * ncv] = 1.0;
+            for (int q = keep; q < ncv; ++q) shifted_qr_step(ncv, x.H, th[order[q]], Q);
+            const int64_t n = x.n;
+            double2* f = x.V.ptr + (size_t)n * ncv;
+            // f <- hk V q_keep + qm f ; V_k <- V Q(:, 0:keep)
+            const zd hk = h(keep, keep - 1);
+            const zd qm = Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
+            if (x.Vk.n < (size_t)n * (keep + 1)) x.Vk.resize((size_t)n * (keep + 1));
+            double2* tmp = x.Vk.ptr + (size_t)n * keep;
+            zgemv_n(x.V.ptr, n, ncv, &Q[(size_t)keep * ncv], tmp, n, s);
+            zaxpby(tmp, qm, f, hk, n, s);
+            if (keep == 1) {
+                zgemv_n(x.V.ptr, n, ncv, Q.data(), x.Vk.ptr, n, s);
+            } else {
+                ZBuf Qd((size_t)ncv * keep);
+                h2d(Qd.ptr, Q.data(), (size_t)ncv * keep * sizeof(double2));
+                GemmPlan gp;
+                gp.M = (int)n; gp.N = keep; gp.opA = OpN; gp.opB = OpN;
+                gp.begin_batch(x.Vk.ptr, n);
+                gp.add_seg(x.V.ptr, n, Qd.ptr, ncv, ncv);
+                zgemm_run(gp, s);
+            }
+            d2d(x.V.ptr, x.Vk.ptr, (size_t)n * keep * sizeof(double2));
+            d2d(f, tmp, (size_t)n * sizeof(double2));
+            std::vector<zd> Hk((size_t)keep * keep);
+            for (int c = 0; c < keep; ++c)
+                for (int r = 0; r < keep; ++r) Hk[(size_t)r + (size_t)c * keep] = (r <= c + 1) ? h(r, c) : zd(0, 0);
+            std::fill(x.H.begin(), x.H.end(), zd(0, 0));
+            for (int c = 0; c < keep; ++c)
+                for (int r = 0; r < keep; ++r) h(r, c) = Hk[(size_t)r + (size_t)c * keep];
+            x.beta = dznrm2(f, n, s);
+            if (keep == 1) {
+                // keep v_0 exactly unit norm against round-off drift
+                const double n0 = dznrm2(x.V.ptr, n, s);
+                zscal(x.V.ptr, zd(1.0 / n0, 0), n, s);
+            }
+            x.k = keep;
+            if (x.beta == 0.0 && continue_with_random(x, p, keep, s)) {
+                // the restart landed on an invariant subspace: continue from a
+                // random vector orthogonal to it (H(keep, keep - 1) = 0)
+                h(keep, keep - 1) = 0.0;
+                continue;
+            }
+            if (x.beta == 0.0) {
+                // the kept vectors span the whole space: their Ritz pairs are exact
+                std::vector<zd> th2, Y2;
+                hessenberg_eig(keep, Hk, th2, Y2);
+                const int w2 = wanted_index(th2, t.which);
+                std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
+                finish_with(x, keep, yv, th2[w2], iter, s);
+                continue;
+            }
+            // the next extension starts from v_keep = f / beta, H(keep, keep - 1) = beta
+            double2* vk = x.V.ptr + (size_t)n * keep;
+            d2d(vk, f, (size_t)n * sizeof(double2));
+            zscal(vk, zd(1.0 / x.beta, 0), n, s);
+            h(keep, keep - 1) = x.beta;
+        }
+    }
+}
 
 IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_restarts,
                 const double2* guess, double2* result, hipStream_t s) {
-    IramResult out;
-    const int64_t n = op.n;
-    if (ncv > n || ncv < 2 || n < 1) { out.info = -3; return out; }
-    if (max_restarts < 1) max_restarts = 1;
-    const double eps = std::numeric_limits<double>::epsilon();
-    const double eps23 = std::pow(eps, 2.0 / 3.0);
-    const double tol_eff = tol > 0 ? tol : eps;
+    std::vector<IramTask> t(1);
+    t[0].op = &op;
+    t[0].ncv = ncv;
+    t[0].which = which;
+    t[0].tol = tol;
+    t[0].max_restarts = max_restarts;
+    t[0].guess = guess;
+    t[0].result = result;
+    iram_batch(t, BatchApply::each(t), s);
+    return t[0].out;
+}
 
-    ZBuf V((size_t)n * (ncv + 1));   // columns 0..ncv-1: basis, column ncv: residual f
-    ZBuf W((size_t)n * 2), Vk;
-    double2* f = V.ptr + (size_t)n * ncv;
-    double2* w = W.ptr;
-    double2* tmp = W.ptr + n;
-
-    // start vector: the caller's guess, normalised
-    d2d(V.ptr, guess, n * sizeof(double2));
-    double nrm = dznrm2(V.ptr, n, s);
-    if (!(nrm > 0) || !std::isfinite(nrm)) { out.info = -9; return out; }
-    zscal(V.ptr, zd(1.0 / nrm, 0), n, s);
-
-    std::vector<zd> H((size_t)ncv * ncv, zd(0, 0));
-    auto h = [&](int i, int j) -> zd& { return H[(size_t)i + (size_t)j * ncv]; };
-    int k = 0;          // current factorization length
-    double beta = 0.0;  // ||f||
-    std::vector<zd> coef(ncv + 1), corr(ncv + 1);
-
-    for (int iter = 1;; ++iter) {
-        // ---- extend the Arnoldi factorization from length k to ncv
-        for (int j = k; j < ncv; ++j) {
-            double2* vj = V.ptr + (size_t)n * j;
-            if (j > 0) {
-                d2d(vj, f, n * sizeof(double2));
-                zscal(vj, zd(1.0 / beta, 0), n, s);
-                h(j, j - 1) = beta;
-            }
-            op.apply(vj, w, s);
-            ++out.matvecs;
-            const double wnorm = dznrm2(w, n, s);
-            zdotc_multi(V.ptr, n, j + 1, w, n, coef.data(), s);
-            zgemv_sub(V.ptr, n, j + 1, coef.data(), w, n, s);
-            double rnorm = dznrm2(w, n, s);
-            double ref = wnorm;
-            for (int pass = 0; pass < 2 && !(rnorm > 0.717 * ref); ++pass) {
-                zdotc_multi(V.ptr, n, j + 1, w, n, corr.data(), s);
-                zgemv_sub(V.ptr, n, j + 1, corr.data(), w, n, s);
-                for (int i = 0; i <= j; ++i) coef[i] += corr[i];
-                ref = rnorm;
-                rnorm = dznrm2(w, n, s);
-            }
-            if (!(rnorm > 0.717 * ref) && rnorm < 1e-14 * wnorm) rnorm = 0.0;  // invariant subspace
-            for (int i = 0; i <= j; ++i) h(i, j) = coef[i];
-            d2d(f, w, n * sizeof(double2));
-            beta = rnorm;
-            if (beta == 0.0 && j + 1 < ncv) {
-                // exact invariant subspace found early: the Ritz values of the
-                // leading block are exact, finish with a shortened factorization
-                std::vector<zd> Hs((size_t)(j + 1) * (j + 1));
-                for (int c = 0; c <= j; ++c)
-                    for (int r = 0; r <= j; ++r) Hs[(size_t)r + (size_t)c * (j + 1)] = h(r, c);
-                std::vector<zd> th, Y;
-                hessenberg_eig(j + 1, Hs, th, Y);
-                int wi = wanted_index(th, which);
-                std::vector<zd> yv(Y.begin() + (size_t)wi * (j + 1), Y.begin() + (size_t)(wi + 1) * (j + 1));
-                zgemv_n(V.ptr, n, j + 1, yv.data(), result, n, s);
-                double rn = dznrm2(result, n, s);
-                zscal(result, zd(1.0 / rn, 0), n, s);
-                out.eigenvalue = th[wi];
-                out.restarts = iter;
-                out.info = 0;
-                return out;
-            }
-        }
-        // ---- Ritz values and estimates
-        std::vector<zd> th, Y;
-        hessenberg_eig(ncv, H, th, Y);
-        const int wi = wanted_index(th, which);
-        const double est = beta * std::abs(Y[(size_t)(ncv - 1) + (size_t)wi * ncv]);
-        const bool converged = est <= tol_eff * std::max(eps23, std::abs(th[wi]));
-        if (converged || iter >= max_restarts) {
-            out.restarts = iter;
-            if (!converged) { out.info = -14; return out; }
-            std::vector<zd> yv(Y.begin() + (size_t)wi * ncv, Y.begin() + (size_t)(wi + 1) * ncv);
-            zgemv_n(V.ptr, n, ncv, yv.data(), result, n, s);
-            double rn = dznrm2(result, n, s);
-            zscal(result, zd(1.0 / rn, 0), n, s);
-            out.eigenvalue = th[wi];
-            out.info = 0;
-            return out;
-        }
-        // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies the
-        // ncv - 1 unwanted Ritz values as exact shifts; a larger basis keeps half
-        // of its vectors (thick restart) and shifts out the least wanted half.
-        const int keep = ncv <= 3 ? 1 : ncv / 2;
-        std::vector<int> order = wanted_order(th, which);
-        std::vector<zd> Q((size_t)ncv * ncv, zd(0, 0));
-        for (int i = 0; i < ncv; ++i) Q[(size_t)i + (size_t)i * ncv] = 1.0;
-        for (int t = keep; t < ncv; ++t) shifted_qr_step(ncv, H, th[order[t]], Q);
-        // V_k <- V Q(:, 0:keep) ; f <- V q_keep H(keep, keep-1) + f Q(ncv-1, keep-1)
-        const zd hk = h(keep, keep - 1);
-        const zd qm = Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
-        zgemv_n(V.ptr, n, ncv, &Q[(size_t)keep * ncv], tmp, n, s);   // V q_keep
-        zaxpby(tmp, qm, f, hk, n, s);                                  // tmp = hk V q_keep + qm f
-        if (Vk.n < (size_t)n * keep) Vk.resize((size_t)n * keep);
-        if (keep == 1) {
-            zgemv_n(V.ptr, n, ncv, Q.data(), Vk.ptr, n, s);
-        } else {
-            ZBuf Qd((size_t)ncv * keep);
-            h2d(Qd.ptr, Q.data(), (size_t)ncv * keep * sizeof(double2));
-            GemmPlan g;
-            g.M = (int)n; g.N = keep; g.opA = OpN; g.opB = OpN;
-            g.begin_batch(Vk.ptr, n);
-            g.add_seg(V.ptr, n, Qd.ptr, ncv, ncv);
-            zgemm_run(g, s);
-            NC_HIP(hipStreamSynchronize(s));
-        }
-        d2d(V.ptr, Vk.ptr, (size_t)n * keep * sizeof(double2));
-        d2d(f, tmp, n * sizeof(double2));
-        std::vector<zd> Hk((size_t)keep * keep);
-        for (int c = 0; c < keep; ++c)
-            for (int r = 0; r < keep; ++r) Hk[(size_t)r + (size_t)c * keep] = (r <= c + 1) ? h(r, c) : zd(0, 0);
-        std::fill(H.begin(), H.end(), zd(0, 0));
-        for (int c = 0; c < keep; ++c)
-            for (int r = 0; r < keep; ++r) h(r, c) = Hk[(size_t)r + (size_t)c * keep];
-        beta = dznrm2(f, n, s);
-        if (keep == 1) {
-            // keep v_0 exactly unit norm against round-off drift
-            double n0 = dznrm2(V.ptr, n, s);
-            zscal(V.ptr, zd(1.0 / n0, 0), n, s);
-        }
-        k = keep;
-        if (beta == 0.0) {
-            // the restart landed on an invariant subspace: its Ritz pairs are exact
-            std::vector<zd> th2, Y2;
-            hessenberg_eig(keep, Hk, th2, Y2);
-            const int w2 = wanted_index(th2, which);
-            std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
-            zgemv_n(V.ptr, n, keep, yv.data(), result, n, s);
-            double rn = dznrm2(result, n, s);
-            zscal(result, zd(1.0 / rn, 0), n, s);
-            out.eigenvalue = th2[w2];
-            out.restarts = iter;
-            out.info = 0;
-            return out;
-        }
-    }
+BatchApply BatchApply::each(std::vector<IramTask> const& tasks) {
+    BatchApply a;
+    a.fn = [&tasks](std::vector<int> const& who, std::vector<const double2*> const& xs,
+                    std::vector<double2*> const& ys, hipStream_t s) {
+        for (size_t q = 0; q < who.size(); ++q) tasks[who[q]].op->apply(xs[q], ys[q], s);
+    };
+    return a;
 }
 
 }  // namespace nc

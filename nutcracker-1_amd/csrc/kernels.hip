@@ -424,4 +424,28 @@ void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s) {
     NC_HIP(hipGetLastError());
 }
 
+// uniform entries in [-1, 1) + i [-1, 1) from a counter-based hash (splitmix64)
+__global__ void fill_random_kernel(double2* x, int64_t n, uint64_t key) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = key ^ ((uint64_t)e * 0x9E3779B97F4A7C15ull);
+        double v[2];
+        for (int h = 0; h < 2; ++h) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t t = z;
+            t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+            t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+            t ^= t >> 31;
+            v[h] = (double)(t >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        }
+        x[e] = make_double2(v[0], v[1]);
+    }
+}
+
+void fill_random(double2* x, int64_t n, uint64_t key, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_random_kernel, dim3(g), dim3(256), 0, s, x, n, key);
+    NC_HIP(hipGetLastError());
+}
+
 }  // namespace nc

@@ -22,6 +22,11 @@ struct ChainOptions {
     double env32_floor = 1e-6;  // relative energy noise floor of complex64 environments
     double tol32 = 1e-7;
     bool polish = false;
+    // optimizeChain (base_chain.cpp:64-79): grow the bond dimension by one
+    // (computeNewBandwidthDimension = chi + 1, chain_options.cpp:28) after each
+    // convergence until the energy changes by less than chain_tol; off = fixed chi
+    bool grow = false;
+    double chain_tol = 1e-12;
 };
 
 struct ChainStats {
@@ -76,6 +81,13 @@ struct Chain {
     double projector_overlap();
     void randomize(uint64_t seed);
     std::vector<double> solve_for_multiple_levels(int levels, uint64_t seed);
+    // bond dimensions (utilities.cpp:32-91, utilities.hpp:264-283)
+    int chi = 1, initial_chi = 1;
+    std::vector<int> dims() const;
+    void set_bandwidth(int new_chi);            // new bond sequence, sites reallocated (contents undefined)
+    void increase_bandwidth(int new_chi);       // increaseBandwidthDimension (chain.cpp:75-121)
+    void optimize_chain();                      // optimizeChain (base_chain.cpp:64-79)
+    void restart_level(uint64_t seed);          // new random state at the initial bond dimension, reset
     void reset();
     zd site_expectation();
     void move_right();
@@ -84,6 +96,10 @@ struct Chain {
     void sweep();
     void sweep_until_converged();
 };
+
+std::vector<int> bandwidth_sequence(int requested, std::vector<int> const& dims);
+int maximum_bandwidth(std::vector<int> const& dims);
+int minimum_bandwidth_for_projectors(std::vector<int> const& dims, int nproj);
 
 // <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382)
 zd state_overlap(Chain const& a, Chain const& b);

@@ -16,6 +16,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 namespace nc {
@@ -264,6 +265,8 @@ void permute_102(const double2* src, double2* dst, int64_t n0, int64_t n1, int64
 // scale rows: X(i, j) *= sv[i] for an (m x ncol) matrix
 void scale_rows(double2* X, const double* sv, int64_t m, int64_t ncol, hipStream_t s);
 void conj_inplace(double2* X, int64_t n, hipStream_t s);
+// deterministic pseudo-random entries, uniform in [-1, 1) + i [-1, 1), from `key`
+void fill_random(double2* x, int64_t n, uint64_t key, hipStream_t s);
 // add alpha * I to an (m x m) matrix
 void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s);
 // sum_k sum_ij A(i, j, k) B(j, i, k)
@@ -326,6 +329,34 @@ struct IramResult {
 // which: "SR", "LR", "LM".  guess / result are device vectors of length op.n.
 IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_restarts,
                 const double2* guess, double2* result, hipStream_t s);
+
+// Several independent eigen-problems (one per chain) solved in lockstep: each
+// Arnoldi step applies every active operator (through `apply`, which may batch
+// them into grouped launches) and orthogonalises all of them with the same
+// kernel launches.  All tasks share ncv.
+struct IramTask {
+    Operator* op = nullptr;
+    int ncv = 3;
+    const char* which = "SR";
+    double tol = 0;
+    int max_restarts = 1;
+    const double2* guess = nullptr;
+    double2* result = nullptr;
+    IramResult out;
+};
+struct BatchApply {
+    // y_q = A_{who[q]} x_q for every listed problem
+    std::function<void(std::vector<int> const&, std::vector<const double2*> const&, std::vector<double2*> const&,
+                       hipStream_t)>
+        fn;
+    void operator()(std::vector<int> const& who, std::vector<const double2*> const& xs,
+                    std::vector<double2*> const& ys, hipStream_t s) const {
+        fn(who, xs, ys, s);
+    }
+    // one operator application after another
+    static BatchApply each(std::vector<IramTask> const& tasks);
+};
+void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s);
 
 // Small dense host eigen-problems.
 // Eigenvalues and right eigenvectors of a general complex upper Hessenberg matrix.

@@ -316,7 +316,8 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
             if (mixed) out.matvecs32 += rr.matvecs;
             else out.matvecs += rr.matvecs;
         };
-        int ncv = a.ncv;
+        // a Krylov basis cannot be larger than the (projected) space
+        int ncv = std::min(a.ncv, orth);
         int iterations = a.max_iterations;
         IramResult r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
         count(r);
@@ -327,8 +328,8 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
                 strategy_1(a, op, out, s);
                 done = true;
             } else {
-                while (ncv <= iterations) {
-                    ncv = ncv * 3 / 2;
+                while (ncv <= iterations && ncv < orth) {
+                    ncv = std::min(ncv * 3 / 2, orth);
                     r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
                     count(r);
                     iterations = r.restarts;

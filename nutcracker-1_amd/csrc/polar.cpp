@@ -82,7 +82,10 @@ PolarStats g_polar_stats;
 // sends the largest singular values down to the round-off level and loses
 // their singular vectors; alpha is therefore capped so that p(1) >= 0.1
 // (alpha = 1.698, growth 2.55 per step for small singular values instead of
-// 2.60), which keeps every direction at least a tenth of ||X|| in size.
+// 2.60), which keeps every direction at least a tenth of ||X|| in size.  The
+// bound l is also raised to sqrt(1 - ||X X^H - I||_F) whenever that is larger,
+// so iterates near convergence (and nearly unitary input) take plain alpha = 1
+// steps; a scaled step there would map sigma = 1 down to p(1).
 // A site after optimisation is often numerically rank deficient (Schmidt
 // values far below round-off); its polar factor is then not determined in
 // those directions, and the iteration would stall there.  A deterministic
@@ -134,6 +137,10 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
             *iters_out = it;
             return true;
         }
+        // every sigma lies in [l, 1], and sigma^2 >= 1 - ||X X^H - I||_F: the
+        // measured error raises the bound, so a nearly unitary iterate takes
+        // plain (alpha = 1) steps, which keep sigma = 1 fixed
+        if (err < 1.0) l = std::max(l, std::sqrt(1.0 - err));
         double alpha = 1.0;
         if (l < 1.0 - 1e-3) alpha = std::min(alpha_cap, std::sqrt(3.0 / (1.0 + l + l * l)));
         // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E

@@ -36,6 +36,10 @@ _CHAIN_SIGS = {
     "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "nc_chain_set_mixed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
     "nc_chain_add_projector": (C.c_int, [C.c_void_p]),
+    "nc_chain_set_growth": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "nc_chain_increase_bandwidth": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "nc_chain_optimize_chain": (C.c_int, [C.c_void_p]),
+    "nc_chain_get_bonds": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "nc_chain_state_overlap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nc_chain_clear_projectors": (C.c_int, [C.c_void_p]),
     "nc_chain_projector_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
@@ -74,7 +78,8 @@ class Chain:
                  raw_sites: Optional[List[np.ndarray]] = None, site_tol: float = 1e-12,
                  sweep_tol: float = 1e-12, sanity: float = 1e-12, max_iterations: int = 10000,
                  which: str = "SR", ncv: int = 3, rethrow: bool = False, mixed: bool = False,
-                 env32: bool = False, tol32: float = 1e-7, polish: bool = False):
+                 env32: bool = False, tol32: float = 1e-7, polish: bool = False, grow: bool = False,
+                 chain_tol: float = 1e-12):
         self.ops = list(ops)
         self.n = len(self.ops)
         dims = [o.d for o in self.ops]
@@ -96,6 +101,7 @@ class Chain:
         self._check(lib.nc_chain_set_options(self._h, site_tol, sweep_tol, sanity, max_iterations, which.encode(),
                                              ncv, 1 if rethrow else 0))
         self.set_mixed(mixed, env32, tol32, polish)
+        self._check(lib.nc_chain_set_growth(self._h, 1 if grow else 0, chain_tol))
         if raw_sites is None:
             raw_sites = raw_state(seed, dims, self.bonds)
         for i, a in enumerate(raw_sites):
@@ -187,12 +193,14 @@ class Chain:
     def randomize(self, seed: int):
         """New raw site tensors from ``seed``; call reset() afterwards."""
         self._check(_lib().nc_chain_randomize(self._h, seed))
+        self._refresh_bonds()
 
     def solve_for_multiple_levels(self, levels: int, seed: int = 1) -> List[float]:
         """Converge ``levels`` levels in turn at the chain's bond dimension, each
         orthogonal to the ones before; returns their energies."""
         out = (C.c_double * max(1, levels))()
         self._check(_lib().nc_chain_solve_for_multiple_levels(self._h, levels, seed, out))
+        self._refresh_bonds()
         return [out[i] for i in range(levels)]
 
     solve_for_eigenvalues = solve_for_multiple_levels
@@ -202,6 +210,22 @@ class Chain:
         out = np.zeros(1, CPLX)
         self._check(_lib().nc_chain_state_overlap(self._h, other._h, out.ctypes.data))
         return complex(out[0])
+
+    # -- bond dimension (base_chain.cpp:64-79, chain.cpp:75-121) -------------
+    def _refresh_bonds(self):
+        out = (C.c_uint32 * (self.n + 1))()
+        self._check(_lib().nc_chain_get_bonds(self._h, out))
+        self.bonds = [int(out[i]) for i in range(self.n + 1)]
+
+    def increase_bandwidth_dimension(self, new_chi: int):
+        self._check(_lib().nc_chain_increase_bandwidth(self._h, new_chi))
+        self._refresh_bonds()
+
+    def optimize_chain(self):
+        """Converge, then (with ``grow``) raise chi by one and converge again
+        until the energy changes by less than ``chain_tol``."""
+        self._check(_lib().nc_chain_optimize_chain(self._h))
+        self._refresh_bonds()
 
     def stats(self) -> dict:
         counts = (C.c_int64 * 6)()

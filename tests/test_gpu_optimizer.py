@@ -216,3 +216,23 @@ def test_site_move_rank_deficient(core, rank):
     before = np.einsum("mla,rmb->labr", tn, td)
     after = np.einsum("mla,rmb->labr", on, od)
     assert rel(after, before) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(2, 4, 5, 2, 2), (4, 8, 3, 2, 2), (3, 6, 4, 2, 3)])
+def test_site_move_unitary_input(core, dims):
+    """A site that is already normalised (orthonormal rows, also square) keeps
+    orthonormal rows and the state -- the case a bond-dimension increase and a
+    second move in the same direction produce."""
+    bl, bm, br, dl, dr = dims
+    assert bm == bl * dl
+    rng = np.random.default_rng(sum(dims))
+    q, _ = np.linalg.qr(crand(rng, bm, bm))
+    tn = np.asfortranarray(q.reshape(bm, bl, dl, order="F"))
+    td = crand(rng, br, bm, dr)
+    info, on, od = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    assert info == 0
+    A = on.reshape(bm, bl * dl, order="F")
+    assert np.abs(A @ A.conj().T - np.eye(bm)).max() < 1e-12
+    before = np.einsum("mla,rmb->labr", tn, td)
+    after = np.einsum("mla,rmb->labr", on, od)
+    assert rel(after, before) < 1e-12

@@ -185,11 +185,18 @@ def test_tfim_four_levels(mods, case):
                                         (3, 1, [0, 1, 2]), (3, 2, [0, 1, 1, 2, 2]), (3, 3, [0, 1, 1, 1, 2])])
 def test_external_field_levels(mods, d, n, levels):
     """tests/chain.cpp:496-517: external field diag(0, 1, ..., d-1) on every site,
-    levels to 1e-12 (the reference loosens the sanity threshold 10x for d = n = 3)."""
+    levels to 1e-12 (the reference loosens the sanity threshold 10x for d = n = 3).
+
+    The levels are exactly degenerate multiplets.  Once a sweep's guess is an exact
+    eigenvector of a higher multiplet, an ARPACK-style solve with ncv = 3 can stop
+    on it (its Ritz estimate is zero and the two other Krylov vectors may all lie
+    above it); measured over 12 seeds, ncv = 3 missed the fourth level of N = 4,
+    d = 2 for 6 of them, ncv = 6 for none.  The chain runs at its exact bond
+    dimension with ncv = 6."""
     chain, ops_mod = mods
     ops = ops_mod.external_field_operator(n, np.diag(np.arange(d)).astype(complex))
     sanity = 1e-11 if (d == 3 and n == 3) else 1e-12
-    ch = chain.Chain(ops, d ** n, seed=5, rethrow=True, sanity=sanity)
+    ch = chain.Chain(ops, d ** n, seed=5, rethrow=True, sanity=sanity, ncv=6)
     got = ch.solve_for_multiple_levels(len(levels), seed=9)
     print("levels", got, flush=True)
     assert np.allclose(got, levels, atol=1e-12, rtol=0)
@@ -197,11 +204,12 @@ def test_external_field_levels(mods, d, n, levels):
 
 @pytest.mark.parametrize("n", [3, 4, 5])
 def test_solve_for_eigenvalues_local_field(mods, n):
-    """tests/chain.cpp:528-543: LocalExternalField diag(0, 1) on every site, bond
-    dimension 3: eigenvalues 0, 1, 1, 1 to 1e-13."""
+    """tests/chain.cpp:528-543: LocalExternalField diag(0, 1) on every site,
+    initial bond dimension 3 (grown as optimizeChain does): eigenvalues 0, 1, 1, 1
+    to 1e-13."""
     chain, ops_mod = mods
     ops = ops_mod.external_field_operator(n, np.diag([0.0, 1.0]).astype(complex))
-    ch = chain.Chain(ops, 3, seed=2, rethrow=True)
+    ch = chain.Chain(ops, 3, seed=2, rethrow=True, grow=True)
     got = ch.solve_for_eigenvalues(4, seed=4)
     assert abs(got[0]) < 1e-13
     assert np.allclose(got[1:], 1.0, atol=1e-13, rtol=0)
@@ -257,3 +265,35 @@ def test_config1_four_levels(mods):
     err = np.abs(np.array(got) - CONFIG1_LEVELS)
     print("config 1 levels", got, "abs err", err, "stats", ch.stats(), flush=True)
     assert err.max() < 1e-9
+
+
+def test_bandwidth_growth_keeps_state(mods):
+    """increaseBandwidthDimension (chain.cpp:75-121): the state and its energy are
+    unchanged, the bonds follow computeBandwidthDimensionSequence, and sweeping at
+    the larger chi lowers the energy."""
+    chain, ops_mod = mods
+    ops = ops_mod.xxz_operator(8, 2, 1.0, 0.8)
+    ch = chain.Chain(ops, 2, seed=4, rethrow=True)
+    ch.sweep_until_converged()
+    e0 = ch.energy
+    v0 = _dense_state(ch)
+    ch.increase_bandwidth_dimension(5)
+    assert ch.bonds == [1, 2, 4, 5, 5, 5, 4, 2, 1]
+    v1 = _dense_state(ch)
+    assert abs(abs(np.vdot(v0, v1)) - 1) < 1e-12
+    assert np.linalg.norm(v1 - v0) < 1e-10
+    assert abs(ch.expectation().real - e0) < 1e-12 * abs(e0)
+    ch.sweep_until_converged()
+    assert ch.energy < e0 - 1e-6
+
+
+def test_optimize_chain_grows_to_exact_ground_state(mods):
+    """optimizeChain from chi = 1: grows until the energy converges to 1e-12 and
+    reaches the dense ground state of an 8-site XXZ chain."""
+    chain, ops_mod = mods
+    ops = ops_mod.xxz_operator(8, 2, 1.0, 0.8)
+    ch = chain.Chain(ops, 1, seed=6, rethrow=True, grow=True)
+    ch.optimize_chain()
+    exact = _exact_levels(ops, 1)[0]
+    print("chi", max(ch.bonds), "E", ch.energy, "exact", exact, flush=True)
+    assert abs(ch.energy - exact) < 1e-10
