@@ -266,7 +266,20 @@ int nc_chain_solve_for_multiple_levels(nc_chain* chain, uint32_t levels, uint64_
 /* <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382): the
    chains need the same length and physical dimensions, not the same bonds. */
 int nc_chain_state_overlap(nc_chain* a, nc_chain* b, nc_zcomplex* overlap);
-/* counts (6): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs;
+/* Independent chains of the same length (a parameter sweep: each its own
+   couplings and bond dimension) sweeping in lockstep: all chains sit on the same
+   site, and their site solves share grouped kernel launches.  A chain whose
+   energy has converged leaves the lockstep, so each performs exactly the sweeps
+   it would perform alone.  The chains stay owned by the caller (destroy the
+   batch first); every chain must be at its first site and have no lower levels. */
+typedef struct nc_chain_batch nc_chain_batch;
+nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n);
+void nc_chain_batch_destroy(nc_chain_batch* batch);
+int nc_chain_batch_sweep(nc_chain_batch* batch);
+int nc_chain_batch_sweep_until_converged(nc_chain_batch* batch);
+/* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,
+   sites of the mixed form finished in FP64 (a complex64 solve that came out above
+   the previous energy);
    times_ms (4): optimize, site move, environment update, reset */
 int nc_chain_stats(nc_chain* chain, int64_t* counts, double* times_ms);
 

@@ -34,6 +34,7 @@ SOURCES = [
     "polar.cpp",
     "overlap.cpp",
     "chain.cpp",
+    "batch.cpp",
     "abi.cpp",
     "chain_abi.cpp",
     "bench_abi.cpp",

@@ -294,45 +294,47 @@ void Chain::move_left() {
     reset_projector_matrix();
 }
 
-int Chain::optimize_site() {
+void Chain::prepare_site(SitePrep& p) {
     hipStream_t s = Context::get().stream;
     const int i = current;
-    double t0 = now_ms();
+    p.t0 = now_ms();
     const int bl = bonds[i], br = bonds[i + 1];
     MpoSite const& op = *ops[i];
-    ZBuf result(site_size(i));
-    Projector none;
-    none.n = (int)site_size(i);
-    none.orth = (int)site_size(i);
-    Projector const* proj = projectors.empty() ? &none : &site_projector;
-    OptimizeArgs a{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, proj,
-                   opts.which, opts.site_tol, opts.max_iterations, sites[i].ptr, result.ptr, opts.ncv};
-    DevBuf<float2> L32, R32;
+    p.result.resize(site_size(i));
+    p.none.n = (int)site_size(i);
+    p.none.orth = (int)site_size(i);
+    p.proj = projectors.empty() ? &p.none : &site_projector;
+    p.a = OptimizeArgs{bl, br, op.cl, op.cr, op.d, left_envs[i].ptr, right_envs[i + 1].ptr, &op, p.proj,
+                       opts.which, opts.site_tol, opts.max_iterations, sites[i].ptr, p.result.ptr, opts.ncv};
     if (opts.mixed) {
         const size_t nl = (size_t)bl * bl * op.cl, nr = (size_t)br * br * op.cr;
-        L32.resize(nl);
-        R32.resize(nr);
-        to_c64(left_envs[i].ptr, L32.ptr, nl, s);
-        to_c64(right_envs[i + 1].ptr, R32.ptr, nr, s);
-        a.L32 = L32.ptr;
-        a.R32 = R32.ptr;
-        a.mixed = true;
-        a.tol32 = opts.tol32;
-        a.polish = opts.polish;
+        p.L32.resize(nl);
+        p.R32.resize(nr);
+        to_c64(left_envs[i].ptr, p.L32.ptr, nl, s);
+        to_c64(right_envs[i + 1].ptr, p.R32.ptr, nr, s);
+        p.a.L32 = p.L32.ptr;
+        p.a.R32 = p.R32.ptr;
+        p.a.mixed = true;
+        p.a.tol32 = opts.tol32;
+        p.a.polish = opts.polish;
     }
-    OptimizeOut o = nc::optimize_site(a, ws, s);
-    stats.matvecs += o.matvecs;
-    stats.matvecs32 += o.matvecs32;
-    ++stats.site_solves;
+}
+
+int Chain::check_site(SitePrep& p, OptimizeOut const& o) {
+    // optimizeStateSite post-checks (optimizer.cpp:265-310) and the regression
+    // check of BaseChain::optimizeSite (base_chain.cpp:85-100)
+    hipStream_t s = Context::get().stream;
+    const int i = current;
+    const int bl = bonds[i], br = bonds[i + 1];
+    MpoSite const& op = *ops[i];
     int status = o.info;
     // complex64 environments are Hermitian only to FP32 round-off and carry its
     // noise into the energy: the complex-eigenvalue and regression checks and the
     // sweep convergence test use that floor
     const double floor_tol = opts.mixed && opts.env32 ? std::max(opts.sanity, opts.env32_floor) : opts.sanity;
     if (status == 0) {
-        // optimizeStateSite post-checks (optimizer.cpp:265-300)
         zd ev = o.eigenvalue;
-        zd ex = expectation(bl, br, op, left_envs[i].ptr, result.ptr, right_envs[i + 1].ptr, ws, s);
+        zd ex = expectation(bl, br, op, left_envs[i].ptr, p.result.ptr, right_envs[i + 1].ptr, ws, s);
         const double san = opts.sanity;
         if ((std::abs(ev) + std::abs(ex)) / 2 > san && std::abs(ev - ex) / (std::abs(ev) + std::abs(ex) + san) > san)
             status = 101;  // OptimizerObtainedEigenvalueDifferentFromExpectationValue
@@ -340,30 +342,62 @@ int Chain::optimize_site() {
             status = 102;  // OptimizerObtainedComplexEigenvalue
         else if (o.normal < 1 - san)
             status = 103;  // OptimizerObtainedVanishingEigenvector
-        else if (proj->active()) {
+        else if (p.proj->active()) {
             ZBuf tmp(site_size(i));
-            if (proj->overlap(result.ptr, tmp.ptr, s) > san)
+            if (p.proj->overlap(p.result.ptr, tmp.ptr, s) > san)
                 status = 105;  // OptimizerObtainedEigenvectorInProjectorSpace
         }
     }
+    if (status == 0 && regression(opts.which, energy, o.eigenvalue.real(), floor_tol))
+        status = 104;  // OptimizerObtainedRegressiveEigenvalue
+    return status;
+}
+
+int Chain::finish_site(SitePrep& p, OptimizeOut const& o_in) {
+    hipStream_t s = Context::get().stream;
+    const int i = current;
+    OptimizeOut o = o_in;
+    stats.matvecs += o.matvecs;
+    stats.matvecs32 += o.matvecs32;
+    ++stats.site_solves;
+    int status = check_site(p, o);
+    if (status == 104 && p.a.mixed) {
+        // a complex64 Krylov vector converged to 1e-7 can sit measurably above
+        // the previous energy when the site's lowest pair is nearly degenerate:
+        // finish that site in FP64, starting from it
+        OptimizeArgs a2 = p.a;
+        a2.mixed = false;
+        a2.polish = false;
+        ZBuf guess(site_size(i));
+        d2d(guess.ptr, p.result.ptr, site_size(i) * sizeof(double2));
+        a2.guess = guess.ptr;
+        o = nc::optimize_site(a2, ws, s);
+        stats.matvecs += o.matvecs;
+        ++stats.fp64_refinements;
+        status = check_site(p, o);
+    }
     if (status == 0) {
+        // acceptance rule of BaseChain::optimizeSite (base_chain.cpp:101-105)
         const double e = o.eigenvalue.real();
-        if (regression(opts.which, energy, e, floor_tol)) {
-            status = 104;  // OptimizerObtainedRegressiveEigenvalue
-        } else {
-            if ((energy >= 0 && e >= 0) || (energy <= 0 && e <= 0) ||
-                outside_tolerance(std::abs(energy), std::abs(e), opts.sanity)) {
-                energy = e;
-                d2d(sites[i].ptr, result.ptr, site_size(i) * sizeof(double2));
-            }
-            optimized = true;
+        if ((energy >= 0 && e >= 0) || (energy <= 0 && e <= 0) ||
+            outside_tolerance(std::abs(energy), std::abs(e), opts.sanity)) {
+            energy = e;
+            d2d(sites[i].ptr, p.result.ptr, site_size(i) * sizeof(double2));
         }
+        optimized = true;
     }
     NC_HIP(hipStreamSynchronize(s));
-    stats.t_opt_ms += now_ms() - t0;
+    stats.t_opt_ms += now_ms() - p.t0;
     if (status != 0) ++stats.failures;
     last_status = status;
     return status;
+}
+
+int Chain::optimize_site() {
+    SitePrep p;
+    prepare_site(p);
+    OptimizeOut o = nc::optimize_site(p.a, ws, Context::get().stream);
+    return finish_site(p, o);
 }
 
 // ---------------------------------------------------------------------------

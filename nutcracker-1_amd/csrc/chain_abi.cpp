@@ -12,6 +12,10 @@ struct nc_chain {
     std::unique_ptr<Chain> chain;
 };
 
+struct nc_chain_batch {
+    std::unique_ptr<ChainBatch> batch;
+};
+
 namespace {
 template <class F>
 int guarded_chain(const char* name, F&& f) {
@@ -203,6 +207,29 @@ int nc_chain_state_overlap(nc_chain* a, nc_chain* b, nc_zcomplex* out) {
     });
 }
 
+nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n) {
+    nc_chain_batch* out = nullptr;
+    guarded_chain("nc_chain_batch_create", [&] {
+        std::vector<Chain*> cs;
+        for (uint32_t i = 0; i < n; ++i) cs.push_back(chains[i]->chain.get());
+        auto* b = new nc_chain_batch();
+        b->batch = std::make_unique<ChainBatch>(std::move(cs));
+        out = b;
+        return 0;
+    });
+    return out;
+}
+
+void nc_chain_batch_destroy(nc_chain_batch* b) { delete b; }
+
+int nc_chain_batch_sweep(nc_chain_batch* b) {
+    return guarded_chain("nc_chain_batch_sweep", [&] { b->batch->sweep_all(); return 0; });
+}
+
+int nc_chain_batch_sweep_until_converged(nc_chain_batch* b) {
+    return guarded_chain("nc_chain_batch_sweep_until_converged", [&] { b->batch->sweep_until_converged(); return 0; });
+}
+
 int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
     return guarded_chain("nc_chain_stats", [&] {
         auto const& s = c->chain->stats;
@@ -212,6 +239,7 @@ int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
         counts[3] = s.sweeps;
         counts[4] = s.failures;
         counts[5] = s.matvecs32;
+        counts[6] = s.fp64_refinements;
         times_ms[0] = s.t_opt_ms;
         times_ms[1] = s.t_move_ms;
         times_ms[2] = s.t_env_ms;

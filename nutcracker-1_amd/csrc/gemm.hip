@@ -59,6 +59,7 @@ struct BatchT {
     typename Cplx<T>::type* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
+    int32_t M, N;  // output size of this batch (filled in on the host)
 };
 
 template <typename T, int OPA, int OPB, int BK>
@@ -128,22 +129,26 @@ __device__ __forceinline__ void store_tiles(int tid, typename Cplx<T>::type (*As
 
 template <typename T, int OPA, int OPB, bool ACC, bool M3, int BK>
 __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict__ batches,
-                                                    const SegT<T>* __restrict__ segs, int M, int N, int tiles_m,
-                                                    int tiles_n) {
+                                                    const SegT<T>* __restrict__ segs) {
     typedef typename Cplx<T>::type CT;
     typedef typename Cplx<T>::v4 V4;
     __shared__ CT As[BK][TBM + TPAD];
     __shared__ CT Bs[BK][TBN + TPAD];
 
-    // XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
-    // contiguous range of tiles (row-major over the tile grid)
+    // every batch has its own M x N (grouped launch of unequal problems): the
+    // grid covers the largest, surplus workgroups of smaller batches exit
+    const BatchT<T> bt = batches[blockIdx.y];
+    const int M = bt.M, N = bt.N;
+    const int tiles_m = (M + TBM - 1) / TBM, tiles_n = (N + TBN - 1) / TBN;
     const int ntiles = tiles_m * tiles_n;
     const int w = blockIdx.x;
+    if (w >= ntiles) return;
+    // XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
+    // contiguous range of tiles (row-major over the tile grid)
     int tile = w;
-    if ((ntiles & 7) == 0) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    if ((ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
     const int tm = tile / tiles_n, tn = tile % tiles_n;
 
-    const BatchT<T> bt = batches[blockIdx.y];
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -317,20 +322,19 @@ Arena& arena() {
 }
 
 template <typename T, int OPA, int OPB>
-void launch_typed(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, int M, int N, int tm, int tn,
-                  hipStream_t st) {
+void launch_typed(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
 #define NC_L(ACCV, M3V) \
-    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, 16>), grid, dim3(256), 0, st, b, s, M, N, tm, tn)
+    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, 16>), grid, dim3(256), 0, st, b, s)
     if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
     else { if (m3) NC_L(false, true); else NC_L(false, false); }
 #undef NC_L
 }
 
 template <typename T>
-void launch_any(int opA, int opB, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, int M, int N,
-                int tm, int tn, hipStream_t st) {
+void launch_any(int opA, int opB, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
+                hipStream_t st) {
 #define NC_O(a, bb) \
-    if (opA == a && opB == bb) { launch_typed<T, a, bb>(acc, m3, grid, b, s, M, N, tm, tn, st); return; }
+    if (opA == a && opB == bb) { launch_typed<T, a, bb>(acc, m3, grid, b, s, st); return; }
     NC_O(OpN, OpN) NC_O(OpN, OpT) NC_O(OpN, OpC) NC_O(OpT, OpN) NC_O(OpC, OpN) NC_O(OpC, OpC) NC_O(OpT, OpT)
 #undef NC_O
     throw Error("GEMM: unsupported operand combination", -71);
@@ -371,12 +375,24 @@ void split_segments(std::vector<Seg> const& segs, int sb, int se, int nsplit, in
 }
 
 template <typename T>
-void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>> const& batches,
+void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
               std::vector<SegT<T>> const& segs, hipStream_t s) {
     typedef typename Cplx<T>::type CT;
-    if (batches.empty() || M <= 0 || N <= 0) return;
-    const int tm = (M + TBM - 1) / TBM, tn = (N + TBN - 1) / TBN;
-    const int64_t tiles = (int64_t)tm * tn * (int64_t)batches.size();
+    if (batches.empty()) return;
+    // per-batch sizes (0 = the plan's); a grouped launch of unequal batches
+    bool uniform = true;
+    int64_t maxtiles = 0, tiles = 0;
+    for (auto& b : batches) {
+        if (b.M == 0) b.M = M;
+        if (b.N == 0) b.N = N;
+        if (b.M != batches[0].M || b.N != batches[0].N) uniform = false;
+        const int64_t t = (int64_t)((b.M + TBM - 1) / TBM) * ((b.N + TBN - 1) / TBN);
+        maxtiles = std::max(maxtiles, t);
+        tiles += t;
+    }
+    if (maxtiles <= 0) return;
+    M = batches[0].M;
+    N = batches[0].N;
     int64_t maxk = 0;
     for (auto const& b : batches) {
         int64_t k = 0;
@@ -385,7 +401,7 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
     }
     // split K when the launch would leave most of the chip idle
     int nsplit = 1;
-    if (g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
+    if (uniform && g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
         while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
     }
     const bool m3 = sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32;
@@ -396,9 +412,9 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
         size_t offs[2];
         int slot;
         char* d = ar.stage(parts, sizes, 2, offs, s, &slot);
-        dim3 grid((unsigned)(tm * tn), (unsigned)batches.size());
-        launch_any<T>(opA, opB, accumulate, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), M,
-                      N, tm, tn, s);
+        dim3 grid((unsigned)maxtiles, (unsigned)batches.size());
+        launch_any<T>(opA, opB, accumulate, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]),
+                      s);
         NC_HIP(hipGetLastError());
         ar.release(slot, s);
         return;
@@ -420,6 +436,8 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
             BatchT<T> x;
             x.C = W + ((int64_t)b * nsplit + q) * slice;
             x.ldc = M;
+            x.M = M;
+            x.N = N;
             x.seg_begin = (int32_t)ns.size();
             for (auto const& sg : chunks[q]) ns.push_back(sg);
             x.seg_end = (int32_t)ns.size();
@@ -434,9 +452,8 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
     size_t offs[4];
     int slot;
     char* d = ar.stage(parts, sizes, 4, offs, s, &slot);
-    dim3 grid((unsigned)(tm * tn), (unsigned)nb.size());
-    launch_any<T>(opA, opB, false, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), M, N, tm,
-                  tn, s);
+    dim3 grid((unsigned)maxtiles, (unsigned)nb.size());
+    launch_any<T>(opA, opB, false, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), s);
     NC_HIP(hipGetLastError());
     unsigned g = (unsigned)std::min<int64_t>(2048, (slice + 255) / 256);
     hipLaunchKernelGGL(splitk_reduce_kernel<CT>, dim3(g, (unsigned)batches.size()), dim3(256), 0, s, W, slice, nsplit,
@@ -448,6 +465,14 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
 }  // namespace
 
 GemmTuning& gemm_tuning() { return g_tuning; }
+
+void* stage_descriptors(const void* host, size_t bytes, hipStream_t s, int* slot) {
+    const void* parts[1] = {host};
+    size_t sizes[1] = {bytes};
+    size_t offs[1];
+    return arena().stage(parts, sizes, 1, offs, s, slot) + offs[0];
+}
+void release_descriptors(int slot, hipStream_t s) { arena().release(slot, s); }
 
 void zgemm_run(GemmPlan const& plan, hipStream_t s) {
     static_assert(sizeof(BatchT<double>) == sizeof(GemmBatch), "descriptor layout");
@@ -462,7 +487,7 @@ void zgemm_run(GemmPlan const& plan, hipStream_t s) {
 void cgemm_run(CGemmPlan const& plan, hipStream_t s) {
     std::vector<BatchT<float>> b;
     std::vector<SegT<float>> sg;
-    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end});
+    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N});
     for (auto const& x : plan.segs) sg.push_back(SegT<float>{x.A, x.B, x.lda, x.ldb, x.K, 0});
     run_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
 }

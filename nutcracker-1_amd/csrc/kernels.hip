@@ -73,6 +73,123 @@ void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q,
     mix_blocks_t<float2>(op, src_is_from, P, Q, rs, op.mix_W32[src_is_from ? 0 : 1].ptr, s);
 }
 
+// the same mixing for several chains: blockIdx.z selects the chain
+template <typename CT>
+struct MixBatchDesc {
+    const CT* P;
+    CT* Q;
+    const int* dst_ptr;
+    const int* blk_src;
+    const CT* W;
+    int64_t rs;
+    int ndst, d;
+};
+
+template <typename CT, int D>
+__global__ __launch_bounds__(256) void mix_batch_kernel(const MixBatchDesc<CT>* __restrict__ descs) {
+    const MixBatchDesc<CT> md = descs[blockIdx.z];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int dst = blockIdx.y;
+    if (dst >= md.ndst || e >= md.rs) return;
+    CT q[D];
+    for (int b = 0; b < D; ++b) { q[b].x = 0; q[b].y = 0; }
+    for (int t = md.dst_ptr[dst]; t < md.dst_ptr[dst + 1]; ++t) {
+        const int src = md.blk_src[t];
+        const CT* Wn = md.W + (int64_t)t * D * D;
+        for (int a = 0; a < D; ++a) {
+            const CT p = md.P[((int64_t)src * D + a) * md.rs + e];
+            for (int b = 0; b < D; ++b) {
+                const CT w = Wn[a + b * D];
+                q[b].x += w.x * p.x - w.y * p.y;
+                q[b].y += w.x * p.y + w.y * p.x;
+            }
+        }
+    }
+    for (int b = 0; b < D; ++b) md.Q[((int64_t)dst * D + b) * md.rs + e] = q[b];
+}
+
+template <typename CT>
+static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&), hipStream_t s) {
+    if (items.empty()) return;
+    const int d = items[0].op->d;
+    std::vector<MixBatchDesc<CT>> ds;
+    int64_t maxrs = 0;
+    int maxdst = 0;
+    for (auto const& it : items) {
+        if (it.op->d != d) throw Error("mix_blocks_batch: chains with different physical dimensions", -4);
+        MixBatchDesc<CT> m;
+        m.P = it.P;
+        m.Q = it.Q;
+        m.dst_ptr = it.op->mix_ptr[0].ptr;
+        m.blk_src = it.op->mix_src[0].ptr;
+        m.W = W(*it.op);
+        m.rs = it.rs;
+        m.ndst = (int)it.op->tos.size();
+        m.d = d;
+        ds.push_back(m);
+        maxrs = std::max(maxrs, it.rs);
+        maxdst = std::max(maxdst, m.ndst);
+    }
+    if (maxrs == 0 || maxdst == 0) return;
+    int slot;
+    auto* dd = (const MixBatchDesc<CT>*)stage_descriptors(ds.data(), ds.size() * sizeof(MixBatchDesc<CT>), s, &slot);
+    dim3 grid((unsigned)((maxrs + 255) / 256), (unsigned)maxdst, (unsigned)ds.size());
+    switch (d) {
+        case 2: hipLaunchKernelGGL((mix_batch_kernel<CT, 2>), grid, dim3(256), 0, s, dd); break;
+        case 3: hipLaunchKernelGGL((mix_batch_kernel<CT, 3>), grid, dim3(256), 0, s, dd); break;
+        case 4: hipLaunchKernelGGL((mix_batch_kernel<CT, 4>), grid, dim3(256), 0, s, dd); break;
+        default: throw Error("mix_blocks_batch: physical dimension above 4 is not supported", -4);
+    }
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+}
+
+static const double2* mixw64(MpoSite const& op) { return op.mix_W[0].ptr; }
+static const float2* mixw32(MpoSite const& op) { return op.mix_W32[0].ptr; }
+void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s) { mix_batch_t(items, mixw64, s); }
+void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s) { mix_batch_t(items, mixw32, s); }
+
+template <typename A, typename B>
+struct ConvDesc {
+    const A* src;
+    B* dst;
+    int64_t n;
+};
+__device__ __forceinline__ float2 conv(double2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ double2 conv(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <typename A, typename B>
+__global__ void convert_batch_kernel(const ConvDesc<A, B>* __restrict__ ds) {
+    const ConvDesc<A, B> c = ds[blockIdx.y];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (int64_t)gridDim.x * blockDim.x)
+        c.dst[i] = conv(c.src[i]);
+}
+template <typename A, typename B>
+static void convert_batch(std::vector<const A*> const& src, std::vector<B*> const& dst, std::vector<int64_t> const& n,
+                          hipStream_t s) {
+    if (src.empty()) return;
+    std::vector<ConvDesc<A, B>> ds(src.size());
+    int64_t maxn = 0;
+    for (size_t q = 0; q < src.size(); ++q) {
+        ds[q] = ConvDesc<A, B>{src[q], dst[q], n[q]};
+        maxn = std::max(maxn, n[q]);
+    }
+    if (maxn == 0) return;
+    int slot;
+    auto* dd = (const ConvDesc<A, B>*)stage_descriptors(ds.data(), ds.size() * sizeof(ConvDesc<A, B>), s, &slot);
+    const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)src.size()), (maxn + 255) / 256);
+    hipLaunchKernelGGL((convert_batch_kernel<A, B>), dim3(g, (unsigned)src.size()), dim3(256), 0, s, dd);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+}
+void to_c64_batch(std::vector<const double2*> const& src, std::vector<float2*> const& dst,
+                  std::vector<int64_t> const& n, hipStream_t s) {
+    convert_batch<double2, float2>(src, dst, n, s);
+}
+void to_c128_batch(std::vector<const float2*> const& src, std::vector<double2*> const& dst,
+                   std::vector<int64_t> const& n, hipStream_t s) {
+    convert_batch<float2, double2>(src, dst, n, s);
+}
+
 __global__ void to_c64_kernel(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         b[i] = make_float2((float)a[i].x, (float)a[i].y);

@@ -31,6 +31,7 @@ struct ChainOptions {
 
 struct ChainStats {
     int64_t matvecs = 0, site_solves = 0, moves = 0, sweeps = 0, failures = 0, matvecs32 = 0;
+    int64_t fp64_refinements = 0;  // mixed form: sites finished in FP64 after a regressive complex64 solve
     double t_opt_ms = 0, t_move_ms = 0, t_env_ms = 0, t_reset_ms = 0;
 };
 
@@ -41,6 +42,16 @@ struct ChainStats {
 struct LevelProjector {
     std::vector<ZBuf> left, middle, right;
     std::vector<int> bonds;
+};
+
+// one site solve in preparation: the optimizer's arguments and their buffers
+struct SitePrep {
+    OptimizeArgs a{};
+    ZBuf result;
+    DevBuf<float2> L32, R32;
+    Projector none;
+    Projector const* proj = nullptr;
+    double t0 = 0;
 };
 
 struct Chain {
@@ -93,7 +104,23 @@ struct Chain {
     void move_right();
     void move_left();
     int optimize_site();
+    void prepare_site(SitePrep& p);
+    int finish_site(SitePrep& p, OptimizeOut const& o);
+    int check_site(SitePrep& p, OptimizeOut const& o);
+    bool converged_flag = false;  // used by the lockstep driver
     void sweep();
+    void sweep_until_converged();
+};
+
+// Independent chains (same length) sweeping in lockstep with batched site
+// solves; the chains are owned by the caller.
+struct ChainBatch {
+    std::vector<Chain*> chains;
+    int64_t lockstep_rounds = 0;
+    explicit ChainBatch(std::vector<Chain*> cs);
+    void optimize_sites(std::vector<Chain*> const& active);
+    void sweep(std::vector<Chain*> const& active);
+    void sweep_all();
     void sweep_until_converged();
 };
 

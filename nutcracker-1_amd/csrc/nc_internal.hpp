@@ -105,6 +105,7 @@ struct GemmBatch {
     double2* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
+    int32_t M, N;  // this batch's output size; 0 = the plan's M, N
 };
 
 enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
@@ -128,8 +129,8 @@ struct GemmPlan {
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
     void clear() { batches.clear(); segs.clear(); }
-    void begin_batch(double2* C, int64_t ldc) {
-        batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size()});
+    void begin_batch(double2* C, int64_t ldc, int M_ = 0, int N_ = 0) {
+        batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_});
     }
     void add_seg(const double2* A, int64_t lda, const double2* B, int64_t ldb, int K) {
         segs.push_back(GemmSeg{A, B, lda, ldb, K, 0});
@@ -152,14 +153,15 @@ struct CGemmBatch {
     float2* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
+    int32_t M, N;  // 0 = the plan's M, N
 };
 struct CGemmPlan {
     std::vector<CGemmBatch> batches;
     std::vector<CGemmSeg> segs;
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
-    void begin_batch(float2* C, int64_t ldc) {
-        batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size()});
+    void begin_batch(float2* C, int64_t ldc, int M_ = 0, int N_ = 0) {
+        batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_});
     }
     void add_seg(const float2* A, int64_t lda, const float2* B, int64_t ldb, int K) {
         segs.push_back(CGemmSeg{A, B, lda, ldb, K, 0});
@@ -188,6 +190,27 @@ struct MpoSite {
 
 // Mix kernel: Q(e, b, dst) = sum_{n : dst(n) = dst} sum_a W_n(a, b) P(e, a, src(n))
 // where e runs over an element slab of size rs.  P has d * nsrc slabs, Q d * ndst.
+// Small descriptor lists for a launch: copied through a pinned ring with a
+// stream-ordered copy (no host synchronisation); release after the launch.
+void* stage_descriptors(const void* host, size_t bytes, hipStream_t s, int* slot);
+void release_descriptors(int slot, hipStream_t s);
+
+// The mix step of several chains' matvecs in one launch (src = from blocks).
+template <typename CT>
+struct MixItem {
+    MpoSite const* op;
+    const CT* P;
+    CT* Q;
+    int64_t rs;
+};
+void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s);
+void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s);
+// element-wise precision changes of several vectors in one launch
+void to_c64_batch(std::vector<const double2*> const& src, std::vector<float2*> const& dst,
+                  std::vector<int64_t> const& n, hipStream_t s);
+void to_c128_batch(std::vector<const float2*> const& src, std::vector<double2*> const& dst,
+                   std::vector<int64_t> const& n, hipStream_t s);
+
 void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q,
                 int64_t rs, hipStream_t s);
 void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q,
@@ -398,6 +421,9 @@ struct OptimizeOut {
     zd eigenvalue{0, 0};
     double normal = 0;
 };
+// Site solves of several independent chains in lockstep (grouped launches).
+void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::vector<Workspace*> const& wss,
+                            std::vector<OptimizeOut>& outs, hipStream_t s);
 OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
 
 // ----------------------------------------------------------------------------

@@ -369,3 +369,167 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
 }
 
 }  // namespace nc
+
+namespace nc {
+
+// ----------------------------------------------------------------------------
+// Site solves of several chains in lockstep.  Every Arnoldi step applies all
+// active H_eff with one grouped GEMM launch for step A (every chain's (from, a)
+// products, unequal sizes), one mix launch and one grouped launch for step C;
+// the orthogonalisation of all chains shares its launches too (iram_batch).
+// Sites with projectors or a dense-solve size (orth <= 4) are solved one by one.
+// ----------------------------------------------------------------------------
+void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::vector<Workspace*> const& wss,
+                            std::vector<OptimizeOut>& outs, hipStream_t s) {
+    const size_t C = args.size();
+    outs.assign(C, OptimizeOut());
+    std::vector<size_t> idx;  // the chains solved together
+    for (size_t c = 0; c < C; ++c) {
+        OptimizeArgs const& a = *args[c];
+        const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+        if (a.proj->active() || nfull <= 4) outs[c] = optimize_site(a, *wss[c], s);
+        else idx.push_back(c);
+    }
+    if (idx.empty()) return;
+    // group by Krylov size
+    std::vector<int> ncvs;
+    for (size_t c : idx) {
+        OptimizeArgs const& a = *args[c];
+        const int nc = std::min<int64_t>(a.ncv, (int64_t)a.d * a.bl * a.br);
+        if (std::find(ncvs.begin(), ncvs.end(), nc) == ncvs.end()) ncvs.push_back(nc);
+    }
+    for (int ncv : ncvs) {
+        std::vector<size_t> grp;
+        for (size_t c : idx) {
+            OptimizeArgs const& a = *args[c];
+            if (std::min<int64_t>(a.ncv, (int64_t)a.d * a.bl * a.br) == ncv) grp.push_back(c);
+        }
+        const bool mixed = args[grp[0]]->mixed;
+        std::vector<std::unique_ptr<HeffOperator>> op64;
+        std::vector<std::unique_ptr<Heff32Operator>> op32;
+        std::vector<IramTask> tasks(grp.size());
+        for (size_t q = 0; q < grp.size(); ++q) {
+            OptimizeArgs const& a = *args[grp[q]];
+            if (a.mixed != mixed) throw Error("optimize_sites_batched: mixed and FP64 chains in one group", -33);
+            op64.emplace_back(new HeffOperator(a, *wss[grp[q]]));
+            if (mixed) op32.emplace_back(new Heff32Operator(a, *wss[grp[q]]));
+            const int64_t rs = (int64_t)a.br * a.bl;
+            if (mixed) wss[grp[q]]->ensure(rs * a.d * a.op->froms.size(), rs * a.d * a.op->tos.size(), (float2*)nullptr);
+            else wss[grp[q]]->ensure(rs * a.d * a.op->froms.size(), rs * a.d * a.op->tos.size());
+            IramTask& t = tasks[q];
+            t.op = mixed ? static_cast<Operator*>(op32.back().get()) : static_cast<Operator*>(op64.back().get());
+            t.ncv = ncv;
+            t.which = a.which;
+            t.tol = mixed ? a.tol32 : a.tol;
+            t.max_restarts = a.max_iterations;
+            t.guess = a.guess;
+            t.result = a.result;
+        }
+        BatchApply apply;
+        apply.fn = [&](std::vector<int> const& who, std::vector<const double2*> const& xs,
+                       std::vector<double2*> const& ys, hipStream_t st) {
+            if (!mixed) {
+                GemmPlan A, Cp;
+                A.M = A.N = Cp.M = Cp.N = 1;
+                std::vector<MixItem<double2>> mix;
+                for (size_t q = 0; q < who.size(); ++q) {
+                    OptimizeArgs const& a = *args[grp[who[q]]];
+                    Workspace& w = *wss[grp[who[q]]];
+                    const int64_t rs = (int64_t)a.br * a.bl;
+                    MpoSite const& op = *a.op;
+                    for (size_t fi = 0; fi < op.froms.size(); ++fi)
+                        for (int aa = 0; aa < op.d; ++aa) {
+                            A.begin_batch(w.P.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl);
+                            A.add_seg(xs[q] + aa * rs, a.br, a.L + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl, a.bl);
+                        }
+                    mix.push_back(MixItem<double2>{&op, w.P.ptr, w.Q.ptr, rs});
+                    Cp.begin_batch(ys[q], a.br, a.br, a.bl * op.d);
+                    for (size_t ti = 0; ti < op.tos.size(); ++ti)
+                        Cp.add_seg(a.R + (int64_t)op.tos[ti] * a.br * a.br, a.br, w.Q.ptr + (int64_t)ti * op.d * rs, a.br,
+                                   a.br);
+                }
+                zgemm_run(A, st);
+                mix_blocks_batch(mix, st);
+                zgemm_run(Cp, st);
+            } else {
+                CGemmPlan A, Cp;
+                A.M = A.N = Cp.M = Cp.N = 1;
+                std::vector<MixItem<float2>> mix;
+                std::vector<const double2*> cin;
+                std::vector<float2*> cout32;
+                std::vector<const float2*> rin32;
+                std::vector<double2*> rout;
+                std::vector<int64_t> ns;
+                for (size_t q = 0; q < who.size(); ++q) {
+                    OptimizeArgs const& a = *args[grp[who[q]]];
+                    Workspace& w = *wss[grp[who[q]]];
+                    Heff32Operator& o32 = *op32[who[q]];
+                    const int64_t rs = (int64_t)a.br * a.bl;
+                    MpoSite const& op = *a.op;
+                    cin.push_back(xs[q]);
+                    cout32.push_back(o32.x32.ptr);
+                    rin32.push_back(o32.y32.ptr);
+                    rout.push_back(ys[q]);
+                    ns.push_back(rs * op.d);
+                    for (size_t fi = 0; fi < op.froms.size(); ++fi)
+                        for (int aa = 0; aa < op.d; ++aa) {
+                            A.begin_batch(w.P32.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl);
+                            A.add_seg(o32.x32.ptr + aa * rs, a.br, a.L32 + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl,
+                                      a.bl);
+                        }
+                    mix.push_back(MixItem<float2>{&op, w.P32.ptr, w.Q32.ptr, rs});
+                    Cp.begin_batch(o32.y32.ptr, a.br, a.br, a.bl * op.d);
+                    for (size_t ti = 0; ti < op.tos.size(); ++ti)
+                        Cp.add_seg(a.R32 + (int64_t)op.tos[ti] * a.br * a.br, a.br,
+                                   w.Q32.ptr + (int64_t)ti * op.d * rs, a.br, a.br);
+                }
+                to_c64_batch(cin, cout32, ns, st);
+                cgemm_run(A, st);
+                mix_blocks_batch(mix, st);
+                cgemm_run(Cp, st);
+                to_c128_batch(rin32, rout, ns, st);
+            }
+        };
+        iram_batch(tasks, apply, s);
+        for (size_t q = 0; q < grp.size(); ++q) {
+            const size_t c = grp[q];
+            OptimizeArgs const& a = *args[c];
+            const int64_t nfull = (int64_t)a.d * a.bl * a.br;
+            IramResult r = tasks[q].out;
+            OptimizeOut& o = outs[c];
+            if (r.info == -14) {  // the retries of optimize_site, one chain at a time
+                o = optimize_site(a, *wss[c], s);
+                if (mixed) o.matvecs32 += r.matvecs;
+                else o.matvecs += r.matvecs;
+                continue;
+            }
+            o.strategy = (a.bl * a.br < a.cl * a.cr) ? 2 : 3;
+            if (mixed) o.matvecs32 += r.matvecs;
+            else o.matvecs += r.matvecs;
+            if (mixed && r.info == 0) {
+                if (a.polish) {
+                    ZBuf start((size_t)nfull);
+                    d2d(start.ptr, a.result, (size_t)nfull * sizeof(double2));
+                    IramResult r2 = iram(*op64[q], a.ncv, a.which, a.tol, a.max_iterations, start.ptr, a.result, s);
+                    o.matvecs += r2.matvecs;
+                    if (r2.info == 0) r = r2;
+                } else {
+                    ZBuf y((size_t)nfull);
+                    op64[q]->apply(a.result, y.ptr, s);
+                    ++o.matvecs;
+                    zd num;
+                    zdotc_multi(a.result, nfull, 1, y.ptr, nfull, &num, s);
+                    const double nn = dznrm2(a.result, nfull, s);
+                    r.eigenvalue = num / (nn * nn);
+                }
+            }
+            o.info = r.info;
+            o.iterations = r.restarts;
+            o.eigenvalue = r.eigenvalue;
+            if (r.info != 0) dzero(a.result, (size_t)nfull * sizeof(double2));
+            o.normal = dznrm2(a.result, nfull, s);
+        }
+    }
+}
+
+}  // namespace nc

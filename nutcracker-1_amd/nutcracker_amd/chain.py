@@ -36,6 +36,10 @@ _CHAIN_SIGS = {
     "nc_chain_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "nc_chain_set_mixed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
     "nc_chain_add_projector": (C.c_int, [C.c_void_p]),
+    "nc_chain_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_uint32]),
+    "nc_chain_batch_destroy": (None, [C.c_void_p]),
+    "nc_chain_batch_sweep": (C.c_int, [C.c_void_p]),
+    "nc_chain_batch_sweep_until_converged": (C.c_int, [C.c_void_p]),
     "nc_chain_set_growth": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "nc_chain_increase_bandwidth": (C.c_int, [C.c_void_p, C.c_uint32]),
     "nc_chain_optimize_chain": (C.c_int, [C.c_void_p]),
@@ -228,9 +232,50 @@ class Chain:
         self._refresh_bonds()
 
     def stats(self) -> dict:
-        counts = (C.c_int64 * 6)()
+        counts = (C.c_int64 * 7)()
         times = (C.c_double * 4)()
         self._check(_lib().nc_chain_stats(self._h, counts, times))
         return {"matvecs": counts[0], "site_solves": counts[1], "moves": counts[2], "sweeps": counts[3],
-                "failures": counts[4], "matvecs32": counts[5], "t_optimize_ms": times[0], "t_move_ms": times[1], "t_env_ms": times[2],
+                "failures": counts[4], "matvecs32": counts[5], "fp64_refinements": counts[6], "t_optimize_ms": times[0], "t_move_ms": times[1], "t_env_ms": times[2],
                 "t_reset_ms": times[3]}
+
+
+class ChainBatch:
+    """Independent chains of the same length sweeping in lockstep, their site
+    solves batched into grouped launches (nc_chain_batch_* in the C-ABI).  The
+    chains remain usable on their own; keep them alive while the batch is."""
+
+    def __init__(self, chains: Sequence[Chain]):
+        self.chains = list(chains)
+        arr = (C.c_void_p * len(self.chains))(*[c._h for c in self.chains])
+        self._h = _lib().nc_chain_batch_create(arr, len(self.chains))
+        if not self._h:
+            raise RuntimeError(core.last_error())
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = core.last_error()
+            if rc > 0 or rc == -14:
+                raise OptimizerFailure("optimizer failure %d: %s" % (rc, msg))
+            raise RuntimeError(msg or ("error %d" % rc))
+
+    def perform_optimization_sweep(self):
+        self._check(_lib().nc_chain_batch_sweep(self._h))
+
+    def sweep_until_converged(self):
+        self._check(_lib().nc_chain_batch_sweep_until_converged(self._h))
+
+    @property
+    def energies(self) -> List[float]:
+        return [c.energy for c in self.chains]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().nc_chain_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
