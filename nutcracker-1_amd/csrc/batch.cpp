@@ -4,15 +4,23 @@
 // The chains keep their own sites, environments and workspaces; at every step
 // of the sweep (chain.cpp:147-163) all of them sit on the same site, and their
 // site solves run together (optimize_sites_batched: one grouped launch per
-// GEMM step for all chains, one orthogonalisation launch per Arnoldi kernel).
-// Moves stay per chain.  A chain whose energy has converged
+// GEMM step for all chains, one orthogonalisation launch per Arnoldi kernel),
+// and so do their moves (one batched Newton-Schulz polar split, grouped GEMMs
+// for the neighbour and the environment update).  A chain whose energy has converged
 // (sweepUntilConverged, base_chain.cpp:112-123) leaves the lockstep; the others
 // go on, so every chain performs exactly the sweeps it would perform alone.
 #include "nc_chain.hpp"
 
+#include <chrono>
+
 namespace nc {
 
 namespace {
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
 bool outside_tol(double a, double b, double t) {
     // utilities.hpp:418-420
     return ((std::abs(a) + std::abs(b)) / 2 > t) && (std::abs(a - b) / (std::abs(a) + std::abs(b) + t) > t);
@@ -48,6 +56,178 @@ void ChainBatch::optimize_sites(std::vector<Chain*> const& active) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Moves of all chains together: one batched polar split, grouped GEMMs for the
+// neighbour and for the environment update (states.cpp:94-164 via
+// norm_denorm_going_*, boundaries.cpp:145-189 via contract_sos_*).
+// ---------------------------------------------------------------------------
+void ChainBatch::env_left_batch(std::vector<Chain*> const& cs, hipStream_t s) {
+    // L_{i+1} = contract_sos_left(L_i, site i), i = current - 1 (after the move)
+    GemmPlan A, Cp;
+    A.M = A.N = Cp.M = Cp.N = 1;
+    Cp.opA = OpN; Cp.opB = OpC;
+    std::vector<MixItem<double2>> mix;
+    for (Chain* c : cs) {
+        const int i = c->current - 1;
+        MpoSite const& op = *c->ops[i];
+        const int bl = c->bonds[i], br = c->bonds[i + 1], d = op.d;
+        const int64_t rs = (int64_t)br * bl;
+        c->ws.ensure(rs * d * op.froms.size(), rs * d * op.tos.size());
+        c->left_envs[i + 1].resize((size_t)br * br * op.cr);
+        const double2* psi = c->sites[i].ptr;
+        for (size_t fi = 0; fi < op.froms.size(); ++fi)
+            for (int a = 0; a < d; ++a) {
+                A.begin_batch(c->ws.P.ptr + ((int64_t)fi * d + a) * rs, br, br, bl);
+                A.add_seg(psi + a * rs, br, c->left_envs[i].ptr + (int64_t)op.froms[fi] * bl * bl, bl, bl);
+            }
+        mix.push_back(MixItem<double2>{&op, c->ws.P.ptr, c->ws.Q.ptr, rs, true});
+        if ((int)op.tos.size() < op.cr) dzero(c->left_envs[i + 1].ptr, (size_t)br * br * op.cr * sizeof(double2));
+        for (size_t ti = 0; ti < op.tos.size(); ++ti) {
+            Cp.begin_batch(c->left_envs[i + 1].ptr + (int64_t)op.tos[ti] * br * br, br, br, br);
+            Cp.add_seg(c->ws.Q.ptr + (int64_t)ti * d * rs, br, psi, br, bl * d);
+        }
+    }
+    zgemm_run(A, s);
+    mix_blocks_batch(mix, s);
+    zgemm_run(Cp, s);
+}
+
+void ChainBatch::env_right_batch(std::vector<Chain*> const& cs, hipStream_t s) {
+    // R_i = contract_sos_right(R_{i+1}, site i), i = current + 1 (after the move)
+    GemmPlan A, Cp;
+    A.M = A.N = Cp.M = Cp.N = 1;
+    Cp.opA = OpC; Cp.opB = OpN;
+    std::vector<MixItem<double2>> mix;
+    for (Chain* c : cs) {
+        const int i = c->current + 1;
+        MpoSite const& op = *c->ops[i];
+        const int bl = c->bonds[i], br = c->bonds[i + 1], d = op.d;
+        const int64_t rs = (int64_t)br * bl;
+        c->ws.ensure(rs * d * op.tos.size(), rs * d * op.froms.size());
+        c->right_envs[i].resize((size_t)bl * bl * op.cl);
+        const double2* psi = c->sites[i].ptr;
+        for (size_t ti = 0; ti < op.tos.size(); ++ti)
+            for (int a = 0; a < d; ++a) {
+                A.begin_batch(c->ws.P.ptr + ((int64_t)ti * d + a) * rs, br, br, bl);
+                A.add_seg(c->right_envs[i + 1].ptr + (int64_t)op.tos[ti] * br * br, br, psi + a * rs, br, br);
+            }
+        mix.push_back(MixItem<double2>{&op, c->ws.P.ptr, c->ws.Q.ptr, rs, false});
+        if ((int)op.froms.size() < op.cl) dzero(c->right_envs[i].ptr, (size_t)bl * bl * op.cl * sizeof(double2));
+        for (size_t fi = 0; fi < op.froms.size(); ++fi) {
+            Cp.begin_batch(c->right_envs[i].ptr + (int64_t)op.froms[fi] * bl * bl, bl, bl, bl);
+            for (int b = 0; b < d; ++b)
+                Cp.add_seg(psi + b * rs, br, c->ws.Q.ptr + ((int64_t)fi * d + b) * rs, br, br);
+        }
+    }
+    zgemm_run(A, s);
+    mix_blocks_batch(mix, s);
+    zgemm_run(Cp, s);
+}
+
+void ChainBatch::move_all(std::vector<Chain*> const& cs, int dir) {
+    hipStream_t s = Context::get().stream;
+    bool plain = true;  // complex64 environments and lower levels take the per-chain path
+    for (Chain* c : cs)
+        if ((c->opts.mixed && c->opts.env32) || !c->projectors.empty()) plain = false;
+    if (!plain) {
+        for (Chain* c : cs) {
+            if (dir > 0) c->move_right();
+            else c->move_left();
+        }
+        return;
+    }
+    for (Chain* c : cs) {
+        if (dir > 0 && c->current + 1 >= c->N) throw Error("move_right past the last site", -63);
+        if (dir < 0 && c->current == 0) throw Error("move_left past the first site", -64);
+    }
+    std::vector<PolarJob> jobs(cs.size());
+    std::vector<ZBuf> Up(cs.size()), Pc(cs.size()), W1(cs.size()), W2(cs.size()), moved(cs.size());
+    for (size_t q = 0; q < cs.size(); ++q) {
+        Chain* c = cs[q];
+        const int i = c->current;
+        if (dir > 0) {
+            // the site (bm, bl, dl) is the wide bm x (bl dl) matrix to normalise
+            const int bl = c->bonds[i], bm = c->bonds[i + 1];
+            const int dl = c->ops[i]->d;
+            if ((int64_t)bl * dl < bm) throw Error("Not enough degrees of freedom to normalize.", -54);
+            Up[q].resize(c->site_size(i));
+            Pc[q].resize((size_t)bm * bm);
+            jobs[q] = PolarJob{bm, (int64_t)bl * dl, c->sites[i].ptr, Up[q].ptr, Pc[q].ptr, 0};
+        } else {
+            // the site (br, bm, dr) as bm x (br dr)
+            const int bm = c->bonds[i], br = c->bonds[i + 1];
+            const int dr = c->ops[i]->d;
+            if ((int64_t)br * dr < bm) throw Error("Not enough degrees of freedom to normalize.", -54);
+            W1[q].resize(c->site_size(i));
+            W2[q].resize(c->site_size(i));
+            Pc[q].resize((size_t)bm * bm);
+            permute_102(c->sites[i].ptr, W1[q].ptr, br, bm, dr, s);
+            jobs[q] = PolarJob{bm, (int64_t)br * dr, W1[q].ptr, W2[q].ptr, Pc[q].ptr, 0};
+        }
+    }
+    polar_split_batch(jobs, s);
+    for (size_t q = 0; q < cs.size(); ++q)
+        if (jobs[q].info != 0) throw Error("NormalizationError", jobs[q].info);
+    // the neighbour absorbs conj(P)
+    GemmPlan g;
+    g.M = g.N = 1;
+    std::vector<ZBuf> T(cs.size());
+    for (size_t q = 0; q < cs.size(); ++q) {
+        Chain* c = cs[q];
+        const int i = c->current;
+        if (dir > 0) {
+            const int bm = c->bonds[i + 1], br = c->bonds[i + 2], dr = c->ops[i + 1]->d;
+            T[q].resize(c->site_size(i + 1));
+            W2[q].resize(c->site_size(i + 1));
+            permute_102(c->sites[i + 1].ptr, T[q].ptr, br, bm, dr, s);  // (br, bm, dr) -> (bm, br, dr)
+            g.begin_batch(W2[q].ptr, bm, bm, br * dr);
+            g.add_seg(Pc[q].ptr, bm, T[q].ptr, bm, bm);
+        } else {
+            const int bll = c->bonds[i - 1], bm = c->bonds[i], dl = c->ops[i - 1]->d;
+            moved[q].resize(c->site_size(i - 1));
+            g.begin_batch(moved[q].ptr, bm, bm, bll * dl);
+            g.add_seg(Pc[q].ptr, bm, c->sites[i - 1].ptr, bm, bm);
+        }
+    }
+    zgemm_run(g, s);
+    std::vector<double2*> mid;
+    std::vector<const double2*> midc;
+    std::vector<int64_t> mn;
+    for (size_t q = 0; q < cs.size(); ++q) {
+        Chain* c = cs[q];
+        const int i = c->current;
+        if (dir > 0) {
+            const int bm = c->bonds[i + 1], br = c->bonds[i + 2], dr = c->ops[i + 1]->d;
+            moved[q].resize(c->site_size(i + 1));
+            permute_102(W2[q].ptr, moved[q].ptr, bm, br, dr, s);  // back to (br, bm, dr)
+            std::swap(c->sites[i], Up[q]);                         // left-normalised site i
+            mn.push_back((int64_t)c->site_size(i + 1));
+        } else {
+            const int bm = c->bonds[i], br = c->bonds[i + 1], dr = c->ops[i]->d;
+            permute_102(W2[q].ptr, c->sites[i].ptr, bm, br, dr, s);  // right-normalised site i
+            mn.push_back((int64_t)c->site_size(i - 1));
+        }
+        mid.push_back(moved[q].ptr);
+        midc.push_back(moved[q].ptr);
+    }
+    // states.cpp:119-122 / 155-158: the new middle site is renormalised
+    std::vector<double> nn = sumsq_batch(midc, mn, s);
+    std::vector<double> f(nn.size());
+    for (size_t q = 0; q < nn.size(); ++q) f[q] = 1.0 / std::sqrt(nn[q]);
+    scale_batch(mid, mn, f, s);
+    for (size_t q = 0; q < cs.size(); ++q) {
+        Chain* c = cs[q];
+        if (dir > 0) std::swap(c->sites[c->current + 1], moved[q]);
+        else std::swap(c->sites[c->current - 1], moved[q]);
+        c->current += dir;
+        c->optimized = false;
+        ++c->stats.moves;
+    }
+    if (dir > 0) env_left_batch(cs, s);
+    else env_right_batch(cs, s);
+    NC_HIP(hipStreamSynchronize(s));
+}
+
 void ChainBatch::sweep(std::vector<Chain*> const& active) {
     // performOptimizationSweep (chain.cpp:147-163) for every active chain at once
     if (active.empty()) return;
@@ -60,11 +240,12 @@ void ChainBatch::sweep(std::vector<Chain*> const& active) {
         if (!c->optimized) todo.push_back(c);
     optimize_sites(todo);
     auto step = [&](int dir) {
-        for (Chain* c : active) {
-            if (dir > 0) c->move_right();
-            else c->move_left();
-        }
+        const double t0 = now_ms();
+        move_all(active, dir);
+        t_moves_ms += now_ms() - t0;
+        const double t1 = now_ms();
         optimize_sites(active);
+        t_solves_ms += now_ms() - t1;
     };
     while (active[0]->current + 1 < N) step(+1);
     while (active[0]->current > 0) step(-1);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void mix_batch_kernel(const MixBatchDesc<CT>* 
 }
 
 template <typename CT>
-static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&), hipStream_t s) {
+static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&, int), hipStream_t s) {
     if (items.empty()) return;
     const int d = items[0].op->d;
     std::vector<MixBatchDesc<CT>> ds;
@@ -117,14 +117,15 @@ static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(Mp
     int maxdst = 0;
     for (auto const& it : items) {
         if (it.op->d != d) throw Error("mix_blocks_batch: chains with different physical dimensions", -4);
+        const int dir = it.src_is_from ? 0 : 1;
         MixBatchDesc<CT> m;
         m.P = it.P;
         m.Q = it.Q;
-        m.dst_ptr = it.op->mix_ptr[0].ptr;
-        m.blk_src = it.op->mix_src[0].ptr;
-        m.W = W(*it.op);
+        m.dst_ptr = it.op->mix_ptr[dir].ptr;
+        m.blk_src = it.op->mix_src[dir].ptr;
+        m.W = W(*it.op, dir);
         m.rs = it.rs;
-        m.ndst = (int)it.op->tos.size();
+        m.ndst = (int)(it.src_is_from ? it.op->tos.size() : it.op->froms.size());
         m.d = d;
         ds.push_back(m);
         maxrs = std::max(maxrs, it.rs);
@@ -144,8 +145,8 @@ static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(Mp
     release_descriptors(slot, s);
 }
 
-static const double2* mixw64(MpoSite const& op) { return op.mix_W[0].ptr; }
-static const float2* mixw32(MpoSite const& op) { return op.mix_W32[0].ptr; }
+static const double2* mixw64(MpoSite const& op, int dir) { return op.mix_W[dir].ptr; }
+static const float2* mixw32(MpoSite const& op, int dir) { return op.mix_W32[dir].ptr; }
 void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s) { mix_batch_t(items, mixw64, s); }
 void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s) { mix_batch_t(items, mixw32, s); }
 

@@ -118,7 +118,11 @@ struct ChainBatch {
     std::vector<Chain*> chains;
     int64_t lockstep_rounds = 0;
     explicit ChainBatch(std::vector<Chain*> cs);
+    double t_moves_ms = 0, t_solves_ms = 0;
     void optimize_sites(std::vector<Chain*> const& active);
+    void move_all(std::vector<Chain*> const& cs, int dir);
+    void env_left_batch(std::vector<Chain*> const& cs, hipStream_t s);
+    void env_right_batch(std::vector<Chain*> const& cs, hipStream_t s);
     void sweep(std::vector<Chain*> const& active);
     void sweep_all();
     void sweep_until_converged();

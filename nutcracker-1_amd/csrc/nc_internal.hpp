@@ -202,6 +202,7 @@ struct MixItem {
     const CT* P;
     CT* Q;
     int64_t rs;
+    bool src_is_from = true;  // false: the right-environment direction (src = to)
 };
 void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s);
 void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s);
@@ -432,6 +433,21 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
 // Returns the SVD info code (0 on success).  ``Pconj`` receives conj(P) (m x m).
 // ----------------------------------------------------------------------------
 int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s);
+// Several polar splits (one per chain) advanced together: every Newton-Schulz
+// step of all of them is two grouped GEMM launches and one host read.
+struct PolarJob {
+    int m = 0;
+    int64_t n = 0;
+    const double2* A = nullptr;
+    double2* Up = nullptr;
+    double2* Pconj = nullptr;
+    int info = 0;
+};
+void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s);
+// sum |x|^2 of several vectors, and x *= f (real) for several vectors
+std::vector<double> sumsq_batch(std::vector<const double2*> const& x, std::vector<int64_t> const& n, hipStream_t s);
+void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, std::vector<double> const& f,
+                 hipStream_t s);
 // the same through rocSOLVER's SVD (used when the iteration cannot converge)
 int polar_split_svd(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s);
 struct PolarStats {

@@ -72,6 +72,31 @@ __global__ void add_hash_noise_kernel(double2* X, int64_t total, double scale) {
 namespace {
 PolarStats g_polar_stats;
 
+// alpha_cap solves (a/2)(3 - a^2) = 0.1 on [1, sqrt(3)]
+double ns_alpha_cap() {
+    static const double cap = [] {
+        double lo = 1.0, hi = std::sqrt(3.0);
+        for (int k = 0; k < 80; ++k) {
+            const double mid = 0.5 * (lo + hi);
+            if (0.5 * mid * (3.0 - mid * mid) > 0.1) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }();
+    return cap;
+}
+
+// one Newton-Schulz schedule step: the lower bound l of the singular values,
+// raised by the measured error, gives alpha; returns alpha and updates l
+double ns_alpha(double& l, double err) {
+    if (err < 1.0) l = std::max(l, std::sqrt(1.0 - err));
+    double alpha = 1.0;
+    if (l < 1.0 - 1e-3) alpha = std::min(ns_alpha_cap(), std::sqrt(3.0 / (1.0 + l + l * l)));
+    const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
+    l = std::min({1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l), p1});
+    return alpha;
+}
+
 // Scaled Newton-Schulz iteration for the unitary polar factor of a wide
 // matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X  for a lower
 // bound l of the smallest singular value (X is first divided by its Frobenius
@@ -112,16 +137,6 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
     zscal(X0.ptr, zd(1.0 / nrm2, 0), total, s);
     double l = 1e-16;
     const int max_iters = 64;
-    // alpha_cap solves (a/2)(3 - a^2) = 0.1 on [1, sqrt(3)]
-    static const double alpha_cap = [] {
-        double lo = 1.0, hi = std::sqrt(3.0);
-        for (int k = 0; k < 80; ++k) {
-            const double mid = 0.5 * (lo + hi);
-            if (0.5 * mid * (3.0 - mid * mid) > 0.1) lo = mid;
-            else hi = mid;
-        }
-        return lo;
-    }();
     bool last = false;
     for (int it = 0; it < max_iters; ++it) {
         GemmPlan g;  // G = X X^H
@@ -140,9 +155,7 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
         // every sigma lies in [l, 1], and sigma^2 >= 1 - ||X X^H - I||_F: the
         // measured error raises the bound, so a nearly unitary iterate takes
         // plain (alpha = 1) steps, which keep sigma = 1 fixed
-        if (err < 1.0) l = std::max(l, std::sqrt(1.0 - err));
-        double alpha = 1.0;
-        if (l < 1.0 - 1e-3) alpha = std::min(alpha_cap, std::sqrt(3.0 / (1.0 + l + l * l)));
+        const double alpha = ns_alpha(l, err);
         // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
         zscal(G.ptr, zd(-0.5 * alpha * alpha * alpha, 0), (int64_t)m * m, s);
         add_identity(G.ptr, m, zd(0.5 * alpha * (3.0 - alpha * alpha), 0), s);
@@ -152,8 +165,6 @@ bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStr
         u.add_seg(G.ptr, m, X0.ptr, m, m);
         zgemm_run(u, s);
         std::swap(X0.ptr, X1.ptr);
-        const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
-        l = std::min({1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l), p1});
         // cubic convergence: once every |sigma^2 - 1| < 1e-7 one more step reaches round-off
         if (err < 1e-7) last = true;
     }
@@ -257,6 +268,254 @@ int norm_denorm_left(int bl, int bm, int br, int dl, int dr, const double2* to_d
     p.add_seg(Pc.ptr, bm, to_denorm, bm, bm);
     zgemm_run(p, s);
     return info;
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms for many chains: descriptor lists staged without a host
+// synchronisation, one launch for all items.
+// ---------------------------------------------------------------------------
+namespace {
+struct BItem {
+    double2* x;
+    int64_t n;   // elements (or m for the m x m kernels)
+    double a, b;
+};
+
+__global__ void bsumsq_kernel(const BItem* __restrict__ it, double* __restrict__ partial) {
+    const BItem d = it[blockIdx.y];
+    double acc = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = d.x[e];
+        acc += v.x * v.x + v.y * v.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void bscale_kernel(const BItem* __restrict__ it) {
+    const BItem d = it[blockIdx.y];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = d.x[e];
+        d.x[e] = make_double2(v.x * d.a, v.y * d.a);
+    }
+}
+
+// X (n x n) <- a X + b I
+__global__ void bidentity_kernel(const BItem* __restrict__ it) {
+    const BItem d = it[blockIdx.y];
+    const int64_t total = d.n * d.n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = d.x[e];
+        v.x *= d.a;
+        v.y *= d.a;
+        if (e % d.n == e / d.n) v.x += d.b;
+        d.x[e] = v;
+    }
+}
+
+__global__ void bnoise_kernel(const BItem* __restrict__ it) {
+    const BItem d = it[blockIdx.y];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = (uint64_t)e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double ur = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        z = z * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double ui = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        double2 v = d.x[e];
+        d.x[e] = make_double2(v.x + d.a * ur, v.y + d.a * ui);
+    }
+}
+
+__global__ void bherm_kernel(const BItem* __restrict__ it) {
+    const BItem d = it[blockIdx.y];
+    const int64_t m = d.n, total = m * m;
+    double2* P = d.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % m, j = e / m;
+        if (i > j) continue;
+        const double2 a = P[i + j * m], b = P[j + i * m];
+        P[i + j * m] = make_double2(0.5 * (a.x + b.x), 0.5 * (b.y - a.y));
+        P[j + i * m] = make_double2(0.5 * (b.x + a.x), 0.5 * (a.y - b.y));
+    }
+}
+
+template <typename K>
+void launch_items(K kernel, std::vector<BItem> const& items, int64_t max_elems, hipStream_t s) {
+    if (items.empty() || max_elems <= 0) return;
+    int slot;
+    auto* d = (const BItem*)stage_descriptors(items.data(), items.size() * sizeof(BItem), s, &slot);
+    const unsigned g = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)items.size()), (max_elems + 255) / 256));
+    hipLaunchKernelGGL(kernel, dim3(g, (unsigned)items.size()), dim3(256), 0, s, d);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+}
+}  // namespace
+
+std::vector<double> sumsq_batch(std::vector<const double2*> const& x, std::vector<int64_t> const& n, hipStream_t s) {
+    const size_t P = x.size();
+    std::vector<double> out(P, 0.0);
+    if (P == 0) return out;
+    std::vector<BItem> items(P);
+    int64_t maxn = 1;
+    for (size_t p = 0; p < P; ++p) {
+        items[p] = BItem{const_cast<double2*>(x[p]), n[p], 0, 0};
+        maxn = std::max(maxn, n[p]);
+    }
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(1, 2048 / (int64_t)P),
+                                                                         (maxn + 255) / 256));
+    DevBuf<double> partial((size_t)P * g);
+    int slot;
+    auto* d = (const BItem*)stage_descriptors(items.data(), items.size() * sizeof(BItem), s, &slot);
+    hipLaunchKernelGGL(bsumsq_kernel, dim3(g, (unsigned)P), dim3(256), 0, s, d, partial.ptr);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+    std::vector<double> h((size_t)P * g);
+    d2h(h.data(), partial.ptr, h.size() * sizeof(double));
+    for (size_t p = 0; p < P; ++p)
+        for (unsigned b = 0; b < g; ++b) out[p] += h[p * g + b];
+    return out;
+}
+
+void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, std::vector<double> const& f,
+                 hipStream_t s) {
+    std::vector<BItem> items;
+    int64_t maxn = 0;
+    for (size_t p = 0; p < x.size(); ++p) {
+        items.push_back(BItem{x[p], n[p], f[p], 0});
+        maxn = std::max(maxn, n[p]);
+    }
+    launch_items(bscale_kernel, items, maxn, s);
+}
+
+void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
+    // the algorithm of polar_newton_schulz / polar_split, every step for all jobs
+    const size_t J = jobs.size();
+    struct St {
+        ZBuf X0, X1, G;
+        double l = 1e-16;
+        bool last = false, done = false, ok = false;
+        int iters = 0;
+    };
+    std::vector<St> st(J);
+    std::vector<const double2*> xs;
+    std::vector<int64_t> ns;
+    for (auto const& j : jobs) {
+        if (j.m <= 0 || j.n < j.m) throw Error("polar_split: need 0 < m <= n", -51);
+        xs.push_back(j.A);
+        ns.push_back((int64_t)j.m * j.n);
+    }
+    std::vector<double> a2 = sumsq_batch(xs, ns, s);
+    std::vector<BItem> noise;
+    int64_t maxn = 0;
+    for (size_t q = 0; q < J; ++q) {
+        const double nrm = std::sqrt(a2[q]);
+        jobs[q].info = 0;
+        if (!std::isfinite(nrm)) { jobs[q].info = 1000; st[q].done = true; continue; }
+        ++g_polar_stats.calls;
+        if (!(nrm > 0)) { st[q].done = true; continue; }  // the SVD below
+        const int64_t total = ns[q];
+        st[q].X0.resize(total);
+        st[q].X1.resize(total);
+        st[q].G.resize((size_t)jobs[q].m * jobs[q].m);
+        d2d(st[q].X0.ptr, jobs[q].A, (size_t)total * sizeof(double2));
+        noise.push_back(BItem{st[q].X0.ptr, total, 1e-14 * nrm / std::sqrt((double)total) * std::sqrt(1.5), 0});
+        maxn = std::max(maxn, total);
+    }
+    launch_items(bnoise_kernel, noise, maxn, s);
+    {
+        std::vector<const double2*> x0;
+        std::vector<int64_t> n0;
+        std::vector<size_t> ids;
+        for (size_t q = 0; q < J; ++q)
+            if (!st[q].done) { x0.push_back(st[q].X0.ptr); n0.push_back(ns[q]); ids.push_back(q); }
+        std::vector<double> s2 = sumsq_batch(x0, n0, s);
+        std::vector<double2*> xv;
+        std::vector<double> f;
+        for (size_t k = 0; k < ids.size(); ++k) { xv.push_back(st[ids[k]].X0.ptr); f.push_back(1.0 / std::sqrt(s2[k])); }
+        scale_batch(xv, n0, f, s);
+    }
+    for (int it = 0; it < 64; ++it) {
+        std::vector<size_t> act;
+        for (size_t q = 0; q < J; ++q)
+            if (!st[q].done) act.push_back(q);
+        if (act.empty()) break;
+        // G = X X^H, E = G - I, err = ||E||_F
+        GemmPlan g;
+        g.M = g.N = 1; g.opA = OpN; g.opB = OpC;
+        std::vector<BItem> idn;
+        std::vector<const double2*> ev;
+        std::vector<int64_t> en;
+        int64_t maxm = 0;
+        for (size_t q : act) {
+            const int m = jobs[q].m;
+            g.begin_batch(st[q].G.ptr, m, m, m);
+            g.add_seg(st[q].X0.ptr, m, st[q].X0.ptr, m, (int)jobs[q].n);
+            idn.push_back(BItem{st[q].G.ptr, m, 1.0, -1.0});
+            ev.push_back(st[q].G.ptr);
+            en.push_back((int64_t)m * m);
+            maxm = std::max<int64_t>(maxm, (int64_t)m * m);
+        }
+        zgemm_run(g, s);
+        launch_items(bidentity_kernel, idn, maxm, s);
+        std::vector<double> e2 = sumsq_batch(ev, en, s);
+        GemmPlan u;
+        u.M = u.N = 1; u.opA = OpN; u.opB = OpN;
+        std::vector<BItem> hset;
+        std::vector<size_t> upd;
+        for (size_t k = 0; k < act.size(); ++k) {
+            const size_t q = act[k];
+            const int m = jobs[q].m;
+            const double err = std::sqrt(e2[k]);
+            if (!std::isfinite(err)) { st[q].done = true; continue; }  // the SVD below
+            if (st[q].last || err < 1e-15 * m) {
+                d2d(jobs[q].Up, st[q].X0.ptr, (size_t)ns[q] * sizeof(double2));
+                st[q].done = st[q].ok = true;
+                st[q].iters = it;
+                continue;
+            }
+            const double alpha = ns_alpha(st[q].l, err);
+            // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
+            hset.push_back(BItem{st[q].G.ptr, m, -0.5 * alpha * alpha * alpha, 0.5 * alpha * (3.0 - alpha * alpha)});
+            u.begin_batch(st[q].X1.ptr, m, m, (int)jobs[q].n);
+            u.add_seg(st[q].G.ptr, m, st[q].X0.ptr, m, m);
+            upd.push_back(q);
+            if (err < 1e-7) st[q].last = true;
+        }
+        if (upd.empty()) continue;
+        launch_items(bidentity_kernel, hset, maxm, s);
+        zgemm_run(u, s);
+        for (size_t q : upd) std::swap(st[q].X0, st[q].X1);
+    }
+    // P = A U_p^H (Hermitian part) for the converged ones; the SVD for the rest
+    GemmPlan pg;
+    pg.M = pg.N = 1; pg.opA = OpN; pg.opB = OpC;
+    std::vector<BItem> herm;
+    int64_t maxm = 0;
+    for (size_t q = 0; q < J; ++q) {
+        if (jobs[q].info == 1000) continue;
+        if (!st[q].ok) {
+            ++g_polar_stats.svd_fallbacks;
+            jobs[q].info = polar_split_svd(jobs[q].m, jobs[q].n, jobs[q].A, jobs[q].Up, jobs[q].Pconj, s);
+            continue;
+        }
+        g_polar_stats.iterations += st[q].iters;
+        const int m = jobs[q].m;
+        pg.begin_batch(jobs[q].Pconj, m, m, m);
+        pg.add_seg(jobs[q].A, m, jobs[q].Up, m, (int)jobs[q].n);
+        herm.push_back(BItem{jobs[q].Pconj, m, 0, 0});
+        maxm = std::max<int64_t>(maxm, (int64_t)m * m);
+    }
+    if (!pg.batches.empty()) zgemm_run(pg, s);
+    launch_items(bherm_kernel, herm, maxm, s);
 }
 
 }  // namespace nc
