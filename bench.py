@@ -19,8 +19,11 @@ dimension chi = 1024.
   2 (N - 1) x the mean bulk site step; |dE| is the difference of the two chains'
   energies after each site solve of the window, and the state infidelity
   1 - |<psi_mixed|psi_FP64>|^2 is taken after the window.
+* Batching: 64 TFIM chains (N = 32, h and chi drawn from a seed) sweep once in
+  lockstep with grouped launches; a sample of them also sweeps one chain after
+  another, for the batching speedup.
 
-    python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-batch] [--no-cpu]
 
 Prints one JSON line.  The roofline object describes the dominant kernel of the
 FP64 path (the grouped complex FP64 GEMM, two launches per matvec);
@@ -146,6 +149,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form (64 chains) timing")
+    ap.add_argument("--batch-chains", type=int, default=64)
+    ap.add_argument("--batch-sample", type=int, default=16)
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy calibration kernel")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
@@ -299,6 +305,66 @@ def main():
                    "Rayleigh quotient" % (n_steps, start, start + args.window - 1, args.ncv),
             "fp64": {"step_s": round(s64, 4), "reset_s": round(w64["reset_s"], 2), "sites": w64["sites"]},
             "mixed": {"step_s": round(smx, 4), "reset_s": round(wmx["reset_s"], 2), "sites": wmx["sites"]},
+        }
+
+    # ---- batching form: 64 TFIM chains in lockstep ------------------------
+    if not args.no_batch:
+        from nutcracker_amd.chain import ChainBatch
+        from nutcracker_amd.operators import tfim_zz_x_operator
+        brng = np.random.default_rng(args.seed + 4)
+        nb, nsites = args.batch_chains, 32
+        hs = brng.uniform(0.5, 1.5, nb)
+        chis = brng.choice([32, 64, 128, 256], nb)
+
+        def make(ids):
+            # ncv = 8: the ordered-phase chains have nearly degenerate ground doublets
+            return [chain_mod.Chain(tfim_zz_x_operator(nsites, float(hs[i])), int(chis[i]), seed=5000 + int(i),
+                                    ncv=8, mixed=True) for i in ids]
+
+        t0 = time.perf_counter()
+        cs = make(range(nb))
+        t_setup = time.perf_counter() - t0
+        batch = ChainBatch(cs)
+        t0 = time.perf_counter()
+        batch.perform_optimization_sweep()
+        t_sweep = time.perf_counter() - t0
+        solves = sum(c.stats()["site_solves"] for c in cs)
+        mvs = sum(c.stats()["matvecs"] + c.stats()["matvecs32"] for c in cs)
+        energies = [c.energy for c in cs]
+        bt = batch.times()
+        batch.close()
+        for c in cs:
+            c.close()
+        log("batch of %d chains: one lockstep sweep %.2f s (%d site solves, %d matvecs)" % (nb, t_sweep, solves, mvs))
+        # the same sweep one chain after another, on a sample of the chains
+        sample = list(range(0, nb, max(1, nb // args.batch_sample)))[:args.batch_sample]
+        cs = make(sample)
+        t0 = time.perf_counter()
+        b2 = ChainBatch(cs)
+        b2.perform_optimization_sweep()
+        t_b = time.perf_counter() - t0
+        b2.close()
+        for c in cs:
+            c.close()
+        cs = make(sample)
+        t0 = time.perf_counter()
+        for c in cs:
+            c.perform_optimization_sweep()
+        t_s = time.perf_counter() - t0
+        for c in cs:
+            c.close()
+        log("sample of %d chains: lockstep %.2f s, one after another %.2f s" % (len(sample), t_b, t_s))
+        result["batch"] = {
+            "workload": "%d TFIM chains (H = -sum ZZ - h sum X, N=%d, d=2, MPO bond 3), h ~ U[0.5, 1.5], chi drawn "
+                        "from {32, 64, 128, 256}; complex64 contractions, FP64 eigen-steps; one lockstep sweep from "
+                        "the seeded right-canonical states" % (nb, nsites),
+            "chains": nb, "chi_counts": {str(k): int((chis == k).sum()) for k in (32, 64, 128, 256)},
+            "setup_s": round(t_setup, 2), "sweep_s": round(t_sweep, 2),
+            "sweep_moves_s": round(bt["moves_ms"] / 1e3, 2), "sweep_site_solves_s": round(bt["site_solves_ms"] / 1e3, 2),
+            "chain_sweeps_per_s": round(nb / t_sweep, 3), "site_solves": int(solves), "matvecs": int(mvs),
+            "sample_chains": len(sample), "sample_lockstep_s": round(t_b, 2), "sample_one_by_one_s": round(t_s, 2),
+            "batching_speedup": round(t_s / t_b, 2),
+            "energies_after_one_sweep": [round(e, 10) for e in energies],
         }
 
     # ---- CPU baseline ----------------------------------------------------

@@ -277,6 +277,8 @@ nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n);
 void nc_chain_batch_destroy(nc_chain_batch* batch);
 int nc_chain_batch_sweep(nc_chain_batch* batch);
 int nc_chain_batch_sweep_until_converged(nc_chain_batch* batch);
+/* wall time spent in lockstep moves (times_ms[0]) and site solves (times_ms[1]) */
+int nc_chain_batch_times(nc_chain_batch* batch, double* times_ms);
 /* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,
    sites of the mixed form finished in FP64 (a complex64 solve that came out above
    the previous energy);

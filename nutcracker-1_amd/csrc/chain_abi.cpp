@@ -222,6 +222,14 @@ nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n) {
 
 void nc_chain_batch_destroy(nc_chain_batch* b) { delete b; }
 
+int nc_chain_batch_times(nc_chain_batch* b, double* times_ms) {
+    return guarded_chain("nc_chain_batch_times", [&] {
+        times_ms[0] = b->batch->t_moves_ms;
+        times_ms[1] = b->batch->t_solves_ms;
+        return 0;
+    });
+}
+
 int nc_chain_batch_sweep(nc_chain_batch* b) {
     return guarded_chain("nc_chain_batch_sweep", [&] { b->batch->sweep_all(); return 0; });
 }

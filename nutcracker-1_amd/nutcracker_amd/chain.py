@@ -40,6 +40,7 @@ _CHAIN_SIGS = {
     "nc_chain_batch_destroy": (None, [C.c_void_p]),
     "nc_chain_batch_sweep": (C.c_int, [C.c_void_p]),
     "nc_chain_batch_sweep_until_converged": (C.c_int, [C.c_void_p]),
+    "nc_chain_batch_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "nc_chain_set_growth": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "nc_chain_increase_bandwidth": (C.c_int, [C.c_void_p, C.c_uint32]),
     "nc_chain_optimize_chain": (C.c_int, [C.c_void_p]),
@@ -264,6 +265,11 @@ class ChainBatch:
 
     def sweep_until_converged(self):
         self._check(_lib().nc_chain_batch_sweep_until_converged(self._h))
+
+    def times(self) -> dict:
+        t = (C.c_double * 2)()
+        self._check(_lib().nc_chain_batch_times(self._h, t))
+        return {"moves_ms": t[0], "site_solves_ms": t[1]}
 
     @property
     def energies(self) -> List[float]:
