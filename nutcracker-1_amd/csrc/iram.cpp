@@ -340,6 +340,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     ZBuf Hd((size_t)P * (ncv + 1) * ncv);
     DevBuf<double> pnorm((size_t)P * nblk), betas((size_t)P * ncv), scale(P);
     DevBuf<KDesc> ddesc(P);
+    ZBuf coefs;  // Ritz / restart coefficients of one round, uploaded together
     std::vector<KDesc> hdesc(P);
     std::vector<double2> Hh((size_t)P * (ncv + 1) * ncv);
     std::vector<double> bh((size_t)P * ncv);
@@ -383,7 +384,14 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         }
         d2h(Hh.data(), Hd.ptr, Hh.size() * sizeof(double2));
         d2h(bh.data(), betas.ptr, bh.size() * sizeof(double));
-        // ---- per problem: Ritz values, convergence, restart
+        // ---- per problem (host): Ritz values, convergence, restart shifts.  The
+        // device work of the finishing and restarting problems is then done for
+        // all of them at once: one grouped GEMM, batched vector updates, one
+        // host read of their norms.
+        struct Fin { int p, m, iter; std::vector<zd> y; zd theta; };
+        struct Rst { int p, keep; std::vector<zd> Q; zd hk, qm; std::vector<zd> Hk; };
+        std::vector<Fin> fins;
+        std::vector<Rst> rsts;
         for (int p = 0; p < P; ++p) {
             KState& x = st[p];
             if (x.done) continue;
@@ -419,8 +427,8 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 std::vector<zd> th, Y;
                 hessenberg_eig(m, Hs, th, Y);
                 const int wi = wanted_index(th, t.which);
-                std::vector<zd> yv(Y.begin() + (size_t)wi * m, Y.begin() + (size_t)(wi + 1) * m);
-                finish_with(x, m, yv, th[wi], iter, s);
+                fins.push_back(Fin{p, m, iter, std::vector<zd>(Y.begin() + (size_t)wi * m,
+                                                                Y.begin() + (size_t)(wi + 1) * m), th[wi]});
                 continue;
             }
             x.beta = bh[(size_t)p * ncv + ncv - 1];
@@ -437,54 +445,130 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                     x.done = true;
                     continue;
                 }
-                std::vector<zd> yv(Y.begin() + (size_t)wi * ncv, Y.begin() + (size_t)(wi + 1) * ncv);
-                finish_with(x, ncv, yv, th[wi], iter, s);
+                fins.push_back(Fin{p, ncv, iter, std::vector<zd>(Y.begin() + (size_t)wi * ncv,
+                                                                  Y.begin() + (size_t)(wi + 1) * ncv), th[wi]});
                 continue;
             }
             // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies
             // the ncv - 1 unwanted Ritz values as exact shifts; a larger basis
             // keeps half of its vectors (thick restart) and shifts out the rest.
-            const int keep = ncv <= 3 ? 1 : ncv / 2;
+            Rst r;
+            r.p = p;
+            r.keep = ncv <= 3 ? 1 : ncv / 2;
+            const int keep = r.keep;
             std::vector<int> order = wanted_order(th, t.which);
-            std::vector<zd> Q((size_t)ncv * ncv, zd(0, 0));
-            for (int i = 0; i < ncv; ++i) Q[(size_t)i + (size_t)i * ncv] = 1.0;
-            for (int q = keep; q < ncv; ++q) shifted_qr_step(ncv, x.H, th[order[q]], Q);
-            const int64_t n = x.n;
-            double2* f = x.V.ptr + (size_t)n * ncv;
-            // f <- hk V q_keep + qm f ; V_k <- V Q(:, 0:keep)
-            const zd hk = h(keep, keep - 1);
-            const zd qm = Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
-            if (x.Vk.n < (size_t)n * (keep + 1)) x.Vk.resize((size_t)n * (keep + 1));
-            double2* tmp = x.Vk.ptr + (size_t)n * keep;
-            zgemv_n(x.V.ptr, n, ncv, &Q[(size_t)keep * ncv], tmp, n, s);
-            zaxpby(tmp, qm, f, hk, n, s);
-            if (keep == 1) {
-                zgemv_n(x.V.ptr, n, ncv, Q.data(), x.Vk.ptr, n, s);
-            } else {
-                ZBuf Qd((size_t)ncv * keep);
-                h2d(Qd.ptr, Q.data(), (size_t)ncv * keep * sizeof(double2));
-                GemmPlan gp;
-                gp.M = (int)n; gp.N = keep; gp.opA = OpN; gp.opB = OpN;
-                gp.begin_batch(x.Vk.ptr, n);
-                gp.add_seg(x.V.ptr, n, Qd.ptr, ncv, ncv);
-                zgemm_run(gp, s);
-            }
-            d2d(x.V.ptr, x.Vk.ptr, (size_t)n * keep * sizeof(double2));
-            d2d(f, tmp, (size_t)n * sizeof(double2));
-            std::vector<zd> Hk((size_t)keep * keep);
+            r.Q.assign((size_t)ncv * ncv, zd(0, 0));
+            for (int i = 0; i < ncv; ++i) r.Q[(size_t)i + (size_t)i * ncv] = 1.0;
+            for (int q = keep; q < ncv; ++q) shifted_qr_step(ncv, x.H, th[order[q]], r.Q);
+            r.hk = h(keep, keep - 1);
+            r.qm = r.Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
+            r.Hk.assign((size_t)keep * keep, zd(0, 0));
             for (int c = 0; c < keep; ++c)
-                for (int r = 0; r < keep; ++r) Hk[(size_t)r + (size_t)c * keep] = (r <= c + 1) ? h(r, c) : zd(0, 0);
+                for (int rr = 0; rr < keep; ++rr) r.Hk[(size_t)rr + (size_t)c * keep] = (rr <= c + 1) ? h(rr, c) : zd(0, 0);
+            rsts.push_back(std::move(r));
+        }
+        if (fins.empty() && rsts.empty()) continue;
+        // coefficients of all of them in one upload
+        std::vector<double2> hc;
+        std::vector<size_t> off_f(fins.size()), off_r(rsts.size());
+        for (size_t q = 0; q < fins.size(); ++q) {
+            off_f[q] = hc.size();
+            for (zd v : fins[q].y) hc.push_back(make_double2(v.real(), v.imag()));
+        }
+        for (size_t q = 0; q < rsts.size(); ++q) {
+            off_r[q] = hc.size();
+            for (size_t e = 0; e < (size_t)ncv * (rsts[q].keep + 1); ++e)
+                hc.push_back(make_double2(rsts[q].Q[e].real(), rsts[q].Q[e].imag()));
+        }
+        if (coefs.n < hc.size()) coefs.resize(hc.size());
+        h2d(coefs.ptr, hc.data(), hc.size() * sizeof(double2));
+        // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
+        GemmPlan g;
+        g.M = g.N = 1;
+        for (size_t q = 0; q < fins.size(); ++q) {
+            KState& x = st[fins[q].p];
+            g.begin_batch(x.task->result, x.n, (int)x.n, 1);
+            g.add_seg(x.V.ptr, x.n, coefs.ptr + off_f[q], fins[q].m, fins[q].m);
+        }
+        for (size_t q = 0; q < rsts.size(); ++q) {
+            KState& x = st[rsts[q].p];
+            if (x.Vk.n < (size_t)x.n * (ncv + 1)) x.Vk.resize((size_t)x.n * (ncv + 1));
+            g.begin_batch(x.Vk.ptr, x.n, (int)x.n, rsts[q].keep + 1);
+            g.add_seg(x.V.ptr, x.n, coefs.ptr + off_r[q], ncv, ncv);
+        }
+        zgemm_run(g, s);
+        // f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the basis
+        {
+            std::vector<double2*> y;
+            std::vector<const double2*> xv;
+            std::vector<zd> a, b;
+            std::vector<int64_t> nn;
+            for (auto const& r : rsts) {
+                KState& x = st[r.p];
+                y.push_back(x.Vk.ptr + (size_t)x.n * ncv);
+                xv.push_back(x.V.ptr + (size_t)x.n * ncv);
+                a.push_back(r.qm);
+                b.push_back(zd(0, 0));
+                nn.push_back(x.n);
+            }
+            axpby_batch(y, a, xv, b, nn, s);
+            xv.clear(); a.clear(); b.clear();
+            for (auto const& r : rsts) {
+                KState& x = st[r.p];
+                xv.push_back(x.Vk.ptr + (size_t)x.n * r.keep);
+                a.push_back(r.hk);
+                b.push_back(zd(1, 0));
+            }
+            axpby_batch(y, a, xv, b, nn, s);
+            for (auto const& r : rsts) std::swap(st[r.p].V, st[r.p].Vk);
+        }
+        // norms of the results, the new residuals and (keep = 1) of v_0
+        std::vector<const double2*> nv;
+        std::vector<int64_t> nl;
+        for (auto const& f : fins) { nv.push_back(st[f.p].task->result); nl.push_back(st[f.p].n); }
+        for (auto const& r : rsts) { nv.push_back(st[r.p].V.ptr + (size_t)st[r.p].n * ncv); nl.push_back(st[r.p].n); }
+        for (auto const& r : rsts)
+            if (r.keep == 1) { nv.push_back(st[r.p].V.ptr); nl.push_back(st[r.p].n); }
+        std::vector<double> ss = sumsq_batch(nv, nl, s);
+        std::vector<double2*> sx;
+        std::vector<int64_t> sn;
+        std::vector<double> sf;
+        size_t k2 = 0;
+        for (auto const& f : fins) {
+            KState& x = st[f.p];
+            sx.push_back(x.task->result);
+            sn.push_back(x.n);
+            sf.push_back(1.0 / std::sqrt(ss[k2++]));
+            x.task->out.eigenvalue = f.theta;
+            x.task->out.restarts = f.iter;
+            x.task->out.info = 0;
+            x.done = true;
+        }
+        std::vector<double> betas_r;
+        for (size_t q = 0; q < rsts.size(); ++q) betas_r.push_back(std::sqrt(ss[k2++]));
+        for (auto const& r : rsts)
+            if (r.keep == 1) {  // keep v_0 exactly unit norm against round-off drift
+                sx.push_back(st[r.p].V.ptr);
+                sn.push_back(st[r.p].n);
+                sf.push_back(1.0 / std::sqrt(ss[k2++]));
+            }
+        scale_batch(sx, sn, sf, s);
+        std::vector<double2*> vy;
+        std::vector<const double2*> vx;
+        std::vector<zd> va, vb;
+        std::vector<int64_t> vn;
+        for (size_t q = 0; q < rsts.size(); ++q) {
+            Rst const& r = rsts[q];
+            KState& x = st[r.p];
+            IramTask& t = *x.task;
+            const int keep = r.keep;
+            auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
             std::fill(x.H.begin(), x.H.end(), zd(0, 0));
             for (int c = 0; c < keep; ++c)
-                for (int r = 0; r < keep; ++r) h(r, c) = Hk[(size_t)r + (size_t)c * keep];
-            x.beta = dznrm2(f, n, s);
-            if (keep == 1) {
-                // keep v_0 exactly unit norm against round-off drift
-                const double n0 = dznrm2(x.V.ptr, n, s);
-                zscal(x.V.ptr, zd(1.0 / n0, 0), n, s);
-            }
+                for (int rr = 0; rr < keep; ++rr) h(rr, c) = r.Hk[(size_t)rr + (size_t)c * keep];
+            x.beta = betas_r[q];
             x.k = keep;
-            if (x.beta == 0.0 && continue_with_random(x, p, keep, s)) {
+            if (x.beta == 0.0 && continue_with_random(x, r.p, keep, s)) {
                 // the restart landed on an invariant subspace: continue from a
                 // random vector orthogonal to it (H(keep, keep - 1) = 0)
                 h(keep, keep - 1) = 0.0;
@@ -492,19 +576,22 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             }
             if (x.beta == 0.0) {
                 // the kept vectors span the whole space: their Ritz pairs are exact
-                std::vector<zd> th2, Y2;
+                std::vector<zd> Hk = r.Hk, th2, Y2;
                 hessenberg_eig(keep, Hk, th2, Y2);
                 const int w2 = wanted_index(th2, t.which);
                 std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
-                finish_with(x, keep, yv, th2[w2], iter, s);
+                finish_with(x, keep, yv, th2[w2], x.iter, s);
                 continue;
             }
             // the next extension starts from v_keep = f / beta, H(keep, keep - 1) = beta
-            double2* vk = x.V.ptr + (size_t)n * keep;
-            d2d(vk, f, (size_t)n * sizeof(double2));
-            zscal(vk, zd(1.0 / x.beta, 0), n, s);
+            vy.push_back(x.V.ptr + (size_t)x.n * keep);
+            vx.push_back(x.V.ptr + (size_t)x.n * ncv);
+            va.push_back(zd(1.0 / x.beta, 0));
+            vb.push_back(zd(0, 0));
+            vn.push_back(x.n);
             h(keep, keep - 1) = x.beta;
         }
+        axpby_batch(vy, va, vx, vb, vn, s);
     }
 }
 

@@ -566,4 +566,50 @@ void fill_random(double2* x, int64_t n, uint64_t key, hipStream_t s) {
     NC_HIP(hipGetLastError());
 }
 
+// y = a x + b y for several vectors (b = 0: y = a x, y not read)
+struct AxpbyDesc {
+    double2* y;
+    const double2* x;
+    double2 a, b;
+    int64_t n;
+};
+__global__ void axpby_batch_kernel(const AxpbyDesc* __restrict__ ds) {
+    const AxpbyDesc d = ds[blockIdx.y];
+    const bool usey = d.b.x != 0.0 || d.b.y != 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 o = make_double2(0.0, 0.0);
+        if (usey) {
+            const double2 yv = d.y[i];
+            o.x = d.b.x * yv.x - d.b.y * yv.y;
+            o.y = d.b.x * yv.y + d.b.y * yv.x;
+        }
+        if (d.x) {
+            const double2 xv = d.x[i];
+            o.x += d.a.x * xv.x - d.a.y * xv.y;
+            o.y += d.a.x * xv.y + d.a.y * xv.x;
+        }
+        d.y[i] = o;
+    }
+}
+
+void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::vector<const double2*> const& x,
+                 std::vector<zd> const& b, std::vector<int64_t> const& n, hipStream_t s) {
+    if (y.empty()) return;
+    std::vector<AxpbyDesc> ds(y.size());
+    int64_t maxn = 0;
+    for (size_t q = 0; q < y.size(); ++q) {
+        ds[q] = AxpbyDesc{y[q], x[q], make_double2(a[q].real(), a[q].imag()), make_double2(b[q].real(), b[q].imag()),
+                          n[q]};
+        maxn = std::max(maxn, n[q]);
+    }
+    if (maxn == 0) return;
+    int slot;
+    auto* dd = (const AxpbyDesc*)stage_descriptors(ds.data(), ds.size() * sizeof(AxpbyDesc), s, &slot);
+    const unsigned g = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)y.size()), (maxn + 255) / 256));
+    hipLaunchKernelGGL(axpby_batch_kernel, dim3(g, (unsigned)y.size()), dim3(256), 0, s, dd);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+}
+
 }  // namespace nc

@@ -206,6 +206,10 @@ struct MixItem {
 };
 void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s);
 void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s);
+// y_q = a_q x_q + b_q y_q for several vectors in one launch (x_q may be null;
+// b_q = 0 does not read y_q)
+void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::vector<const double2*> const& x,
+                 std::vector<zd> const& b, std::vector<int64_t> const& n, hipStream_t s);
 // element-wise precision changes of several vectors in one launch
 void to_c64_batch(std::vector<const double2*> const& src, std::vector<float2*> const& dst,
                   std::vector<int64_t> const& n, hipStream_t s);
