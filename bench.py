@@ -337,23 +337,26 @@ def main():
             c.close()
         log("batch of %d chains: one lockstep sweep %.2f s (%d site solves, %d matvecs)" % (nb, t_sweep, solves, mvs))
         # the same sweep one chain after another, on a sample of the chains
-        sample = list(range(0, nb, max(1, nb // args.batch_sample)))[:args.batch_sample]
-        cs = make(sample)
-        t0 = time.perf_counter()
-        b2 = ChainBatch(cs)
-        b2.perform_optimization_sweep()
-        t_b = time.perf_counter() - t0
-        b2.close()
-        for c in cs:
-            c.close()
-        cs = make(sample)
-        t0 = time.perf_counter()
-        for c in cs:
-            c.perform_optimization_sweep()
-        t_s = time.perf_counter() - t0
-        for c in cs:
-            c.close()
-        log("sample of %d chains: lockstep %.2f s, one after another %.2f s" % (len(sample), t_b, t_s))
+        # (--batch-sample 0 skips the comparison)
+        sample = list(range(0, nb, max(1, nb // max(1, args.batch_sample))))[:args.batch_sample]
+        t_b = t_s = float("nan")
+        if sample:
+            cs = make(sample)
+            t0 = time.perf_counter()
+            b2 = ChainBatch(cs)
+            b2.perform_optimization_sweep()
+            t_b = time.perf_counter() - t0
+            b2.close()
+            for c in cs:
+                c.close()
+            cs = make(sample)
+            t0 = time.perf_counter()
+            for c in cs:
+                c.perform_optimization_sweep()
+            t_s = time.perf_counter() - t0
+            for c in cs:
+                c.close()
+            log("sample of %d chains: lockstep %.2f s, one after another %.2f s" % (len(sample), t_b, t_s))
         result["batch"] = {
             "workload": "%d TFIM chains (H = -sum ZZ - h sum X, N=%d, d=2, MPO bond 3), h ~ U[0.5, 1.5], chi drawn "
                         "from {32, 64, 128, 256}; complex64 contractions, FP64 eigen-steps; one lockstep sweep from "
@@ -362,8 +365,9 @@ def main():
             "setup_s": round(t_setup, 2), "sweep_s": round(t_sweep, 2),
             "sweep_moves_s": round(bt["moves_ms"] / 1e3, 2), "sweep_site_solves_s": round(bt["site_solves_ms"] / 1e3, 2),
             "chain_sweeps_per_s": round(nb / t_sweep, 3), "site_solves": int(solves), "matvecs": int(mvs),
-            "sample_chains": len(sample), "sample_lockstep_s": round(t_b, 2), "sample_one_by_one_s": round(t_s, 2),
-            "batching_speedup": round(t_s / t_b, 2),
+            "sample_chains": len(sample), "sample_lockstep_s": round(t_b, 2) if sample else None,
+            "sample_one_by_one_s": round(t_s, 2) if sample else None,
+            "batching_speedup": round(t_s / t_b, 2) if sample else None,
             "energies_after_one_sweep": [round(e, 10) for e in energies],
         }
 
