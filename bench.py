@@ -336,6 +336,9 @@ def main():
         for c in cs:
             c.close()
         log("batch of %d chains: one lockstep sweep %.2f s (%d site solves, %d matvecs)" % (nb, t_sweep, solves, mvs))
+        log("batch split (s): moves %.2f, prepare %.2f, batched solves %.2f (Krylov %.2f), post-checks %.2f" % (
+            bt["moves_ms"] / 1e3, bt["prepare_ms"] / 1e3, bt["batched_solves_ms"] / 1e3, bt["krylov_ms"] / 1e3,
+            bt["post_checks_ms"] / 1e3))
         # the same sweep one chain after another, on a sample of the chains
         # (--batch-sample 0 skips the comparison)
         sample = list(range(0, nb, max(1, nb // max(1, args.batch_sample))))[:args.batch_sample]
@@ -364,6 +367,8 @@ def main():
             "chains": nb, "chi_counts": {str(k): int((chis == k).sum()) for k in (32, 64, 128, 256)},
             "setup_s": round(t_setup, 2), "sweep_s": round(t_sweep, 2),
             "sweep_moves_s": round(bt["moves_ms"] / 1e3, 2), "sweep_site_solves_s": round(bt["site_solves_ms"] / 1e3, 2),
+            "site_solves_split_s": {k: round(bt[k + "_ms"] / 1e3, 2)
+                                    for k in ("prepare", "batched_solves", "krylov", "post_checks")},
             "chain_sweeps_per_s": round(nb / t_sweep, 3), "site_solves": int(solves), "matvecs": int(mvs),
             "sample_chains": len(sample), "sample_lockstep_s": round(t_b, 2) if sample else None,
             "sample_one_by_one_s": round(t_s, 2) if sample else None,

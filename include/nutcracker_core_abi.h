@@ -277,8 +277,15 @@ nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n);
 void nc_chain_batch_destroy(nc_chain_batch* batch);
 int nc_chain_batch_sweep(nc_chain_batch* batch);
 int nc_chain_batch_sweep_until_converged(nc_chain_batch* batch);
-/* wall time spent in lockstep moves (times_ms[0]) and site solves (times_ms[1]) */
+/* wall time (6 values, ms) spent in lockstep moves [0] and site solves [1]; the
+   site solves split into per-chain preparation [2], the batched solves [3] and
+   per-chain post-checks [4]; [5] is the part of [3] inside the Krylov iteration */
 int nc_chain_batch_times(nc_chain_batch* batch, double* times_ms);
+/* Launch option of the batched Krylov solver (no counterpart in the reference):
+   on = 1 (default) records each Arnoldi extension of a lockstep site solve in a
+   HIP graph and replays it on later restarts; on = 0 launches the same kernels
+   one by one.  Results are identical either way.  Returns the previous setting. */
+int nc_set_krylov_graphs(int on);
 /* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,
    sites of the mixed form finished in FP64 (a complex64 solve that came out above
    the previous energy);

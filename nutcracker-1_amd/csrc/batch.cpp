@@ -35,25 +35,103 @@ ChainBatch::ChainBatch(std::vector<Chain*> cs) : chains(std::move(cs)) {
     }
 }
 
+namespace {
+// contract_sos_right for several chains in grouped launches: out = R' from R
+// (br x br x cr) and the site psi (br, bl, d), out bl x bl x cl
+struct EnvRightItem {
+    MpoSite const* op;
+    int bl, br;
+    const double2* R;
+    const double2* psi;
+    double2* out;
+    Workspace* ws;
+};
+void grouped_env_right(std::vector<EnvRightItem> const& items, hipStream_t s) {
+    GemmPlan A, Cp;
+    A.M = A.N = Cp.M = Cp.N = 1;
+    Cp.opA = OpC; Cp.opB = OpN;
+    std::vector<MixItem<double2>> mix;
+    for (auto const& it : items) {
+        MpoSite const& op = *it.op;
+        const int bl = it.bl, br = it.br, d = op.d;
+        const int64_t rs = (int64_t)br * bl;
+        Workspace& w = *it.ws;
+        w.ensure(rs * d * op.tos.size(), rs * d * op.froms.size());
+        for (size_t ti = 0; ti < op.tos.size(); ++ti)
+            for (int a = 0; a < d; ++a) {
+                A.begin_batch(w.P.ptr + ((int64_t)ti * d + a) * rs, br, br, bl);
+                A.add_seg(it.R + (int64_t)op.tos[ti] * br * br, br, it.psi + a * rs, br, br);
+            }
+        mix.push_back(MixItem<double2>{&op, w.P.ptr, w.Q.ptr, rs, false});
+        if ((int)op.froms.size() < op.cl) dzero(it.out, (size_t)bl * bl * op.cl * sizeof(double2));
+        for (size_t fi = 0; fi < op.froms.size(); ++fi) {
+            Cp.begin_batch(it.out + (int64_t)op.froms[fi] * bl * bl, bl, bl, bl);
+            for (int b = 0; b < d; ++b)
+                Cp.add_seg(it.psi + b * rs, br, w.Q.ptr + ((int64_t)fi * d + b) * rs, br, br);
+        }
+    }
+    zgemm_run(A, s);
+    mix_blocks_batch(mix, s);
+    zgemm_run(Cp, s);
+}
+}  // namespace
+
 void ChainBatch::optimize_sites(std::vector<Chain*> const& active) {
     hipStream_t s = Context::get().stream;
     std::vector<std::unique_ptr<SitePrep>> preps;
     std::vector<OptimizeArgs const*> args;
     std::vector<Workspace*> wss;
+    double t0 = now_ms();
     for (Chain* c : active) {
         preps.emplace_back(new SitePrep());
         c->prepare_site(*preps.back());
         args.push_back(&preps.back()->a);
         wss.push_back(&c->ws);
     }
+    double t1 = now_ms();
+    t_prepare_ms += t1 - t0;
     std::vector<OptimizeOut> outs;
-    optimize_sites_batched(args, wss, outs, s);
+    optimize_sites_batched(args, wss, outs, s, &t_krylov_ms);
+    t0 = now_ms();
+    t_batched_ms += t0 - t1;
+    // the expectation values of the post-checks (compute_expectation, core.f95:
+    // 1045-1088) of every solved chain: one grouped environment contraction and
+    // one batched trace
+    {
+        std::vector<EnvRightItem> items;
+        std::vector<ZBuf> Ns;
+        std::vector<size_t> who;
+        for (size_t q = 0; q < active.size(); ++q) {
+            if (outs[q].info != 0) continue;
+            Chain* c = active[q];
+            const int i = c->current;
+            MpoSite const& op = *c->ops[i];
+            const int bl = c->bonds[i], br = c->bonds[i + 1];
+            Ns.emplace_back((size_t)bl * bl * op.cl);
+            items.push_back(EnvRightItem{&op, bl, br, c->right_envs[i + 1].ptr, preps[q]->result.ptr, Ns.back().ptr,
+                                         &c->ws});
+            who.push_back(q);
+        }
+        grouped_env_right(items, s);
+        std::vector<RedItem> tr;
+        for (size_t k = 0; k < who.size(); ++k) {
+            Chain* c = active[who[k]];
+            const int i = c->current;
+            tr.push_back(RedItem{Ns[k].ptr, c->left_envs[i].ptr, (int64_t)c->bonds[i], (int64_t)c->ops[i]->cl, true});
+        }
+        const std::vector<zd> ex = reduce_batch(tr, s);
+        for (size_t k = 0; k < who.size(); ++k) {
+            preps[who[k]]->have_ex = true;
+            preps[who[k]]->ex = ex[k];
+        }
+    }
     for (size_t q = 0; q < active.size(); ++q) {
         Chain* c = active[q];
         int st = c->finish_site(*preps[q], outs[q]);
         if (st != 0 && c->opts.rethrow)
             throw Error("optimizer failure at site " + std::to_string(c->current) + " of a batched chain", st);
     }
+    t_finish_ms += now_ms() - t0;
 }
 
 // ---------------------------------------------------------------------------
@@ -92,36 +170,18 @@ void ChainBatch::env_left_batch(std::vector<Chain*> const& cs, hipStream_t s) {
     zgemm_run(Cp, s);
 }
 
+
 void ChainBatch::env_right_batch(std::vector<Chain*> const& cs, hipStream_t s) {
     // R_i = contract_sos_right(R_{i+1}, site i), i = current + 1 (after the move)
-    GemmPlan A, Cp;
-    A.M = A.N = Cp.M = Cp.N = 1;
-    Cp.opA = OpC; Cp.opB = OpN;
-    std::vector<MixItem<double2>> mix;
+    std::vector<EnvRightItem> items;
     for (Chain* c : cs) {
         const int i = c->current + 1;
         MpoSite const& op = *c->ops[i];
-        const int bl = c->bonds[i], br = c->bonds[i + 1], d = op.d;
-        const int64_t rs = (int64_t)br * bl;
-        c->ws.ensure(rs * d * op.tos.size(), rs * d * op.froms.size());
-        c->right_envs[i].resize((size_t)bl * bl * op.cl);
-        const double2* psi = c->sites[i].ptr;
-        for (size_t ti = 0; ti < op.tos.size(); ++ti)
-            for (int a = 0; a < d; ++a) {
-                A.begin_batch(c->ws.P.ptr + ((int64_t)ti * d + a) * rs, br, br, bl);
-                A.add_seg(c->right_envs[i + 1].ptr + (int64_t)op.tos[ti] * br * br, br, psi + a * rs, br, br);
-            }
-        mix.push_back(MixItem<double2>{&op, c->ws.P.ptr, c->ws.Q.ptr, rs, false});
-        if ((int)op.froms.size() < op.cl) dzero(c->right_envs[i].ptr, (size_t)bl * bl * op.cl * sizeof(double2));
-        for (size_t fi = 0; fi < op.froms.size(); ++fi) {
-            Cp.begin_batch(c->right_envs[i].ptr + (int64_t)op.froms[fi] * bl * bl, bl, bl, bl);
-            for (int b = 0; b < d; ++b)
-                Cp.add_seg(psi + b * rs, br, c->ws.Q.ptr + ((int64_t)fi * d + b) * rs, br, br);
-        }
+        c->right_envs[i].resize((size_t)c->bonds[i] * c->bonds[i] * op.cl);
+        items.push_back(EnvRightItem{&op, c->bonds[i], c->bonds[i + 1], c->right_envs[i + 1].ptr, c->sites[i].ptr,
+                                     c->right_envs[i].ptr, &c->ws});
     }
-    zgemm_run(A, s);
-    mix_blocks_batch(mix, s);
-    zgemm_run(Cp, s);
+    grouped_env_right(items, s);
 }
 
 void ChainBatch::move_all(std::vector<Chain*> const& cs, int dir) {

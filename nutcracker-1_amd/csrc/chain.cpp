@@ -334,7 +334,9 @@ int Chain::check_site(SitePrep& p, OptimizeOut const& o) {
     const double floor_tol = opts.mixed && opts.env32 ? std::max(opts.sanity, opts.env32_floor) : opts.sanity;
     if (status == 0) {
         zd ev = o.eigenvalue;
-        zd ex = expectation(bl, br, op, left_envs[i].ptr, p.result.ptr, right_envs[i + 1].ptr, ws, s);
+        const zd ex = p.have_ex ? p.ex : expectation(bl, br, op, left_envs[i].ptr, p.result.ptr, right_envs[i + 1].ptr,
+                                                     ws, s);
+        p.have_ex = false;  // a refined result is checked against its own value
         const double san = opts.sanity;
         if ((std::abs(ev) + std::abs(ex)) / 2 > san && std::abs(ev - ex) / (std::abs(ev) + std::abs(ex) + san) > san)
             status = 101;  // OptimizerObtainedEigenvalueDifferentFromExpectationValue

@@ -226,6 +226,10 @@ int nc_chain_batch_times(nc_chain_batch* b, double* times_ms) {
     return guarded_chain("nc_chain_batch_times", [&] {
         times_ms[0] = b->batch->t_moves_ms;
         times_ms[1] = b->batch->t_solves_ms;
+        times_ms[2] = b->batch->t_prepare_ms;
+        times_ms[3] = b->batch->t_batched_ms;
+        times_ms[4] = b->batch->t_finish_ms;
+        times_ms[5] = b->batch->t_krylov_ms;
         return 0;
     });
 }
@@ -254,6 +258,12 @@ int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
         times_ms[3] = s.t_reset_ms;
         return 0;
     });
+}
+
+int nc_set_krylov_graphs(int on) {
+    const int prev = nc::krylov_graphs() ? 1 : 0;
+    nc::krylov_graphs() = on != 0;
+    return prev;
 }
 
 }  // extern "C"

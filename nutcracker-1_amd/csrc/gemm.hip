@@ -59,7 +59,8 @@ struct BatchT {
     typename Cplx<T>::type* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
-    int32_t M, N;  // output size of this batch (filled in on the host)
+    int32_t M, N;      // output size of this batch (filled in on the host)
+    const int* live;   // null, or a device flag: 0 = skip this batch in this launch
 };
 
 template <typename T, int OPA, int OPB, int BK>
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
     // every batch has its own M x N (grouped launch of unequal problems): the
     // grid covers the largest, surplus workgroups of smaller batches exit
     const BatchT<T> bt = batches[blockIdx.y];
+    if (bt.live && *bt.live == 0) return;
     const int M = bt.M, N = bt.N;
     const int tiles_m = (M + TBM - 1) / TBM, tiles_n = (N + TBN - 1) / TBN;
     const int ntiles = tiles_m * tiles_n;
@@ -258,8 +260,10 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
 // C_b = sum_c W_{b,c} (+ C_b) for the split-K partial products
 template <typename CT>
 __global__ void splitk_reduce_kernel(const CT* __restrict__ W, int64_t slice, int nsplit, CT* const* __restrict__ Cptr,
-                                     const int64_t* __restrict__ ldc, int M, int N, int accumulate) {
+                                     const int64_t* __restrict__ ldc, const int* const* __restrict__ live, int M, int N,
+                                     int accumulate) {
     const int b = blockIdx.y;
+    if (live[b] && *live[b] == 0) return;
     const int64_t total = (int64_t)M * N;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e % M, c = e / M;
@@ -308,7 +312,8 @@ struct Arena {
         next = (next + 1) % kSlots;
         NC_HIP(hipEventSynchronize(done[slot]));
         char* h = host + slot * kSlot;
-        for (int i = 0; i < nparts; ++i) std::memcpy(h + offsets[i], parts[i], sizes[i]);
+        for (int i = 0; i < nparts; ++i)
+            if (sizes[i]) std::memcpy(h + offsets[i], parts[i], sizes[i]);
         char* d = dev + slot * kSlot;
         NC_HIP(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, s));
         *slot_out = slot;
@@ -374,10 +379,34 @@ void split_segments(std::vector<Seg> const& segs, int sb, int se, int nsplit, in
     }
 }
 
+// A plan's launch shape and final descriptor lists (split-K expanded)
 template <typename T>
-void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
-              std::vector<SegT<T>> const& segs, hipStream_t s) {
+struct Built {
     typedef typename Cplx<T>::type CT;
+    std::vector<BatchT<T>> b;
+    std::vector<SegT<T>> sg;
+    std::vector<CT*> cptr;          // split-K: the real outputs, one per original batch
+    std::vector<int64_t> ldcs;
+    std::vector<const int*> lives;
+    int opA = OpN, opB = OpN;
+    bool acc = false, m3 = false;
+    int nsplit = 1;
+    int64_t slice = 0, maxtiles = 0;
+    int M = 0, N = 0;
+    CT* W = nullptr;                // split-K workspace
+    bool empty() const { return b.empty() || maxtiles <= 0; }
+};
+
+// Fills per-batch sizes, chooses split-K and expands it.  `work(bytes)` gives the
+// split-K workspace.
+template <typename T, typename WorkFn>
+void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
+                std::vector<SegT<T>> const& segs, Built<T>& out, WorkFn work) {
+    typedef typename Cplx<T>::type CT;
+    out = Built<T>();
+    out.opA = opA;
+    out.opB = opB;
+    out.acc = accumulate;
     if (batches.empty()) return;
     // per-batch sizes (0 = the plan's); a grouped launch of unequal batches
     bool uniform = true;
@@ -390,9 +419,12 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
         maxtiles = std::max(maxtiles, t);
         tiles += t;
     }
+    out.maxtiles = maxtiles;
     if (maxtiles <= 0) return;
     M = batches[0].M;
     N = batches[0].N;
+    out.M = M;
+    out.N = N;
     int64_t maxk = 0;
     for (auto const& b : batches) {
         int64_t k = 0;
@@ -404,31 +436,18 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
     if (uniform && g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
         while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
     }
-    const bool m3 = sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32;
-    Arena& ar = arena();
+    out.nsplit = nsplit;
+    out.m3 = sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32;
     if (nsplit == 1) {
-        const void* parts[2] = {batches.data(), segs.data()};
-        size_t sizes[2] = {batches.size() * sizeof(BatchT<T>), segs.size() * sizeof(SegT<T>)};
-        size_t offs[2];
-        int slot;
-        char* d = ar.stage(parts, sizes, 2, offs, s, &slot);
-        dim3 grid((unsigned)maxtiles, (unsigned)batches.size());
-        launch_any<T>(opA, opB, accumulate, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]),
-                      s);
-        NC_HIP(hipGetLastError());
-        ar.release(slot, s);
+        out.b = batches;
+        out.sg = segs;
         return;
     }
     // split-K: partial products into a workspace, then a reduction into C
     const int64_t slice = (int64_t)M * N;
-    static thread_local DevBuf<char> work;
-    const size_t need = (size_t)slice * nsplit * batches.size() * sizeof(CT);
-    if (work.n < need) work.resize(need);
-    CT* W = (CT*)work.ptr;
-    std::vector<BatchT<T>> nb;
-    std::vector<SegT<T>> ns;
-    std::vector<CT*> cptr;
-    std::vector<int64_t> ldcs;
+    out.slice = slice;
+    CT* W = (CT*)work((size_t)slice * nsplit * batches.size() * sizeof(CT));
+    out.W = W;
     for (size_t b = 0; b < batches.size(); ++b) {
         std::vector<std::vector<SegT<T>>> chunks;
         split_segments(segs, batches[b].seg_begin, batches[b].seg_end, nsplit, opA, opB, chunks);
@@ -438,28 +457,111 @@ void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Batch
             x.ldc = M;
             x.M = M;
             x.N = N;
-            x.seg_begin = (int32_t)ns.size();
-            for (auto const& sg : chunks[q]) ns.push_back(sg);
-            x.seg_end = (int32_t)ns.size();
-            nb.push_back(x);
+            x.live = batches[b].live;
+            x.seg_begin = (int32_t)out.sg.size();
+            for (auto const& sg : chunks[q]) out.sg.push_back(sg);
+            x.seg_end = (int32_t)out.sg.size();
+            out.b.push_back(x);
         }
-        cptr.push_back(batches[b].C);
-        ldcs.push_back(batches[b].ldc);
+        out.cptr.push_back(batches[b].C);
+        out.ldcs.push_back(batches[b].ldc);
+        out.lives.push_back(batches[b].live);
     }
-    const void* parts[4] = {nb.data(), ns.data(), cptr.data(), ldcs.data()};
-    size_t sizes[4] = {nb.size() * sizeof(BatchT<T>), ns.size() * sizeof(SegT<T>), cptr.size() * sizeof(CT*),
-                       ldcs.size() * sizeof(int64_t)};
-    size_t offs[4];
+}
+
+// device copies of a built plan's lists
+template <typename T>
+struct BuiltDev {
+    const BatchT<T>* b;
+    const SegT<T>* sg;
+    typename Cplx<T>::type* const* cptr;
+    const int64_t* ldcs;
+    const int* const* lives;
+};
+
+template <typename T>
+void parts_of(Built<T> const& bp, const void* parts[5], size_t sizes[5]) {
+    parts[0] = bp.b.data();     sizes[0] = bp.b.size() * sizeof(BatchT<T>);
+    parts[1] = bp.sg.data();    sizes[1] = bp.sg.size() * sizeof(SegT<T>);
+    parts[2] = bp.cptr.data();  sizes[2] = bp.cptr.size() * sizeof(void*);
+    parts[3] = bp.ldcs.data();  sizes[3] = bp.ldcs.size() * sizeof(int64_t);
+    parts[4] = bp.lives.data(); sizes[4] = bp.lives.size() * sizeof(void*);
+}
+
+template <typename T>
+BuiltDev<T> dev_of(char* d, const size_t offs[5]) {
+    typedef typename Cplx<T>::type CT;
+    return BuiltDev<T>{(const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), (CT* const*)(d + offs[2]),
+                       (const int64_t*)(d + offs[3]), (const int* const*)(d + offs[4])};
+}
+
+template <typename T>
+void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
+    typedef typename Cplx<T>::type CT;
+    dim3 grid((unsigned)bp.maxtiles, (unsigned)bp.b.size());
+    launch_any<T>(bp.opA, bp.opB, bp.nsplit == 1 ? bp.acc : false, bp.m3, grid, dv.b, dv.sg, s);
+    NC_HIP(hipGetLastError());
+    if (bp.nsplit == 1) return;
+    unsigned g = (unsigned)std::min<int64_t>(2048, (bp.slice + 255) / 256);
+    hipLaunchKernelGGL(splitk_reduce_kernel<CT>, dim3(g, (unsigned)bp.cptr.size()), dim3(256), 0, s, bp.W, bp.slice,
+                       bp.nsplit, dv.cptr, dv.ldcs, dv.lives, bp.M, bp.N, bp.acc ? 1 : 0);
+    NC_HIP(hipGetLastError());
+}
+
+template <typename T>
+void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
+              std::vector<SegT<T>> const& segs, hipStream_t s) {
+    static thread_local DevBuf<char> work;
+    Built<T> bp;
+    build_plan<T>(opA, opB, accumulate, M, N, batches, segs, bp, [&](size_t bytes) {
+        if (work.n < bytes) work.resize(bytes);
+        return (void*)work.ptr;
+    });
+    if (bp.empty()) return;
+    const void* parts[5];
+    size_t sizes[5], offs[5];
+    parts_of(bp, parts, sizes);
     int slot;
-    char* d = ar.stage(parts, sizes, 4, offs, s, &slot);
-    dim3 grid((unsigned)maxtiles, (unsigned)nb.size());
-    launch_any<T>(opA, opB, false, m3, grid, (const BatchT<T>*)(d + offs[0]), (const SegT<T>*)(d + offs[1]), s);
-    NC_HIP(hipGetLastError());
-    unsigned g = (unsigned)std::min<int64_t>(2048, (slice + 255) / 256);
-    hipLaunchKernelGGL(splitk_reduce_kernel<CT>, dim3(g, (unsigned)batches.size()), dim3(256), 0, s, W, slice, nsplit,
-                       (CT* const*)(d + offs[2]), (const int64_t*)(d + offs[3]), M, N, accumulate ? 1 : 0);
-    NC_HIP(hipGetLastError());
-    ar.release(slot, s);
+    char* d = arena().stage(parts, sizes, 5, offs, s, &slot);
+    launch_built(bp, dev_of<T>(d, offs), s);
+    arena().release(slot, s);
+}
+
+// a plan uploaded once into device memory of its own
+template <typename T>
+struct PreparedGemmT final : PreparedLaunch {
+    Built<T> bp;
+    DevBuf<char> desc, work;
+    BuiltDev<T> dv{};
+    void launch(hipStream_t s) const override {
+        if (!bp.empty()) launch_built(bp, dv, s);
+    }
+};
+
+template <typename T>
+std::unique_ptr<PreparedLaunch> prepare_plan(int opA, int opB, bool accumulate, int M, int N,
+                                             std::vector<BatchT<T>>& batches, std::vector<SegT<T>> const& segs) {
+    std::unique_ptr<PreparedGemmT<T>> p(new PreparedGemmT<T>());
+    PreparedGemmT<T>* q = p.get();
+    build_plan<T>(opA, opB, accumulate, M, N, batches, segs, q->bp, [q](size_t bytes) {
+        q->work.resize(bytes);
+        return (void*)q->work.ptr;
+    });
+    if (q->bp.empty()) return p;
+    const void* parts[5];
+    size_t sizes[5], offs[5], off = 0;
+    parts_of(q->bp, parts, sizes);
+    for (int i = 0; i < 5; ++i) {
+        offs[i] = off;
+        off = (off + sizes[i] + 255) & ~size_t(255);
+    }
+    std::vector<char> host(off);
+    for (int i = 0; i < 5; ++i)
+        if (sizes[i]) std::memcpy(host.data() + offs[i], parts[i], sizes[i]);
+    q->desc.resize(off);
+    h2d(q->desc.ptr, host.data(), off);
+    q->dv = dev_of<T>(q->desc.ptr, offs);
+    return p;
 }
 
 }  // namespace
@@ -484,12 +586,31 @@ void zgemm_run(GemmPlan const& plan, hipStream_t s) {
     run_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
 }
 
+static void to_float_lists(CGemmPlan const& plan, std::vector<BatchT<float>>& b, std::vector<SegT<float>>& sg) {
+    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N, x.live});
+    for (auto const& x : plan.segs) sg.push_back(SegT<float>{x.A, x.B, x.lda, x.ldb, x.K, 0});
+}
+
 void cgemm_run(CGemmPlan const& plan, hipStream_t s) {
     std::vector<BatchT<float>> b;
     std::vector<SegT<float>> sg;
-    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N});
-    for (auto const& x : plan.segs) sg.push_back(SegT<float>{x.A, x.B, x.lda, x.ldb, x.K, 0});
+    to_float_lists(plan, b, sg);
     run_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
+}
+
+std::unique_ptr<PreparedLaunch> prepare_gemm(GemmPlan const& plan) {
+    std::vector<BatchT<double>> b(plan.batches.size());
+    std::vector<SegT<double>> sg(plan.segs.size());
+    std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
+    std::memcpy(sg.data(), plan.segs.data(), sg.size() * sizeof(GemmSeg));
+    return prepare_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg);
+}
+
+std::unique_ptr<PreparedLaunch> prepare_gemm(CGemmPlan const& plan) {
+    std::vector<BatchT<float>> b;
+    std::vector<SegT<float>> sg;
+    to_float_lists(plan, b, sg);
+    return prepare_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg);
 }
 
 }  // namespace nc

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 namespace nc {
@@ -191,6 +192,83 @@ __global__ __launch_bounds__(KT) void kscale_kernel(const KDesc* __restrict__ de
     }
 }
 
+// operator vectors of a persistent batch
+struct IoDesc {
+    void* in;
+    void* out;
+};
+
+// in[p] = V(:, j) (narrowed to complex64 when SINGLE) for every problem that
+// takes step j; live[p] tells the operator's launches which problems do
+template <bool SINGLE>
+__global__ __launch_bounds__(KT) void kgather_kernel(const KDesc* __restrict__ desc, const IoDesc* __restrict__ io,
+                                                     int* __restrict__ live, int j) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    const bool on = j >= d.kstart;
+    if (blockIdx.x == 0 && threadIdx.x == 0) live[p] = on ? 1 : 0;
+    if (!on) return;
+    const double2* __restrict__ src = d.V + (int64_t)j * d.n;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < d.n; i += (int64_t)gridDim.x * KT) {
+        const double2 v = src[i];
+        if (SINGLE) static_cast<float2*>(io[p].in)[i] = make_float2((float)v.x, (float)v.y);
+        else static_cast<double2*>(io[p].in)[i] = v;
+    }
+}
+
+// V(:, j+1) = out[p] (widened from complex64 when SINGLE)
+template <bool SINGLE>
+__global__ __launch_bounds__(KT) void kscatter_kernel(const KDesc* __restrict__ desc, const IoDesc* __restrict__ io,
+                                                      int j) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    double2* __restrict__ dst = d.V + (int64_t)(j + 1) * d.n;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < d.n; i += (int64_t)gridDim.x * KT) {
+        if (SINGLE) {
+            const float2 v = static_cast<const float2*>(io[p].out)[i];
+            dst[i] = make_double2((double)v.x, (double)v.y);
+        } else {
+            dst[i] = static_cast<const double2*>(io[p].out)[i];
+        }
+    }
+}
+
+// page-locked host array (stream-ordered copies without a host wait)
+template <typename T>
+struct Pinned {
+    T* ptr = nullptr;
+    size_t n = 0;
+    Pinned() = default;
+    Pinned(Pinned const&) = delete;
+    Pinned& operator=(Pinned const&) = delete;
+    ~Pinned() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    void reserve(size_t count) {
+        if (count <= n) return;
+        if (ptr) NC_HIP(hipHostFree(ptr));
+        ptr = nullptr;
+        NC_HIP(hipHostMalloc((void**)&ptr, count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+    T& operator[](size_t i) { return ptr[i]; }
+    T const& operator[](size_t i) const { return ptr[i]; }
+};
+
+// instantiated graphs of one solve, destroyed with it
+struct GraphCache {
+    std::vector<std::pair<int, hipGraphExec_t>> execs;
+    ~GraphCache() {
+        for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+    }
+    hipGraphExec_t find(int key) const {
+        for (auto const& e : execs)
+            if (e.first == key) return e.second;
+        return nullptr;
+    }
+};
+
 int wanted_index(std::vector<zd> const& th, const char* which) {
     int best = 0;
     for (int i = 1; i < (int)th.size(); ++i) {
@@ -337,13 +415,63 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                                                                    std::max(1, 2048 / P)));
     const int ldp = ncv + 2;
     ZBuf partial((size_t)P * ldp * nblk), coef1((size_t)P * ldp), coef2((size_t)P * ldp);
-    ZBuf Hd((size_t)P * (ncv + 1) * ncv);
-    DevBuf<double> pnorm((size_t)P * nblk), betas((size_t)P * ncv), scale(P);
+    // Hessenberg columns and subdiagonals of all problems in one buffer, read
+    // back with one copy per extension
+    const size_t nH = (size_t)P * (ncv + 1) * ncv, nB = (size_t)P * ncv;
+    ZBuf HB(nH + (nB + 1) / 2);
+    double2* const Hd_ptr = HB.ptr;
+    double* const betas_ptr = reinterpret_cast<double*>(HB.ptr + nH);
+    DevBuf<double> pnorm((size_t)P * nblk), scale(P);
     DevBuf<KDesc> ddesc(P);
     ZBuf coefs;  // Ritz / restart coefficients of one round, uploaded together
-    std::vector<KDesc> hdesc(P);
-    std::vector<double2> Hh((size_t)P * (ncv + 1) * ncv);
-    std::vector<double> bh((size_t)P * ncv);
+    Pinned<KDesc> hdesc;
+    hdesc.reserve(P);
+    Pinned<double2> hHB;
+    hHB.reserve(HB.n);
+    const double2* const Hh = hHB.ptr;
+    const double* const bh = reinterpret_cast<const double*>(hHB.ptr + nH);
+    Pinned<double2> hcoef;
+    // persistent operator: fixed in/out vectors, launches prepared once
+    const bool pers = apply.persistent();
+    DevBuf<int> live;
+    DevBuf<IoDesc> iod;
+    GraphCache graphs;
+    if (pers) {
+        if ((int)apply.in.size() != P || (int)apply.out.size() != P)
+            throw Error("iram_batch: persistent operator needs one input and one output per problem", -8);
+        std::vector<IoDesc> hio(P);
+        for (int p = 0; p < P; ++p) hio[p] = IoDesc{apply.in[p], apply.out[p]};
+        iod.resize(P);
+        h2d(iod.ptr, hio.data(), (size_t)P * sizeof(IoDesc));
+        std::vector<int> zeros(P, 0);
+        live.resize(P);
+        h2d(live.ptr, zeros.data(), (size_t)P * sizeof(int));
+        apply.bind(live.ptr, s);
+    }
+    const dim3 g(nblk, P), gr(ldp, P);
+    // one Arnoldi step of every problem active at step j, persistent form
+    auto step = [&](int j) {
+        if (apply.single) hipLaunchKernelGGL(kgather_kernel<true>, g, dim3(KT), 0, s, ddesc.ptr, iod.ptr, live.ptr, j);
+        else hipLaunchKernelGGL(kgather_kernel<false>, g, dim3(KT), 0, s, ddesc.ptr, iod.ptr, live.ptr, j);
+        NC_HIP(hipGetLastError());
+        apply.launch(s);
+        if (apply.single) hipLaunchKernelGGL(kscatter_kernel<true>, g, dim3(KT), 0, s, ddesc.ptr, iod.ptr, j);
+        else hipLaunchKernelGGL(kscatter_kernel<false>, g, dim3(KT), 0, s, ddesc.ptr, iod.ptr, j);
+        NC_HIP(hipGetLastError());
+    };
+    // the eight Gram-Schmidt kernels of step j
+    auto orthogonalize = [&](int j) {
+        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr);
+        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
+        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr);
+        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr);
+        hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr, ldp,
+                           pnorm.ptr, nblk, Hd_ptr, betas_ptr, scale.ptr);
+        hipLaunchKernelGGL(kscale_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, ncv, scale.ptr);
+        NC_HIP(hipGetLastError());
+    };
 
     for (;;) {
         int jmin = ncv;
@@ -355,50 +483,86 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             if (!st[p].done) jmin = std::min(jmin, st[p].k);
         }
         if (jmin >= ncv) break;  // every problem finished
-        h2d(ddesc.ptr, hdesc.data(), (size_t)P * sizeof(KDesc));
+        // stream-ordered: the host array is rewritten only after the next read-back
+        NC_HIP(hipMemcpyAsync(ddesc.ptr, hdesc.ptr, (size_t)P * sizeof(KDesc), hipMemcpyHostToDevice, s));
         // ---- extend every active factorization from its k to ncv
-        std::vector<int> who;
-        std::vector<const double2*> xs;
-        std::vector<double2*> ys;
-        for (int j = jmin; j < ncv; ++j) {
-            who.clear(); xs.clear(); ys.clear();
-            for (int p = 0; p < P; ++p) {
-                if (st[p].done || st[p].k > j) continue;
-                who.push_back(p);
-                xs.push_back(st[p].V.ptr + (size_t)j * st[p].n);
-                ys.push_back(st[p].V.ptr + (size_t)(j + 1) * st[p].n);
-                ++tasks[p].out.matvecs;
+        for (int j = jmin; j < ncv; ++j)
+            for (int p = 0; p < P; ++p)
+                if (!st[p].done && st[p].k <= j) ++tasks[p].out.matvecs;
+        if (pers) {
+            // The extension from jmin depends only on device data (the problem
+            // table above, the bases, the prepared launches), so it is recorded
+            // once per jmin and replayed on later restarts.
+            auto extend = [&]() {
+                for (int j = jmin; j < ncv; ++j) {
+                    step(j);
+                    orthogonalize(j);
+                }
+            };
+            if (krylov_graphs()) {
+                hipGraphExec_t ge = graphs.find(jmin);
+                if (!ge) {
+                    hipGraph_t gph = nullptr;
+                    NC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        extend();
+                    } catch (...) {
+                        (void)hipStreamEndCapture(s, &gph);
+                        if (gph) (void)hipGraphDestroy(gph);
+                        throw;
+                    }
+                    NC_HIP(hipStreamEndCapture(s, &gph));
+                    const hipError_t e = hipGraphInstantiate(&ge, gph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(gph);
+                    NC_HIP(e);
+                    graphs.execs.emplace_back(jmin, ge);
+                }
+                NC_HIP(hipGraphLaunch(ge, s));
+            } else {
+                extend();
             }
-            apply(who, xs, ys, s);
-            const dim3 g(nblk, P), gr(ldp, P);
-            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
-            hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr);
-            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
-            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
-            hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr);
-            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr);
-            hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr, ldp,
-                               pnorm.ptr, nblk, Hd.ptr, betas.ptr, scale.ptr);
-            hipLaunchKernelGGL(kscale_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, ncv, scale.ptr);
-            NC_HIP(hipGetLastError());
+        } else {
+            std::vector<int> who;
+            std::vector<const double2*> xs;
+            std::vector<double2*> ys;
+            for (int j = jmin; j < ncv; ++j) {
+                who.clear(); xs.clear(); ys.clear();
+                for (int p = 0; p < P; ++p) {
+                    if (st[p].done || st[p].k > j) continue;
+                    who.push_back(p);
+                    xs.push_back(st[p].V.ptr + (size_t)j * st[p].n);
+                    ys.push_back(st[p].V.ptr + (size_t)(j + 1) * st[p].n);
+                }
+                apply(who, xs, ys, s);
+                orthogonalize(j);
+            }
         }
-        d2h(Hh.data(), Hd.ptr, Hh.size() * sizeof(double2));
-        d2h(bh.data(), betas.ptr, bh.size() * sizeof(double));
+        NC_HIP(hipMemcpyAsync(hHB.ptr, HB.ptr, HB.n * sizeof(double2), hipMemcpyDeviceToHost, s));
+        NC_HIP(hipStreamSynchronize(s));
         // ---- per problem (host): Ritz values, convergence, restart shifts.  The
         // device work of the finishing and restarting problems is then done for
         // all of them at once: one grouped GEMM, batched vector updates, one
         // host read of their norms.
         struct Fin { int p, m, iter; std::vector<zd> y; zd theta; };
         struct Rst { int p, keep; std::vector<zd> Q; zd hk, qm; std::vector<zd> Hk; };
-        std::vector<Fin> fins;
-        std::vector<Rst> rsts;
-        for (int p = 0; p < P; ++p) {
+        // Per problem, on host threads (no device work): merge the new
+        // Hessenberg columns, Ritz values, convergence test, restart shifts.
+        enum { D_NONE, D_STOP, D_FAIL, D_FIN, D_RST };
+        struct Dec {
+            int kind = D_NONE;
+            int stop = -1;
+            Fin f;
+            Rst r;
+        };
+        std::vector<Dec> dec(P);
+        host_parallel_for(P, [&](int p) {
             KState& x = st[p];
-            if (x.done) continue;
+            if (x.done) return;
+            Dec& D = dec[p];
             IramTask& t = *x.task;
             const int iter = ++x.iter;
             auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
-            const double2* Hp = Hh.data() + (size_t)p * (ncv + 1) * ncv;
+            const double2* Hp = Hh + (size_t)p * (ncv + 1) * ncv;
             int stop = -1;  // first step whose new vector vanished (invariant subspace)
             for (int c = x.k; c < ncv; ++c) {
                 for (int r = 0; r <= c; ++r) h(r, c) = zd(Hp[r + (size_t)c * (ncv + 1)].x, Hp[r + (size_t)c * (ncv + 1)].y);
@@ -408,6 +572,62 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 if (stop >= 0) break;
             }
             if (stop >= 0) {
+                D.kind = D_STOP;
+                D.stop = stop;
+                return;
+            }
+            x.beta = bh[(size_t)p * ncv + ncv - 1];
+            std::vector<zd> th, Y;
+            hessenberg_eig(ncv, x.H, th, Y);
+            const int wi = wanted_index(th, t.which);
+            const double est = x.beta * std::abs(Y[(size_t)(ncv - 1) + (size_t)wi * ncv]);
+            const double tol_eff = t.tol > 0 ? t.tol : eps;
+            const bool converged = est <= tol_eff * std::max(eps23, std::abs(th[wi]));
+            if (converged || iter >= t.max_restarts) {
+                if (!converged) {
+                    D.kind = D_FAIL;
+                    return;
+                }
+                D.kind = D_FIN;
+                D.f = Fin{p, ncv, iter, std::vector<zd>(Y.begin() + (size_t)wi * ncv, Y.begin() + (size_t)(wi + 1) * ncv),
+                          th[wi]};
+                return;
+            }
+            // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies
+            // the ncv - 1 unwanted Ritz values as exact shifts; a larger basis
+            // keeps half of its vectors (thick restart) and shifts out the rest.
+            D.kind = D_RST;
+            Rst& r = D.r;
+            r.p = p;
+            r.keep = ncv <= 3 ? 1 : ncv / 2;
+            const int keep = r.keep;
+            std::vector<int> order = wanted_order(th, t.which);
+            r.Q.assign((size_t)ncv * ncv, zd(0, 0));
+            for (int i = 0; i < ncv; ++i) r.Q[(size_t)i + (size_t)i * ncv] = 1.0;
+            for (int q = keep; q < ncv; ++q) shifted_qr_step(ncv, x.H, th[order[q]], r.Q);
+            r.hk = h(keep, keep - 1);
+            r.qm = r.Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
+            r.Hk.assign((size_t)keep * keep, zd(0, 0));
+            for (int c = 0; c < keep; ++c)
+                for (int rr = 0; rr < keep; ++rr) r.Hk[(size_t)rr + (size_t)c * keep] = (rr <= c + 1) ? h(rr, c) : zd(0, 0);
+        });
+        std::vector<Fin> fins;
+        std::vector<Rst> rsts;
+        for (int p = 0; p < P; ++p) {
+            Dec& D = dec[p];
+            KState& x = st[p];
+            IramTask& t = *x.task;
+            if (D.kind == D_FIN) {
+                fins.push_back(std::move(D.f));
+            } else if (D.kind == D_RST) {
+                rsts.push_back(std::move(D.r));
+            } else if (D.kind == D_FAIL) {
+                t.out.restarts = x.iter;
+                t.out.info = -14;
+                x.done = true;
+            } else if (D.kind == D_STOP) {
+                auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
+                const int stop = D.stop;
                 // invariant subspace found early: continue from a random vector
                 // orthogonal to it, as ARPACK does (the extension does not count
                 // as a restart)
@@ -427,45 +647,9 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 std::vector<zd> th, Y;
                 hessenberg_eig(m, Hs, th, Y);
                 const int wi = wanted_index(th, t.which);
-                fins.push_back(Fin{p, m, iter, std::vector<zd>(Y.begin() + (size_t)wi * m,
-                                                                Y.begin() + (size_t)(wi + 1) * m), th[wi]});
-                continue;
+                fins.push_back(Fin{p, m, x.iter, std::vector<zd>(Y.begin() + (size_t)wi * m,
+                                                                  Y.begin() + (size_t)(wi + 1) * m), th[wi]});
             }
-            x.beta = bh[(size_t)p * ncv + ncv - 1];
-            std::vector<zd> th, Y;
-            hessenberg_eig(ncv, x.H, th, Y);
-            const int wi = wanted_index(th, t.which);
-            const double est = x.beta * std::abs(Y[(size_t)(ncv - 1) + (size_t)wi * ncv]);
-            const double tol_eff = t.tol > 0 ? t.tol : eps;
-            const bool converged = est <= tol_eff * std::max(eps23, std::abs(th[wi]));
-            if (converged || iter >= t.max_restarts) {
-                if (!converged) {
-                    t.out.restarts = iter;
-                    t.out.info = -14;
-                    x.done = true;
-                    continue;
-                }
-                fins.push_back(Fin{p, ncv, iter, std::vector<zd>(Y.begin() + (size_t)wi * ncv,
-                                                                  Y.begin() + (size_t)(wi + 1) * ncv), th[wi]});
-                continue;
-            }
-            // ---- implicit restart.  ARPACK keeps nev (= 1) vectors and applies
-            // the ncv - 1 unwanted Ritz values as exact shifts; a larger basis
-            // keeps half of its vectors (thick restart) and shifts out the rest.
-            Rst r;
-            r.p = p;
-            r.keep = ncv <= 3 ? 1 : ncv / 2;
-            const int keep = r.keep;
-            std::vector<int> order = wanted_order(th, t.which);
-            r.Q.assign((size_t)ncv * ncv, zd(0, 0));
-            for (int i = 0; i < ncv; ++i) r.Q[(size_t)i + (size_t)i * ncv] = 1.0;
-            for (int q = keep; q < ncv; ++q) shifted_qr_step(ncv, x.H, th[order[q]], r.Q);
-            r.hk = h(keep, keep - 1);
-            r.qm = r.Q[(size_t)(ncv - 1) + (size_t)(keep - 1) * ncv];
-            r.Hk.assign((size_t)keep * keep, zd(0, 0));
-            for (int c = 0; c < keep; ++c)
-                for (int rr = 0; rr < keep; ++rr) r.Hk[(size_t)rr + (size_t)c * keep] = (rr <= c + 1) ? h(rr, c) : zd(0, 0);
-            rsts.push_back(std::move(r));
         }
         if (fins.empty() && rsts.empty()) continue;
         // coefficients of all of them in one upload
@@ -481,22 +665,26 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 hc.push_back(make_double2(rsts[q].Q[e].real(), rsts[q].Q[e].imag()));
         }
         if (coefs.n < hc.size()) coefs.resize(hc.size());
-        h2d(coefs.ptr, hc.data(), hc.size() * sizeof(double2));
+        // stream-ordered upload; the host array is rewritten only after a later
+        // host wait (the norms below)
+        hcoef.reserve(hc.size());
+        std::copy(hc.begin(), hc.end(), hcoef.ptr);
+        NC_HIP(hipMemcpyAsync(coefs.ptr, hcoef.ptr, hc.size() * sizeof(double2), hipMemcpyHostToDevice, s));
         // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
-        GemmPlan g;
-        g.M = g.N = 1;
+        GemmPlan gp;
+        gp.M = gp.N = 1;
         for (size_t q = 0; q < fins.size(); ++q) {
             KState& x = st[fins[q].p];
-            g.begin_batch(x.task->result, x.n, (int)x.n, 1);
-            g.add_seg(x.V.ptr, x.n, coefs.ptr + off_f[q], fins[q].m, fins[q].m);
+            gp.begin_batch(x.task->result, x.n, (int)x.n, 1);
+            gp.add_seg(x.V.ptr, x.n, coefs.ptr + off_f[q], fins[q].m, fins[q].m);
         }
         for (size_t q = 0; q < rsts.size(); ++q) {
             KState& x = st[rsts[q].p];
             if (x.Vk.n < (size_t)x.n * (ncv + 1)) x.Vk.resize((size_t)x.n * (ncv + 1));
-            g.begin_batch(x.Vk.ptr, x.n, (int)x.n, rsts[q].keep + 1);
-            g.add_seg(x.V.ptr, x.n, coefs.ptr + off_r[q], ncv, ncv);
+            gp.begin_batch(x.Vk.ptr, x.n, (int)x.n, rsts[q].keep + 1);
+            gp.add_seg(x.V.ptr, x.n, coefs.ptr + off_r[q], ncv, ncv);
         }
-        zgemm_run(g, s);
+        zgemm_run(gp, s);
         // f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the basis
         {
             std::vector<double2*> y;
@@ -607,6 +795,15 @@ IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_re
     t[0].result = result;
     iram_batch(t, BatchApply::each(t), s);
     return t[0].out;
+}
+
+bool& krylov_graphs() {
+    // NC_KRYLOV_GRAPHS=0 in the environment sets the default to direct launches
+    static bool on = [] {
+        const char* e = std::getenv("NC_KRYLOV_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 BatchApply BatchApply::each(std::vector<IramTask> const& tasks) {

@@ -83,11 +83,13 @@ struct MixBatchDesc {
     const CT* W;
     int64_t rs;
     int ndst, d;
+    const int* live;  // null, or a device flag: 0 = skip this chain
 };
 
 template <typename CT, int D>
 __global__ __launch_bounds__(256) void mix_batch_kernel(const MixBatchDesc<CT>* __restrict__ descs) {
     const MixBatchDesc<CT> md = descs[blockIdx.z];
+    if (md.live && *md.live == 0) return;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int dst = blockIdx.y;
     if (dst >= md.ndst || e >= md.rs) return;
@@ -109,10 +111,10 @@ __global__ __launch_bounds__(256) void mix_batch_kernel(const MixBatchDesc<CT>* 
 }
 
 template <typename CT>
-static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&, int), hipStream_t s) {
-    if (items.empty()) return;
-    const int d = items[0].op->d;
-    std::vector<MixBatchDesc<CT>> ds;
+static bool mix_descs(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&, int),
+                      std::vector<MixBatchDesc<CT>>& ds, dim3& grid, int& d) {
+    if (items.empty()) return false;
+    d = items[0].op->d;
     int64_t maxrs = 0;
     int maxdst = 0;
     for (auto const& it : items) {
@@ -127,28 +129,73 @@ static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(Mp
         m.rs = it.rs;
         m.ndst = (int)(it.src_is_from ? it.op->tos.size() : it.op->froms.size());
         m.d = d;
+        m.live = it.live;
         ds.push_back(m);
         maxrs = std::max(maxrs, it.rs);
         maxdst = std::max(maxdst, m.ndst);
     }
-    if (maxrs == 0 || maxdst == 0) return;
-    int slot;
-    auto* dd = (const MixBatchDesc<CT>*)stage_descriptors(ds.data(), ds.size() * sizeof(MixBatchDesc<CT>), s, &slot);
-    dim3 grid((unsigned)((maxrs + 255) / 256), (unsigned)maxdst, (unsigned)ds.size());
+    if (maxrs == 0 || maxdst == 0) return false;
+    if (d < 2 || d > 4) throw Error("mix_blocks_batch: physical dimension above 4 is not supported", -4);
+    grid = dim3((unsigned)((maxrs + 255) / 256), (unsigned)maxdst, (unsigned)ds.size());
+    return true;
+}
+
+template <typename CT>
+static void mix_launch(const MixBatchDesc<CT>* dd, dim3 grid, int d, hipStream_t s) {
     switch (d) {
         case 2: hipLaunchKernelGGL((mix_batch_kernel<CT, 2>), grid, dim3(256), 0, s, dd); break;
         case 3: hipLaunchKernelGGL((mix_batch_kernel<CT, 3>), grid, dim3(256), 0, s, dd); break;
-        case 4: hipLaunchKernelGGL((mix_batch_kernel<CT, 4>), grid, dim3(256), 0, s, dd); break;
-        default: throw Error("mix_blocks_batch: physical dimension above 4 is not supported", -4);
+        default: hipLaunchKernelGGL((mix_batch_kernel<CT, 4>), grid, dim3(256), 0, s, dd); break;
     }
     NC_HIP(hipGetLastError());
+}
+
+template <typename CT>
+static void mix_batch_t(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoSite const&, int), hipStream_t s) {
+    std::vector<MixBatchDesc<CT>> ds;
+    dim3 grid;
+    int d;
+    if (!mix_descs(items, W, ds, grid, d)) return;
+    int slot;
+    auto* dd = (const MixBatchDesc<CT>*)stage_descriptors(ds.data(), ds.size() * sizeof(MixBatchDesc<CT>), s, &slot);
+    mix_launch(dd, grid, d, s);
     release_descriptors(slot, s);
+}
+
+template <typename CT>
+struct PreparedMix final : PreparedLaunch {
+    DevBuf<MixBatchDesc<CT>> desc;
+    dim3 grid;
+    int d = 0;
+    bool any = false;
+    void launch(hipStream_t s) const override {
+        if (any) mix_launch(desc.ptr, grid, d, s);
+    }
+};
+
+template <typename CT>
+static std::unique_ptr<PreparedLaunch> prepare_mix_t(std::vector<MixItem<CT>> const& items,
+                                                     const CT* (*W)(MpoSite const&, int)) {
+    std::unique_ptr<PreparedMix<CT>> p(new PreparedMix<CT>());
+    std::vector<MixBatchDesc<CT>> ds;
+    p->any = mix_descs(items, W, ds, p->grid, p->d);
+    if (p->any) {
+        p->desc.resize(ds.size());
+        h2d(p->desc.ptr, ds.data(), ds.size() * sizeof(MixBatchDesc<CT>));
+    }
+    return p;
 }
 
 static const double2* mixw64(MpoSite const& op, int dir) { return op.mix_W[dir].ptr; }
 static const float2* mixw32(MpoSite const& op, int dir) { return op.mix_W32[dir].ptr; }
 void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s) { mix_batch_t(items, mixw64, s); }
 void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s) { mix_batch_t(items, mixw32, s); }
+std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<double2>> const& items) {
+    return prepare_mix_t(items, mixw64);
+}
+std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<float2>> const& items) {
+    return prepare_mix_t(items, mixw32);
+}
 
 template <typename A, typename B>
 struct ConvDesc {
@@ -610,6 +657,79 @@ void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::
     hipLaunchKernelGGL(axpby_batch_kernel, dim3(g, (unsigned)y.size()), dim3(256), 0, s, dd);
     NC_HIP(hipGetLastError());
     release_descriptors(slot, s);
+}
+
+// ---------------------------------------------------------------------------
+// Several reductions in one launch: blockIdx.y selects the item, each block
+// writes one partial sum, a second kernel sums the partials of every item in a
+// fixed order (deterministic), and the host reads all results at once.
+// ---------------------------------------------------------------------------
+struct RedItemDev {
+    const double2* x;
+    const double2* y;
+    int64_t n;     // elements (dot) or b (trace)
+    int64_t c;     // trace: third extent
+    int kind, _pad;
+};
+
+__global__ __launch_bounds__(RED_THREADS) void reduce_batch_kernel(const RedItemDev* __restrict__ items,
+                                                                   double2* __restrict__ partial) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    const RedItemDev it = items[blockIdx.y];
+    double re = 0, im = 0;
+    if (it.kind == 0) {  // sum conj(x) y
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < it.n; e += (int64_t)gridDim.x * blockDim.x) {
+            const double2 a = it.x[e], b = it.y[e];
+            re += a.x * b.x + a.y * b.y;
+            im += a.x * b.y - a.y * b.x;
+        }
+    } else {  // sum_{i,j,k} x(i,j,k) y(j,i,k), b x b x c
+        const int64_t b = it.n, total = it.n * it.n * it.c;
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t i = e % b, j = (e / b) % b, k = e / (b * b);
+            const double2 a = it.x[e], v = it.y[j + i * b + k * b * b];
+            re += a.x * v.x - a.y * v.y;
+            im += a.x * v.y + a.y * v.x;
+        }
+    }
+    re = warp_sum(re);
+    im = warp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+        partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
+std::vector<zd> reduce_batch(std::vector<RedItem> const& items, hipStream_t s) {
+    std::vector<zd> out(items.size(), zd(0, 0));
+    if (items.empty()) return out;
+    std::vector<RedItemDev> dv(items.size());
+    int64_t maxn = 1;
+    for (size_t q = 0; q < items.size(); ++q) {
+        RedItem const& it = items[q];
+        dv[q] = RedItemDev{it.x, it.y, it.n, it.c, it.trace ? 1 : 0, 0};
+        maxn = std::max(maxn, it.trace ? it.n * it.n * it.c : it.n);
+    }
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((maxn + RED_THREADS - 1) / RED_THREADS,
+                                                                std::max<int64_t>(1, 4096 / (int64_t)items.size())));
+    static thread_local DevBuf<double2> partial, res;
+    static thread_local std::vector<double2> host;
+    if (partial.n < (size_t)nb * items.size()) partial.resize((size_t)nb * items.size());
+    if (res.n < items.size()) res.resize(items.size());
+    int slot;
+    auto* d = (const RedItemDev*)stage_descriptors(dv.data(), dv.size() * sizeof(RedItemDev), s, &slot);
+    hipLaunchKernelGGL(reduce_batch_kernel, dim3(nb, (unsigned)items.size()), dim3(RED_THREADS), 0, s, d, partial.ptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)items.size()), dim3(RED_THREADS), 0, s, partial.ptr, nb,
+                       res.ptr);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+    host.resize(items.size());
+    d2h(host.data(), res.ptr, items.size() * sizeof(double2));
+    for (size_t q = 0; q < items.size(); ++q) out[q] = zd(host[q].x, host[q].y);
+    return out;
 }
 
 }  // namespace nc

@@ -52,6 +52,9 @@ struct SitePrep {
     Projector none;
     Projector const* proj = nullptr;
     double t0 = 0;
+    // expectation value of the result, when a batch already computed it
+    bool have_ex = false;
+    zd ex{0, 0};
 };
 
 struct Chain {
@@ -119,6 +122,8 @@ struct ChainBatch {
     int64_t lockstep_rounds = 0;
     explicit ChainBatch(std::vector<Chain*> cs);
     double t_moves_ms = 0, t_solves_ms = 0;
+    // parts of the site solves: per-chain preparation, the batched solve, per-chain post-checks
+    double t_prepare_ms = 0, t_batched_ms = 0, t_finish_ms = 0, t_krylov_ms = 0;
     void optimize_sites(std::vector<Chain*> const& active);
     void move_all(std::vector<Chain*> const& cs, int dir);
     void env_left_batch(std::vector<Chain*> const& cs, hipStream_t s);

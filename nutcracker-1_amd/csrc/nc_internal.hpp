@@ -105,7 +105,8 @@ struct GemmBatch {
     double2* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
-    int32_t M, N;  // this batch's output size; 0 = the plan's M, N
+    int32_t M, N;             // this batch's output size; 0 = the plan's M, N
+    const int* live = nullptr;  // device flag read at launch: 0 = skip this batch
 };
 
 enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
@@ -129,8 +130,8 @@ struct GemmPlan {
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
     void clear() { batches.clear(); segs.clear(); }
-    void begin_batch(double2* C, int64_t ldc, int M_ = 0, int N_ = 0) {
-        batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_});
+    void begin_batch(double2* C, int64_t ldc, int M_ = 0, int N_ = 0, const int* live = nullptr) {
+        batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_, live});
     }
     void add_seg(const double2* A, int64_t lda, const double2* B, int64_t ldb, int K) {
         segs.push_back(GemmSeg{A, B, lda, ldb, K, 0});
@@ -153,15 +154,16 @@ struct CGemmBatch {
     float2* C;
     int64_t ldc;
     int32_t seg_begin, seg_end;
-    int32_t M, N;  // 0 = the plan's M, N
+    int32_t M, N;               // 0 = the plan's M, N
+    const int* live = nullptr;  // device flag read at launch: 0 = skip this batch
 };
 struct CGemmPlan {
     std::vector<CGemmBatch> batches;
     std::vector<CGemmSeg> segs;
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
-    void begin_batch(float2* C, int64_t ldc, int M_ = 0, int N_ = 0) {
-        batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_});
+    void begin_batch(float2* C, int64_t ldc, int M_ = 0, int N_ = 0, const int* live = nullptr) {
+        batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_, live});
     }
     void add_seg(const float2* A, int64_t lda, const float2* B, int64_t ldb, int K) {
         segs.push_back(CGemmSeg{A, B, lda, ldb, K, 0});
@@ -169,6 +171,17 @@ struct CGemmPlan {
     }
 };
 void cgemm_run(CGemmPlan const& plan, hipStream_t s);
+
+// A launch whose descriptors were uploaded once into device memory of its own:
+// launch() does no host work beyond the kernel launches, so it can run many
+// times (and be recorded by a stream capture).  The pointers it was built with
+// must stay valid while it is used.
+struct PreparedLaunch {
+    virtual ~PreparedLaunch() = default;
+    virtual void launch(hipStream_t s) const = 0;
+};
+std::unique_ptr<PreparedLaunch> prepare_gemm(GemmPlan const& plan);
+std::unique_ptr<PreparedLaunch> prepare_gemm(CGemmPlan const& plan);
 
 // ----------------------------------------------------------------------------
 // Sparse MPO block list, prepared for grouped contraction.
@@ -203,9 +216,12 @@ struct MixItem {
     CT* Q;
     int64_t rs;
     bool src_is_from = true;  // false: the right-environment direction (src = to)
+    const int* live = nullptr;  // device flag read at launch: 0 = skip this chain
 };
 void mix_blocks_batch(std::vector<MixItem<double2>> const& items, hipStream_t s);
 void mix_blocks_batch(std::vector<MixItem<float2>> const& items, hipStream_t s);
+std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<double2>> const& items);
+std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<float2>> const& items);
 // y_q = a_q x_q + b_q y_q for several vectors in one launch (x_q may be null;
 // b_q = 0 does not read y_q)
 void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::vector<const double2*> const& x,
@@ -383,8 +399,41 @@ struct BatchApply {
     }
     // one operator application after another
     static BatchApply each(std::vector<IramTask> const& tasks);
+
+    // Persistent form (optional; used when `launch` is set).  Problem p's operator
+    // reads the fixed vector in[p] and writes out[p] (complex64 when `single`,
+    // complex128 otherwise).  bind(live, s) prepares launches for every problem,
+    // each skipped while the device flag live[p] is 0; launch(s) runs them for
+    // all problems with no host work, so the Krylov solver can record whole
+    // Arnoldi extensions in a graph.
+    std::vector<void*> in, out;
+    bool single = false;
+    std::function<void(const int* live, hipStream_t)> bind;
+    std::function<void(hipStream_t)> launch;
+    bool persistent() const { return (bool)launch; }
 };
+// Launch options of the Krylov solver: record each Arnoldi extension of a
+// persistent batch in a HIP graph and replay it (default on; the environment
+// variable NC_KRYLOV_GRAPHS=0 makes direct launches the default).
+bool& krylov_graphs();
 void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s);
+
+// Several reductions with one launch pair and one host read: an item is
+// sum conj(x) y over n elements, or (trace) sum_{i,j,k} x(i,j,k) y(j,i,k) with
+// x, y of shape n x n x c.  Deterministic (fixed summation order).
+struct RedItem {
+    const double2* x;
+    const double2* y;
+    int64_t n;
+    int64_t c = 1;
+    bool trace = false;
+};
+std::vector<zd> reduce_batch(std::vector<RedItem> const& items, hipStream_t s);
+
+// Runs fn(i) for i in [0, n) on a small pool of host threads (the caller
+// included; serially for small n or when another parallel region is running).
+// fn must not call HIP.
+void host_parallel_for(int n, std::function<void(int)> const& fn);
 
 // Small dense host eigen-problems.
 // Eigenvalues and right eigenvectors of a general complex upper Hessenberg matrix.
@@ -427,8 +476,9 @@ struct OptimizeOut {
     double normal = 0;
 };
 // Site solves of several independent chains in lockstep (grouped launches).
+// krylov_ms, when given, accumulates the wall time of the Krylov iterations.
 void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::vector<Workspace*> const& wss,
-                            std::vector<OptimizeOut>& outs, hipStream_t s);
+                            std::vector<OptimizeOut>& outs, hipStream_t s, double* krylov_ms = nullptr);
 OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
 
 // ----------------------------------------------------------------------------

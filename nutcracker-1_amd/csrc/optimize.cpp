@@ -4,6 +4,8 @@
 // 1266-1339, 1513-1664).
 #include "nc_internal.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -379,8 +381,65 @@ namespace nc {
 // the orthogonalisation of all chains shares its launches too (iram_batch).
 // Sites with projectors or a dense-solve size (orth <= 4) are solved one by one.
 // ----------------------------------------------------------------------------
+namespace {
+// Grouped H_eff matvec plans of several chains, y_q = H_q x_q (complex128):
+// step A (X_a L_f), the block mixing, step C (sum_t R_t Q_t).  live[q], when
+// given, gates chain q at launch time.
+void heff_plans64(std::vector<OptimizeArgs const*> const& as, std::vector<Workspace*> const& ws,
+                  std::vector<const double2*> const& xs, std::vector<double2*> const& ys, const int* live,
+                  GemmPlan& A, std::vector<MixItem<double2>>& mix, GemmPlan& Cp) {
+    A.M = A.N = Cp.M = Cp.N = 1;
+    for (size_t q = 0; q < as.size(); ++q) {
+        OptimizeArgs const& a = *as[q];
+        Workspace& w = *ws[q];
+        const int64_t rs = (int64_t)a.br * a.bl;
+        MpoSite const& op = *a.op;
+        const int* lv = live ? live + q : nullptr;
+        w.ensure(rs * a.d * op.froms.size(), rs * a.d * op.tos.size());
+        for (size_t fi = 0; fi < op.froms.size(); ++fi)
+            for (int aa = 0; aa < op.d; ++aa) {
+                A.begin_batch(w.P.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl, lv);
+                A.add_seg(xs[q] + aa * rs, a.br, a.L + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl, a.bl);
+            }
+        MixItem<double2> mi{&op, w.P.ptr, w.Q.ptr, rs};
+        mi.live = lv;
+        mix.push_back(mi);
+        Cp.begin_batch(ys[q], a.br, a.br, a.bl * op.d, lv);
+        for (size_t ti = 0; ti < op.tos.size(); ++ti)
+            Cp.add_seg(a.R + (int64_t)op.tos[ti] * a.br * a.br, a.br, w.Q.ptr + (int64_t)ti * op.d * rs, a.br, a.br);
+    }
+}
+
+// the same with complex64 operands (L32, R32 and the complex64 workspaces)
+void heff_plans32(std::vector<OptimizeArgs const*> const& as, std::vector<Workspace*> const& ws,
+                  std::vector<const float2*> const& xs, std::vector<float2*> const& ys, const int* live,
+                  CGemmPlan& A, std::vector<MixItem<float2>>& mix, CGemmPlan& Cp) {
+    A.M = A.N = Cp.M = Cp.N = 1;
+    for (size_t q = 0; q < as.size(); ++q) {
+        OptimizeArgs const& a = *as[q];
+        Workspace& w = *ws[q];
+        const int64_t rs = (int64_t)a.br * a.bl;
+        MpoSite const& op = *a.op;
+        const int* lv = live ? live + q : nullptr;
+        w.ensure(rs * a.d * op.froms.size(), rs * a.d * op.tos.size(), (float2*)nullptr);
+        for (size_t fi = 0; fi < op.froms.size(); ++fi)
+            for (int aa = 0; aa < op.d; ++aa) {
+                A.begin_batch(w.P32.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl, lv);
+                A.add_seg(xs[q] + aa * rs, a.br, a.L32 + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl, a.bl);
+            }
+        MixItem<float2> mi{&op, w.P32.ptr, w.Q32.ptr, rs};
+        mi.live = lv;
+        mix.push_back(mi);
+        Cp.begin_batch(ys[q], a.br, a.br, a.bl * op.d, lv);
+        for (size_t ti = 0; ti < op.tos.size(); ++ti)
+            Cp.add_seg(a.R32 + (int64_t)op.tos[ti] * a.br * a.br, a.br, w.Q32.ptr + (int64_t)ti * op.d * rs, a.br,
+                       a.br);
+    }
+}
+}  // namespace
+
 void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::vector<Workspace*> const& wss,
-                            std::vector<OptimizeOut>& outs, hipStream_t s) {
+                            std::vector<OptimizeOut>& outs, hipStream_t s, double* krylov_ms) {
     const size_t C = args.size();
     outs.assign(C, OptimizeOut());
     std::vector<size_t> idx;  // the chains solved together
@@ -425,77 +484,79 @@ void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::v
             t.guess = a.guess;
             t.result = a.result;
         }
+        // Persistent grouped operator: every chain's matvec reads a fixed input
+        // vector and writes a fixed output vector, so the grouped launches are
+        // prepared once for the whole solve and gated per chain by the solver's
+        // live flags.
         BatchApply apply;
-        apply.fn = [&](std::vector<int> const& who, std::vector<const double2*> const& xs,
-                       std::vector<double2*> const& ys, hipStream_t st) {
+        std::vector<ZBuf> xin(mixed ? 0 : grp.size()), yout(mixed ? 0 : grp.size());
+        std::vector<std::unique_ptr<PreparedLaunch>> prepared;
+        apply.single = mixed;
+        for (size_t q = 0; q < grp.size(); ++q) {
+            if (mixed) {
+                apply.in.push_back(op32[q]->x32.ptr);
+                apply.out.push_back(op32[q]->y32.ptr);
+            } else {
+                const int64_t nfull = op64[q]->n;
+                xin[q].resize((size_t)nfull);
+                yout[q].resize((size_t)nfull);
+                apply.in.push_back(xin[q].ptr);
+                apply.out.push_back(yout[q].ptr);
+            }
+        }
+        std::vector<OptimizeArgs const*> gargs;
+        std::vector<Workspace*> gws;
+        for (size_t q = 0; q < grp.size(); ++q) {
+            gargs.push_back(args[grp[q]]);
+            gws.push_back(wss[grp[q]]);
+        }
+        apply.bind = [&](const int* live, hipStream_t) {
             if (!mixed) {
                 GemmPlan A, Cp;
-                A.M = A.N = Cp.M = Cp.N = 1;
                 std::vector<MixItem<double2>> mix;
-                for (size_t q = 0; q < who.size(); ++q) {
-                    OptimizeArgs const& a = *args[grp[who[q]]];
-                    Workspace& w = *wss[grp[who[q]]];
-                    const int64_t rs = (int64_t)a.br * a.bl;
-                    MpoSite const& op = *a.op;
-                    for (size_t fi = 0; fi < op.froms.size(); ++fi)
-                        for (int aa = 0; aa < op.d; ++aa) {
-                            A.begin_batch(w.P.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl);
-                            A.add_seg(xs[q] + aa * rs, a.br, a.L + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl, a.bl);
-                        }
-                    mix.push_back(MixItem<double2>{&op, w.P.ptr, w.Q.ptr, rs});
-                    Cp.begin_batch(ys[q], a.br, a.br, a.bl * op.d);
-                    for (size_t ti = 0; ti < op.tos.size(); ++ti)
-                        Cp.add_seg(a.R + (int64_t)op.tos[ti] * a.br * a.br, a.br, w.Q.ptr + (int64_t)ti * op.d * rs, a.br,
-                                   a.br);
+                std::vector<const double2*> xs;
+                std::vector<double2*> ys;
+                for (size_t q = 0; q < grp.size(); ++q) {
+                    xs.push_back(xin[q].ptr);
+                    ys.push_back(yout[q].ptr);
                 }
-                zgemm_run(A, st);
-                mix_blocks_batch(mix, st);
-                zgemm_run(Cp, st);
+                heff_plans64(gargs, gws, xs, ys, live, A, mix, Cp);
+                prepared.push_back(prepare_gemm(A));
+                prepared.push_back(prepare_mix(mix));
+                prepared.push_back(prepare_gemm(Cp));
             } else {
                 CGemmPlan A, Cp;
-                A.M = A.N = Cp.M = Cp.N = 1;
                 std::vector<MixItem<float2>> mix;
-                std::vector<const double2*> cin;
-                std::vector<float2*> cout32;
-                std::vector<const float2*> rin32;
-                std::vector<double2*> rout;
-                std::vector<int64_t> ns;
-                for (size_t q = 0; q < who.size(); ++q) {
-                    OptimizeArgs const& a = *args[grp[who[q]]];
-                    Workspace& w = *wss[grp[who[q]]];
-                    Heff32Operator& o32 = *op32[who[q]];
-                    const int64_t rs = (int64_t)a.br * a.bl;
-                    MpoSite const& op = *a.op;
-                    cin.push_back(xs[q]);
-                    cout32.push_back(o32.x32.ptr);
-                    rin32.push_back(o32.y32.ptr);
-                    rout.push_back(ys[q]);
-                    ns.push_back(rs * op.d);
-                    for (size_t fi = 0; fi < op.froms.size(); ++fi)
-                        for (int aa = 0; aa < op.d; ++aa) {
-                            A.begin_batch(w.P32.ptr + ((int64_t)fi * op.d + aa) * rs, a.br, a.br, a.bl);
-                            A.add_seg(o32.x32.ptr + aa * rs, a.br, a.L32 + (int64_t)op.froms[fi] * a.bl * a.bl, a.bl,
-                                      a.bl);
-                        }
-                    mix.push_back(MixItem<float2>{&op, w.P32.ptr, w.Q32.ptr, rs});
-                    Cp.begin_batch(o32.y32.ptr, a.br, a.br, a.bl * op.d);
-                    for (size_t ti = 0; ti < op.tos.size(); ++ti)
-                        Cp.add_seg(a.R32 + (int64_t)op.tos[ti] * a.br * a.br, a.br,
-                                   w.Q32.ptr + (int64_t)ti * op.d * rs, a.br, a.br);
+                std::vector<const float2*> xs;
+                std::vector<float2*> ys;
+                for (size_t q = 0; q < grp.size(); ++q) {
+                    xs.push_back(op32[q]->x32.ptr);
+                    ys.push_back(op32[q]->y32.ptr);
                 }
-                to_c64_batch(cin, cout32, ns, st);
-                cgemm_run(A, st);
-                mix_blocks_batch(mix, st);
-                cgemm_run(Cp, st);
-                to_c128_batch(rin32, rout, ns, st);
+                heff_plans32(gargs, gws, xs, ys, live, A, mix, Cp);
+                prepared.push_back(prepare_gemm(A));
+                prepared.push_back(prepare_mix(mix));
+                prepared.push_back(prepare_gemm(Cp));
             }
         };
+        apply.launch = [&](hipStream_t st) {
+            for (auto const& pl : prepared) pl->launch(st);
+        };
+        const auto k0 = std::chrono::steady_clock::now();
         iram_batch(tasks, apply, s);
+        if (krylov_ms)
+            *krylov_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - k0).count();
+        // Post-processing of the whole group: the mixed form's FP64 Rayleigh
+        // quotient as one grouped FP64 matvec, and every dot product and norm in
+        // one batched reduction (one host read).
+        std::vector<size_t> rq, nrm;  // positions in grp: Rayleigh quotient; norm of the result
+        std::vector<IramResult> rs(grp.size());
         for (size_t q = 0; q < grp.size(); ++q) {
             const size_t c = grp[q];
             OptimizeArgs const& a = *args[c];
             const int64_t nfull = (int64_t)a.d * a.bl * a.br;
-            IramResult r = tasks[q].out;
+            IramResult& r = rs[q];
+            r = tasks[q].out;
             OptimizeOut& o = outs[c];
             if (r.info == -14) {  // the retries of optimize_site, one chain at a time
                 o = optimize_site(a, *wss[c], s);
@@ -514,20 +575,57 @@ void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::v
                     o.matvecs += r2.matvecs;
                     if (r2.info == 0) r = r2;
                 } else {
-                    ZBuf y((size_t)nfull);
-                    op64[q]->apply(a.result, y.ptr, s);
+                    rq.push_back(q);
                     ++o.matvecs;
-                    zd num;
-                    zdotc_multi(a.result, nfull, 1, y.ptr, nfull, &num, s);
-                    const double nn = dznrm2(a.result, nfull, s);
-                    r.eigenvalue = num / (nn * nn);
                 }
             }
-            o.info = r.info;
-            o.iterations = r.restarts;
-            o.eigenvalue = r.eigenvalue;
             if (r.info != 0) dzero(a.result, (size_t)nfull * sizeof(double2));
-            o.normal = dznrm2(a.result, nfull, s);
+            else nrm.push_back(q);
+        }
+        std::vector<ZBuf> hy(rq.size());
+        if (!rq.empty()) {
+            std::vector<OptimizeArgs const*> ra;
+            std::vector<Workspace*> rw;
+            std::vector<const double2*> xs;
+            std::vector<double2*> ys;
+            for (size_t k = 0; k < rq.size(); ++k) {
+                OptimizeArgs const& a = *args[grp[rq[k]]];
+                hy[k].resize((size_t)a.d * a.bl * a.br);
+                ra.push_back(&a);
+                rw.push_back(wss[grp[rq[k]]]);
+                xs.push_back(a.result);
+                ys.push_back(hy[k].ptr);
+            }
+            GemmPlan A, Cp;
+            std::vector<MixItem<double2>> mix;
+            heff_plans64(ra, rw, xs, ys, nullptr, A, mix, Cp);
+            zgemm_run(A, s);
+            mix_blocks_batch(mix, s);
+            zgemm_run(Cp, s);
+        }
+        std::vector<RedItem> items;
+        for (size_t k = 0; k < rq.size(); ++k) {
+            OptimizeArgs const& a = *args[grp[rq[k]]];
+            items.push_back(RedItem{a.result, hy[k].ptr, (int64_t)a.d * a.bl * a.br});
+        }
+        for (size_t q : nrm) {
+            OptimizeArgs const& a = *args[grp[q]];
+            items.push_back(RedItem{a.result, a.result, (int64_t)a.d * a.bl * a.br});
+        }
+        const std::vector<zd> red = reduce_batch(items, s);
+        std::vector<double> norm_of(grp.size(), 0.0);
+        for (size_t k = 0; k < nrm.size(); ++k) norm_of[nrm[k]] = std::sqrt(std::max(0.0, red[rq.size() + k].real()));
+        for (size_t k = 0; k < rq.size(); ++k) {
+            const double nn = norm_of[rq[k]];
+            rs[rq[k]].eigenvalue = red[k] / (nn * nn);
+        }
+        for (size_t q = 0; q < grp.size(); ++q) {
+            if (tasks[q].out.info == -14) continue;
+            OptimizeOut& o = outs[grp[q]];
+            o.info = rs[q].info;
+            o.iterations = rs[q].restarts;
+            o.eigenvalue = rs[q].eigenvalue;
+            o.normal = norm_of[q];
         }
     }
 }

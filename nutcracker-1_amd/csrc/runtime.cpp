@@ -2,8 +2,13 @@
 // allocator and host <-> device copies.
 #include "nc_internal.hpp"
 
+#include <atomic>
+#include <condition_variable>
+#include <cstdlib>
+#include <exception>
 #include <map>
 #include <mutex>
+#include <thread>
 
 namespace nc {
 
@@ -141,6 +146,106 @@ int GemmPlan::max_k() const {
         best = std::max(best, k);
     }
     return best;
+}
+
+// ---------------------------------------------------------------------------
+// Host thread pool for independent per-problem host work (small eigen-problems
+// of a batch).  Workers = min(hardware threads, OMP_NUM_THREADS, 16) - 1; the
+// calling thread takes part.
+// ---------------------------------------------------------------------------
+namespace {
+struct HostJob {
+    std::function<void(int)> const* fn = nullptr;
+    int n = 0;
+    std::atomic<int> next{0}, finished{0};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::exception_ptr err;
+    // claims indices until none is left; a worker that arrives late finds none
+    void work() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) break;
+            try {
+                (*fn)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu);
+                if (!err) err = std::current_exception();
+            }
+            if (finished.fetch_add(1) + 1 == n) {
+                std::lock_guard<std::mutex> g(mu);
+                cv.notify_all();
+            }
+        }
+    }
+};
+struct HostPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::shared_ptr<HostJob> job;
+    uint64_t gen = 0;
+    explicit HostPool(int workers) {
+        for (int w = 0; w < workers; ++w) std::thread([this] { loop(); }).detach();
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::shared_ptr<HostJob> j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return gen != seen; });
+                seen = gen;
+                j = job;
+            }
+            if (j) j->work();
+        }
+    }
+    void run(int n, std::function<void(int)> const& f) {
+        auto j = std::make_shared<HostJob>();
+        j->fn = &f;
+        j->n = n;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            job = j;
+            ++gen;
+        }
+        cv.notify_all();
+        j->work();
+        {
+            std::unique_lock<std::mutex> lk(j->mu);
+            j->cv.wait(lk, [&] { return j->finished.load() == n; });
+        }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            job.reset();
+        }
+        if (j->err) std::rethrow_exception(j->err);
+    }
+};
+HostPool* host_pool() {
+    static HostPool* p = [] {
+        int hw = (int)std::thread::hardware_concurrency();
+        if (hw <= 0) hw = 1;
+        if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+            const int v = std::atoi(e);
+            if (v > 0) hw = std::min(hw, v);
+        }
+        hw = std::min(hw, 16);
+        return hw > 1 ? new HostPool(hw - 1) : nullptr;
+    }();
+    return p;
+}
+std::mutex g_pool_mu;  // one parallel region at a time
+}  // namespace
+
+void host_parallel_for(int n, std::function<void(int)> const& fn) {
+    HostPool* p = n >= 4 ? host_pool() : nullptr;
+    std::unique_lock<std::mutex> lk(g_pool_mu, std::defer_lock);
+    if (p && lk.try_lock()) {
+        p->run(n, fn);
+        return;
+    }
+    for (int i = 0; i < n; ++i) fn(i);
 }
 
 }  // namespace nc

@@ -50,7 +50,15 @@ _CHAIN_SIGS = {
     "nc_chain_projector_overlap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "nc_chain_randomize": (C.c_int, [C.c_void_p, C.c_uint64]),
     "nc_chain_solve_for_multiple_levels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_double)]),
+    "nc_set_krylov_graphs": (C.c_int, [C.c_int]),
 }
+
+
+def set_krylov_graphs(on: bool) -> bool:
+    """Record each Arnoldi extension of a lockstep (ChainBatch) site solve in a HIP
+    graph and replay it (default), or launch the kernels one by one.  Returns the
+    previous setting."""
+    return bool(_lib().nc_set_krylov_graphs(1 if on else 0))
 
 
 def _lib():
@@ -267,9 +275,10 @@ class ChainBatch:
         self._check(_lib().nc_chain_batch_sweep_until_converged(self._h))
 
     def times(self) -> dict:
-        t = (C.c_double * 2)()
+        t = (C.c_double * 6)()
         self._check(_lib().nc_chain_batch_times(self._h, t))
-        return {"moves_ms": t[0], "site_solves_ms": t[1]}
+        return {"moves_ms": t[0], "site_solves_ms": t[1], "prepare_ms": t[2], "batched_solves_ms": t[3],
+                "post_checks_ms": t[4], "krylov_ms": t[5]}
 
     @property
     def energies(self) -> List[float]:
