@@ -405,6 +405,14 @@ def main():
         }
         result["gpu_vs_cpu_matvec_speedup"] = round(cpu_ms / ms64, 2)
         result["max_rel_err_vs_oracle"] = err
+    # ---- autotuner choices for the measured shapes -------------------------
+    table = core.autotune_table()
+    keys = ["chi_l %d chi_r %d d %d" % (c, c, dd) for c, dd in ((chi, d), (256, 2), (2048, 3))]
+    result["autotune"] = {
+        "on": bool(core.set_autotune(True)),
+        "keys_tuned": len(table),
+        "measured_shapes": [ln for ln in table if any(k in ln for k in keys) and "chains 1:" in ln],
+    }
     print(json.dumps(result), flush=True)
 
 

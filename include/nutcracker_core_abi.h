@@ -286,6 +286,17 @@ int nc_chain_batch_times(nc_chain_batch* batch, double* times_ms);
    HIP graph and replays it on later restarts; on = 0 launches the same kernels
    one by one.  Results are identical either way.  Returns the previous setting. */
 int nc_set_krylov_graphs(int on);
+/* GEMM autotuner (no counterpart in the reference).  On (default; NC_AUTOTUNE=0
+   in the environment turns the default off), the first launch of every
+   contraction plan with a new shape key (chi_left, chi_right, d, MPO bonds,
+   contraction step; grouped plans: the largest of each and the chain count)
+   times the candidate launch choices -- K step per LDS round 16 or 32, K split
+   1/2/4/8, 3- or 4-multiplication complex product -- and keeps the fastest.
+   nc_set_autotune returns the previous setting; nc_autotune_table writes the
+   choices made so far as text (one line per key) and returns the length needed
+   including the terminating zero. */
+int nc_set_autotune(int on);
+int64_t nc_autotune_table(char* buf, int64_t len);
 /* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,
    sites of the mixed form finished in FP64 (a complex64 solve that came out above
    the previous energy);

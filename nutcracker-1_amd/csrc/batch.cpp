@@ -50,6 +50,18 @@ void grouped_env_right(std::vector<EnvRightItem> const& items, hipStream_t s) {
     GemmPlan A, Cp;
     A.M = A.N = Cp.M = Cp.N = 1;
     Cp.opA = OpC; Cp.opB = OpN;
+    A.key.tag = 'p';
+    Cp.key.tag = 'q';
+    A.key.count = Cp.key.count = (int)items.size();
+    for (auto const& it : items) {
+        for (TuneKey* k : {&A.key, &Cp.key}) {
+            k->bl = std::max(k->bl, it.bl);
+            k->br = std::max(k->br, it.br);
+            k->d = std::max(k->d, it.op->d);
+            k->cl = std::max(k->cl, it.op->cl);
+            k->cr = std::max(k->cr, it.op->cr);
+        }
+    }
     std::vector<MixItem<double2>> mix;
     for (auto const& it : items) {
         MpoSite const& op = *it.op;
@@ -144,6 +156,19 @@ void ChainBatch::env_left_batch(std::vector<Chain*> const& cs, hipStream_t s) {
     GemmPlan A, Cp;
     A.M = A.N = Cp.M = Cp.N = 1;
     Cp.opA = OpN; Cp.opB = OpC;
+    A.key.tag = 'a';
+    Cp.key.tag = 'l';
+    A.key.count = Cp.key.count = (int)cs.size();
+    for (Chain* c : cs) {
+        const int i = c->current - 1;
+        for (TuneKey* k : {&A.key, &Cp.key}) {
+            k->bl = std::max(k->bl, c->bonds[i]);
+            k->br = std::max(k->br, c->bonds[i + 1]);
+            k->d = std::max(k->d, c->ops[i]->d);
+            k->cl = std::max(k->cl, c->ops[i]->cl);
+            k->cr = std::max(k->cr, c->ops[i]->cr);
+        }
+    }
     std::vector<MixItem<double2>> mix;
     for (Chain* c : cs) {
         const int i = c->current - 1;

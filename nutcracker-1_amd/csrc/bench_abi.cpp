@@ -80,6 +80,7 @@ int bench_heff(const char* name, uint32_t bl, uint32_t br, uint32_t cl, uint32_t
             for (uint32_t i = 0; i < preps; ++i) {
                 typename BenchPlan<CT>::type pa;
                 pa.M = (int)br; pa.N = (int)bl;
+                pa.key = TuneKey{'A', (int)bl, (int)br, (int)d, (int)cl, (int)cr, 1};
                 for (size_t fi = 0; fi < op.froms.size(); ++fi)
                     for (uint32_t a = 0; a < d; ++a) {
                         pa.begin_batch(P + ((int64_t)fi * d + a) * rs, br);
@@ -87,6 +88,7 @@ int bench_heff(const char* name, uint32_t bl, uint32_t br, uint32_t cl, uint32_t
                     }
                 typename BenchPlan<CT>::type pc;
                 pc.M = (int)br; pc.N = (int)(bl * d);
+                pc.key = TuneKey{'C', (int)bl, (int)br, (int)d, (int)cl, (int)cr, 1};
                 pc.begin_batch(dy.ptr, br);
                 for (size_t ti = 0; ti < op.tos.size(); ++ti)
                     pc.add_seg(dR.ptr + (int64_t)op.tos[ti] * br * br, br, Q + (int64_t)ti * d * rs, br, (int)br);

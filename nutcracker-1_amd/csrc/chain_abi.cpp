@@ -4,6 +4,7 @@
 #include "nc_chain.hpp"
 #include "../../include/nutcracker_core_abi.h"
 
+#include <algorithm>
 #include <cstring>
 
 using namespace nc;
@@ -258,6 +259,22 @@ int nc_chain_stats(nc_chain* c, int64_t* counts, double* times_ms) {
         times_ms[3] = s.t_reset_ms;
         return 0;
     });
+}
+
+int nc_set_autotune(int on) {
+    const int prev = nc::gemm_autotune() ? 1 : 0;
+    nc::gemm_autotune() = on != 0;
+    return prev;
+}
+
+int64_t nc_autotune_table(char* buf, int64_t len) {
+    const std::string t = nc::gemm_tune_table();
+    if (buf && len > 0) {
+        const size_t n = std::min<size_t>((size_t)len - 1, t.size());
+        std::memcpy(buf, t.data(), n);
+        buf[n] = 0;
+    }
+    return (int64_t)t.size() + 1;
 }
 
 int nc_set_krylov_graphs(int on) {

@@ -87,6 +87,7 @@ void step_a_left(int bl, int br, MpoSite const& op, const CT* L, const CT* X, CT
     const int64_t rs = (int64_t)br * bl;
     typename Plan<CT>::type plan;
     plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+    plan.key = TuneKey{'A', bl, br, op.d, op.cl, op.cr, 1};
     for (size_t fi = 0; fi < op.froms.size(); ++fi)
         for (int a = 0; a < op.d; ++a) {
             plan.begin_batch(P + ((int64_t)fi * op.d + a) * rs, br);
@@ -107,6 +108,7 @@ void heff_matvec_t(int bl, int br, MpoSite const& op, const CT* L, const CT* R, 
     mix_blocks(op, true, P, Q, rs, s);
     typename Plan<CT>::type plan;
     plan.M = br; plan.N = bl * d; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+    plan.key = TuneKey{'C', bl, br, d, op.cl, op.cr, 1};
     plan.begin_batch(y, br);
     for (size_t ti = 0; ti < op.tos.size(); ++ti)
         plan.add_seg(R + (int64_t)op.tos[ti] * br * br, br, Q + (int64_t)ti * d * rs, br, br);
@@ -126,6 +128,7 @@ void env_left_t(int bl, int br, MpoSite const& op, const CT* L, const CT* psi, C
     if ((int)op.tos.size() < op.cr) dzero(newL, (size_t)br * br * op.cr * sizeof(CT));
     typename Plan<CT>::type plan;
     plan.M = br; plan.N = br; plan.opA = OpN; plan.opB = OpC; plan.accumulate = false;
+    plan.key = TuneKey{'L', bl, br, d, op.cl, op.cr, 1};
     for (size_t ti = 0; ti < op.tos.size(); ++ti) {
         plan.begin_batch(newL + (int64_t)op.tos[ti] * br * br, br);
         plan.add_seg(Q + (int64_t)ti * d * rs, br, psi, br, bl * d);
@@ -144,6 +147,7 @@ void env_right_t(int bl, int br, MpoSite const& op, const CT* R, const CT* psi, 
     {   // P'_{t,a} = R_t Psi_a
         typename Plan<CT>::type plan;
         plan.M = br; plan.N = bl; plan.opA = OpN; plan.opB = OpN; plan.accumulate = false;
+        plan.key = TuneKey{'r', bl, br, d, op.cl, op.cr, 1};
         for (size_t ti = 0; ti < op.tos.size(); ++ti)
             for (int a = 0; a < d; ++a) {
                 plan.begin_batch(P + ((int64_t)ti * d + a) * rs, br);
@@ -155,6 +159,7 @@ void env_right_t(int bl, int br, MpoSite const& op, const CT* R, const CT* psi, 
     if ((int)op.froms.size() < op.cl) dzero(newR, (size_t)bl * bl * op.cl * sizeof(CT));
     typename Plan<CT>::type plan;  // N_f = sum_b Psi_b^H Q'_{f,b}
     plan.M = bl; plan.N = bl; plan.opA = OpC; plan.opB = OpN; plan.accumulate = false;
+    plan.key = TuneKey{'R', bl, br, d, op.cl, op.cr, 1};
     for (size_t fi = 0; fi < op.froms.size(); ++fi) {
         plan.begin_batch(newR + (int64_t)op.froms[fi] * bl * bl, bl);
         for (int b = 0; b < d; ++b)

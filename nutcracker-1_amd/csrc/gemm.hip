@@ -18,7 +18,10 @@
 // (neighbouring tiles share rows of A or columns of B in that XCD's L2).
 #include "nc_internal.hpp"
 
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace nc {
 
@@ -326,20 +329,26 @@ Arena& arena() {
     return *a;
 }
 
-template <typename T, int OPA, int OPB>
-void launch_typed(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
+template <typename T, int OPA, int OPB, int BK>
+void launch_bk(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
 #define NC_L(ACCV, M3V) \
-    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, 16>), grid, dim3(256), 0, st, b, s)
+    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, BK>), grid, dim3(256), 0, st, b, s)
     if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
     else { if (m3) NC_L(false, true); else NC_L(false, false); }
 #undef NC_L
 }
 
+template <typename T, int OPA, int OPB>
+void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
+    if (bk == 32) launch_bk<T, OPA, OPB, 32>(acc, m3, grid, b, s, st);
+    else launch_bk<T, OPA, OPB, 16>(acc, m3, grid, b, s, st);
+}
+
 template <typename T>
-void launch_any(int opA, int opB, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
+void launch_any(int opA, int opB, int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
                 hipStream_t st) {
 #define NC_O(a, bb) \
-    if (opA == a && opB == bb) { launch_typed<T, a, bb>(acc, m3, grid, b, s, st); return; }
+    if (opA == a && opB == bb) { launch_typed<T, a, bb>(bk, acc, m3, grid, b, s, st); return; }
     NC_O(OpN, OpN) NC_O(OpN, OpT) NC_O(OpN, OpC) NC_O(OpT, OpN) NC_O(OpC, OpN) NC_O(OpC, OpC) NC_O(OpT, OpT)
 #undef NC_O
     throw Error("GEMM: unsupported operand combination", -71);
@@ -390,6 +399,7 @@ struct Built {
     std::vector<const int*> lives;
     int opA = OpN, opB = OpN;
     bool acc = false, m3 = false;
+    int bk = 16;
     int nsplit = 1;
     int64_t slice = 0, maxtiles = 0;
     int M = 0, N = 0;
@@ -397,11 +407,19 @@ struct Built {
     bool empty() const { return b.empty() || maxtiles <= 0; }
 };
 
+// Launch choices of one plan: K step staged per LDS round, K split (0 = the
+// heuristic below), complex product form (-1 = the precision's default).
+struct GemmChoice {
+    int bk = 16;
+    int nsplit = 0;
+    int m3 = -1;
+};
+
 // Fills per-batch sizes, chooses split-K and expands it.  `work(bytes)` gives the
 // split-K workspace.
 template <typename T, typename WorkFn>
 void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
-                std::vector<SegT<T>> const& segs, Built<T>& out, WorkFn work) {
+                std::vector<SegT<T>> const& segs, Built<T>& out, WorkFn work, GemmChoice ch = GemmChoice()) {
     typedef typename Cplx<T>::type CT;
     out = Built<T>();
     out.opA = opA;
@@ -431,13 +449,17 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
         for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
         maxk = std::max(maxk, k);
     }
-    // split K when the launch would leave most of the chip idle
+    // split K when the launch would leave most of the chip idle (only plans
+    // whose batches share one output size)
     int nsplit = 1;
-    if (uniform && g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
+    if (ch.nsplit > 0) {
+        if (uniform) nsplit = ch.nsplit;
+    } else if (uniform && g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
         while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
     }
     out.nsplit = nsplit;
-    out.m3 = sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32;
+    out.m3 = ch.m3 >= 0 ? ch.m3 != 0 : (sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32);
+    out.bk = ch.bk == 32 ? 32 : 16;
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
@@ -499,7 +521,7 @@ template <typename T>
 void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
     typedef typename Cplx<T>::type CT;
     dim3 grid((unsigned)bp.maxtiles, (unsigned)bp.b.size());
-    launch_any<T>(bp.opA, bp.opB, bp.nsplit == 1 ? bp.acc : false, bp.m3, grid, dv.b, dv.sg, s);
+    launch_any<T>(bp.opA, bp.opB, bp.bk, bp.nsplit == 1 ? bp.acc : false, bp.m3, grid, dv.b, dv.sg, s);
     NC_HIP(hipGetLastError());
     if (bp.nsplit == 1) return;
     unsigned g = (unsigned)std::min<int64_t>(2048, (bp.slice + 255) / 256);
@@ -508,15 +530,137 @@ void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
     NC_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner.  The first time a plan with a given shape key runs, every
+// candidate launch choice (K step 16 or 32, K split 1-8 where the batches share
+// one size, 3- or 4-multiplication complex product) is timed with HIP events on
+// the plan's own operands (after a warm-up, two interleaved rounds, the minimum
+// per candidate); the fastest is kept for the key when it beats the default
+// choice by more than 3 %.  Accumulating plans are never tuned (their launches
+// are not repeatable).
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+    GemmChoice choice;
+    double default_us = 0, best_us = 0;
+    int candidates = 0;
+};
+std::mutex g_tune_mu;
+std::map<std::vector<int>, TuneEntry> g_tune;
+
+std::vector<int> tune_id(int prec, int opA, int opB, TuneKey const& k) {
+    return {prec, opA, opB, k.tag, k.bl, k.br, k.d, k.cl, k.cr, k.count};
+}
+
+template <typename T>
+GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>> const& batches,
+                  std::vector<SegT<T>> const& segs, TuneKey const& key, hipStream_t s) {
+    if (!key.tag || accumulate || !gemm_autotune() || batches.empty()) return GemmChoice();
+    const std::vector<int> id = tune_id(sizeof(T) == sizeof(double) ? 64 : 32, opA, opB, key);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune.find(id);
+        if (it != g_tune.end()) return it->second.choice;
+    }
+    // candidates; every batch runs during tuning (live flags ignored)
+    std::vector<BatchT<T>> bs = batches;
+    bool uniform = true;
+    int64_t maxk = 0;
+    for (auto& b : bs) {
+        b.live = nullptr;
+        if (b.M == 0) b.M = M;
+        if (b.N == 0) b.N = N;
+        if (b.M != bs[0].M || b.N != bs[0].N) uniform = false;
+        int64_t k = 0;
+        for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
+        maxk = std::max(maxk, k);
+    }
+    std::vector<GemmChoice> cands;
+    cands.push_back(GemmChoice());
+    for (int bk : {16, 32})
+        for (int m3 : {0, 1})
+            for (int ns : {1, 2, 4, 8}) {
+                if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
+                cands.push_back(GemmChoice{bk, ns, m3});
+            }
+    // every candidate built once (own descriptors; the split-K workspace is
+    // shared, candidates run one after another)
+    struct Cand {
+        Built<T> bp;
+        DevBuf<char> desc;
+        BuiltDev<T> dv{};
+    };
+    std::vector<Cand> built(cands.size());
+    size_t work_need = 0;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        std::vector<BatchT<T>> b2 = bs;
+        build_plan<T>(opA, opB, false, M, N, b2, segs, built[c].bp, [&](size_t bytes) {
+            work_need = std::max(work_need, bytes);
+            return nullptr;  // fixed up below
+        }, cands[c]);
+    }
+    DevBuf<char> work(std::max<size_t>(work_need, 1));
+    for (size_t c = 0; c < cands.size(); ++c) {
+        std::vector<BatchT<T>> b2 = bs;
+        Cand& cd = built[c];
+        build_plan<T>(opA, opB, false, M, N, b2, segs, cd.bp, [&](size_t) { return (void*)work.ptr; }, cands[c]);
+        if (cd.bp.empty()) return GemmChoice();
+        const void* parts[5];
+        size_t sizes[5], offs[5], off = 0;
+        parts_of(cd.bp, parts, sizes);
+        for (int i = 0; i < 5; ++i) {
+            offs[i] = off;
+            off = (off + sizes[i] + 255) & ~size_t(255);
+        }
+        std::vector<char> host(off);
+        for (int i = 0; i < 5; ++i)
+            if (sizes[i]) std::memcpy(host.data() + offs[i], parts[i], sizes[i]);
+        cd.desc.resize(off);
+        h2d(cd.desc.ptr, host.data(), off);
+        cd.dv = dev_of<T>(cd.desc.ptr, offs);
+    }
+    hipEvent_t e0, e1;
+    NC_HIP(hipEventCreate(&e0));
+    NC_HIP(hipEventCreate(&e1));
+    // clocks up first, then two interleaved rounds; the minimum per candidate
+    for (int w = 0; w < 3; ++w) launch_built(built[0].bp, built[0].dv, s);
+    std::vector<double> us(cands.size(), 1e30);
+    const int reps = 3;
+    for (int round = 0; round < 2; ++round)
+        for (size_t c = 0; c < cands.size(); ++c) {
+            launch_built(built[c].bp, built[c].dv, s);
+            NC_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) launch_built(built[c].bp, built[c].dv, s);
+            NC_HIP(hipEventRecord(e1, s));
+            NC_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            NC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            us[c] = std::min(us[c], 1e3 * ms / reps);
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    size_t best = 0;
+    for (size_t c = 1; c < cands.size(); ++c)
+        if (us[c] < us[best]) best = c;
+    TuneEntry e;
+    e.default_us = us[0];
+    e.best_us = us[best];
+    e.candidates = (int)cands.size();
+    e.choice = us[best] < 0.97 * us[0] ? cands[best] : GemmChoice();
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[id] = e;
+    return e.choice;
+}
+
 template <typename T>
 void run_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>>& batches,
-              std::vector<SegT<T>> const& segs, hipStream_t s) {
+              std::vector<SegT<T>> const& segs, hipStream_t s, TuneKey const& key = TuneKey()) {
     static thread_local DevBuf<char> work;
+    const GemmChoice ch = choose<T>(opA, opB, accumulate, M, N, batches, segs, key, s);
     Built<T> bp;
     build_plan<T>(opA, opB, accumulate, M, N, batches, segs, bp, [&](size_t bytes) {
         if (work.n < bytes) work.resize(bytes);
         return (void*)work.ptr;
-    });
+    }, ch);
     if (bp.empty()) return;
     const void* parts[5];
     size_t sizes[5], offs[5];
@@ -540,13 +684,15 @@ struct PreparedGemmT final : PreparedLaunch {
 
 template <typename T>
 std::unique_ptr<PreparedLaunch> prepare_plan(int opA, int opB, bool accumulate, int M, int N,
-                                             std::vector<BatchT<T>>& batches, std::vector<SegT<T>> const& segs) {
+                                             std::vector<BatchT<T>>& batches, std::vector<SegT<T>> const& segs,
+                                             TuneKey const& key) {
     std::unique_ptr<PreparedGemmT<T>> p(new PreparedGemmT<T>());
     PreparedGemmT<T>* q = p.get();
+    const GemmChoice ch = choose<T>(opA, opB, accumulate, M, N, batches, segs, key, Context::get().stream);
     build_plan<T>(opA, opB, accumulate, M, N, batches, segs, q->bp, [q](size_t bytes) {
         q->work.resize(bytes);
         return (void*)q->work.ptr;
-    });
+    }, ch);
     if (q->bp.empty()) return p;
     const void* parts[5];
     size_t sizes[5], offs[5], off = 0;
@@ -568,6 +714,31 @@ std::unique_ptr<PreparedLaunch> prepare_plan(int opA, int opB, bool accumulate, 
 
 GemmTuning& gemm_tuning() { return g_tuning; }
 
+bool& gemm_autotune() {
+    static bool on = [] {
+        const char* e = std::getenv("NC_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+std::string gemm_tune_table() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::string out;
+    char line[256];
+    for (auto const& kv : g_tune) {
+        auto const& k = kv.first;
+        auto const& e = kv.second;
+        snprintf(line, sizeof line,
+                 "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d split %d %s (%.1f us, default %.1f us, %d candidates)\n",
+                 k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9], e.choice.bk, e.choice.nsplit,
+                 e.choice.m3 < 0 ? "3M/4M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
+                 e.candidates);
+        out += line;
+    }
+    return out;
+}
+
 void* stage_descriptors(const void* host, size_t bytes, hipStream_t s, int* slot) {
     const void* parts[1] = {host};
     size_t sizes[1] = {bytes};
@@ -583,7 +754,7 @@ void zgemm_run(GemmPlan const& plan, hipStream_t s) {
     std::vector<SegT<double>> sg(plan.segs.size());
     std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
     std::memcpy(sg.data(), plan.segs.data(), sg.size() * sizeof(GemmSeg));
-    run_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
+    run_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s, plan.key);
 }
 
 static void to_float_lists(CGemmPlan const& plan, std::vector<BatchT<float>>& b, std::vector<SegT<float>>& sg) {
@@ -595,7 +766,7 @@ void cgemm_run(CGemmPlan const& plan, hipStream_t s) {
     std::vector<BatchT<float>> b;
     std::vector<SegT<float>> sg;
     to_float_lists(plan, b, sg);
-    run_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s);
+    run_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s, plan.key);
 }
 
 std::unique_ptr<PreparedLaunch> prepare_gemm(GemmPlan const& plan) {
@@ -603,14 +774,14 @@ std::unique_ptr<PreparedLaunch> prepare_gemm(GemmPlan const& plan) {
     std::vector<SegT<double>> sg(plan.segs.size());
     std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
     std::memcpy(sg.data(), plan.segs.data(), sg.size() * sizeof(GemmSeg));
-    return prepare_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg);
+    return prepare_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, plan.key);
 }
 
 std::unique_ptr<PreparedLaunch> prepare_gemm(CGemmPlan const& plan) {
     std::vector<BatchT<float>> b;
     std::vector<SegT<float>> sg;
     to_float_lists(plan, b, sg);
-    return prepare_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg);
+    return prepare_plan<float>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, plan.key);
 }
 
 }  // namespace nc

@@ -122,6 +122,19 @@ struct GemmTuning {
 };
 GemmTuning& gemm_tuning();
 
+// Shape key of a GEMM plan for the autotuner: the site's bond dimensions
+// (chi_left, chi_right), physical dimension, MPO bonds, and which contraction
+// step the plan is (tag; 0 = not tuned).  Grouped plans of several chains use
+// the largest bond dimensions and the number of chains.
+struct TuneKey {
+    int tag = 0;
+    int bl = 0, br = 0, d = 0, cl = 0, cr = 0, count = 1;
+};
+// Autotuner switch (default on; NC_AUTOTUNE=0 in the environment turns the
+// default off) and a text table of the choices made so far.
+bool& gemm_autotune();
+std::string gemm_tune_table();
+
 // Host-side builder: keeps descriptor lists and uploads them per launch into a
 // small reusable device arena.
 struct GemmPlan {
@@ -129,6 +142,7 @@ struct GemmPlan {
     std::vector<GemmSeg> segs;
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
+    TuneKey key;
     void clear() { batches.clear(); segs.clear(); }
     void begin_batch(double2* C, int64_t ldc, int M_ = 0, int N_ = 0, const int* live = nullptr) {
         batches.push_back(GemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_, live});
@@ -162,6 +176,7 @@ struct CGemmPlan {
     std::vector<CGemmSeg> segs;
     int M = 0, N = 0, opA = OpN, opB = OpN;
     bool accumulate = false;
+    TuneKey key;
     void begin_batch(float2* C, int64_t ldc, int M_ = 0, int N_ = 0, const int* live = nullptr) {
         batches.push_back(CGemmBatch{C, ldc, (int32_t)segs.size(), (int32_t)segs.size(), M_, N_, live});
     }

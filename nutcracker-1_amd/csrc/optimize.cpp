@@ -382,6 +382,21 @@ namespace nc {
 // Sites with projectors or a dense-solve size (orth <= 4) are solved one by one.
 // ----------------------------------------------------------------------------
 namespace {
+// shape key of a grouped plan: the largest bond dimensions and the chain count
+TuneKey group_key(int tag, std::vector<OptimizeArgs const*> const& as) {
+    TuneKey k;
+    k.tag = tag;
+    k.count = (int)as.size();
+    for (auto const* a : as) {
+        k.bl = std::max(k.bl, a->bl);
+        k.br = std::max(k.br, a->br);
+        k.d = std::max(k.d, a->d);
+        k.cl = std::max(k.cl, a->cl);
+        k.cr = std::max(k.cr, a->cr);
+    }
+    return k;
+}
+
 // Grouped H_eff matvec plans of several chains, y_q = H_q x_q (complex128):
 // step A (X_a L_f), the block mixing, step C (sum_t R_t Q_t).  live[q], when
 // given, gates chain q at launch time.
@@ -389,6 +404,8 @@ void heff_plans64(std::vector<OptimizeArgs const*> const& as, std::vector<Worksp
                   std::vector<const double2*> const& xs, std::vector<double2*> const& ys, const int* live,
                   GemmPlan& A, std::vector<MixItem<double2>>& mix, GemmPlan& Cp) {
     A.M = A.N = Cp.M = Cp.N = 1;
+    A.key = group_key('a', as);
+    Cp.key = group_key('c', as);
     for (size_t q = 0; q < as.size(); ++q) {
         OptimizeArgs const& a = *as[q];
         Workspace& w = *ws[q];
@@ -415,6 +432,8 @@ void heff_plans32(std::vector<OptimizeArgs const*> const& as, std::vector<Worksp
                   std::vector<const float2*> const& xs, std::vector<float2*> const& ys, const int* live,
                   CGemmPlan& A, std::vector<MixItem<float2>>& mix, CGemmPlan& Cp) {
     A.M = A.N = Cp.M = Cp.N = 1;
+    A.key = group_key('a', as);
+    Cp.key = group_key('c', as);
     for (size_t q = 0; q < as.size(); ++q) {
         OptimizeArgs const& a = *as[q];
         Workspace& w = *ws[q];

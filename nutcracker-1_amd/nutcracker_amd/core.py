@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Optional, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -69,6 +69,26 @@ def lib() -> C.CDLL:
             f.argtypes = args
         _lib = h
     return _lib
+
+
+def set_autotune(on: bool) -> bool:
+    """Switch the GEMM autotuner (keyed on chi_left, chi_right, d, MPO bonds and
+    contraction step) on or off; returns the previous setting."""
+    f = lib().nc_set_autotune
+    f.restype = C.c_int
+    f.argtypes = [C.c_int]
+    return bool(f(1 if on else 0))
+
+
+def autotune_table() -> List[str]:
+    """The autotuner's choices so far, one line per shape key."""
+    f = lib().nc_autotune_table
+    f.restype = C.c_int64
+    f.argtypes = [C.c_char_p, C.c_int64]
+    n = f(None, 0)
+    buf = C.create_string_buffer(int(n))
+    f(buf, n)
+    return [ln for ln in buf.value.decode().splitlines() if ln]
 
 
 def last_error() -> str:

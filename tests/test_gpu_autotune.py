@@ -1,0 +1,79 @@
+"""The GEMM autotuner: contraction plans keyed on (chi_left, chi_right, d, MPO
+bonds, step) are timed over their launch choices on first use.  Whatever it
+picks, the matvec must keep the drop-in's accuracy against the oracle's
+strategy-3 matvec (core.f95:532-621): 1e-12 relative in complex128, 2e-6 in
+complex64 -- and agree with the untuned launch to round-off."""
+
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from helpers import crand, hermitian_env, random_sparse_matrices, rel
+
+import oracle.core as oc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from nutcracker_amd import core
+    h = core.lib()
+    for name in ("nc_bench_heff", "nc_bench_heff_c64"):
+        f = getattr(h, name)
+        f.restype = C.c_int
+        f.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)] + [C.c_void_p] * 5 + \
+            [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    return core, h
+
+
+def _heff(h, name, bl, br, cl, cr, d, idx, mats, L, R, x):
+    y = np.zeros_like(x)
+    idx = np.ascontiguousarray(np.asarray(idx, np.uint32).reshape(-1, order="F"))
+    mats = np.ascontiguousarray(np.asarray(mats).reshape(-1, order="F"))
+    total = C.c_double(0)
+    rc = getattr(h, name)(bl, br, cl, cr, d, idx.shape[0] // 2, idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                          mats.ctypes.data, L.ctypes.data, R.ctypes.data, x.ctypes.data, y.ctypes.data, 1, 2,
+                          C.byref(total), None)
+    assert rc == 0
+    return y
+
+
+# (d, chi_left, chi_right, w_left, w_right): a K-split-prone shape (few output
+# tiles, long K), a ragged one, and a spin-1 one
+SHAPES = [(2, 192, 192, 5, 5), (2, 100, 72, 5, 5), (3, 64, 96, 5, 5)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tuned_matvec_against_oracle(lib, shape):
+    core, h = lib
+    d, bl, br, cl, cr = shape
+    print("shape", shape, flush=True)
+    rng = np.random.default_rng(sum(shape) * 104729)
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    x = crand(rng, br, bl, d)
+    ref = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x, oc.iteration_stage_1(bl, cl, cr, d, L, idx, mats))
+    prev = core.set_autotune(True)
+    try:
+        y64 = _heff(h, "nc_bench_heff", bl, br, cl, cr, d, idx, mats, L, R, x)
+        y32 = _heff(h, "nc_bench_heff_c64", bl, br, cl, cr, d, idx, mats, L, R, x)
+        core.set_autotune(False)
+        u64 = _heff(h, "nc_bench_heff", bl, br, cl, cr, d, idx, mats, L, R, x)
+    finally:
+        core.set_autotune(prev)
+    assert rel(y64, ref) < 1e-12
+    assert rel(y64, u64) < 1e-13
+    e32 = rel(y32, ref)
+    assert 1e-9 < e32 < 2e-6
+    table = core.autotune_table()
+    mine = [ln for ln in table if re.search(r"chi_l %d chi_r %d d %d w %d/%d chains 1:" % (bl, br, d, cl, cr), ln)]
+    print("\n".join(mine), flush=True)
+    steps = {(re.search(r"^fp(\d+)", ln).group(1), re.search(r"step (\S)", ln).group(1)) for ln in mine}
+    assert {("64", "A"), ("64", "C"), ("32", "A"), ("32", "C")} <= steps
+    for ln in mine:
+        m = re.search(r"\(([\d.]+) us, default ([\d.]+) us, (\d+) candidates\)", ln)
+        assert m and float(m.group(1)) <= float(m.group(2)) and int(m.group(3)) >= 4
