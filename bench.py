@@ -64,6 +64,22 @@ def hermitian_env(rng, b, c):
     return np.asfortranarray(e)
 
 
+def tfim_ground_energy(n, h):
+    """H = -sum Z_i Z_{i+1} - h sum X_i on an open chain: free fermions (Lieb,
+    Schultz, Mattis), E0 = (tr A - sum_k eps_k) / 2 - n h with eps_k^2 the
+    eigenvalues of (A - B)(A + B)."""
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 2 * h
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -1.0
+        B[i, i + 1] = -1.0
+        B[i + 1, i] = 1.0
+    lam = np.sqrt(np.clip(np.linalg.eigvalsh((A - B) @ (A + B)), 0, None))
+    return 0.5 * (np.trace(A) - lam.sum()) - n * h
+
+
 def flops_matvec(bl, br, d, n_from, n_to, nblk):
     """Minimal block-sparse work of one H_eff application (real flops)."""
     return 8 * d * bl * br * (n_from * bl + n_to * br) + 8 * nblk * bl * br * d * d
@@ -152,6 +168,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form (64 chains) timing")
     ap.add_argument("--batch-chains", type=int, default=64)
     ap.add_argument("--batch-sample", type=int, default=16)
+    ap.add_argument("--no-batch-converge", action="store_true",
+                    help="skip sweeping the 64 chains to convergence (energies against the closed form)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy calibration kernel")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
@@ -375,6 +393,29 @@ def main():
             "batching_speedup": round(t_s / t_b, 2) if sample else None,
             "energies_after_one_sweep": [round(e, 10) for e in energies],
         }
+        if not args.no_batch_converge:
+            # the same 64 chains swept in lockstep until every one has converged;
+            # their ground energies against the free-fermion closed form
+            cs = make(range(nb))
+            t0 = time.perf_counter()
+            b3 = ChainBatch(cs)
+            b3.sweep_until_converged()
+            t_conv = time.perf_counter() - t0
+            b3.close()
+            got = np.array([c.energy for c in cs])
+            sweeps = [c.stats()["sweeps"] for c in cs]
+            for c in cs:
+                c.close()
+            exact = np.array([tfim_ground_energy(nsites, float(h)) for h in hs])
+            relerr = np.abs(got - exact) / np.abs(exact)
+            log("batch converged: %.2f s, sweeps %d-%d, max rel. error vs closed form %.2e"
+                % (t_conv, min(sweeps), max(sweeps), relerr.max()))
+            result["batch"]["converged"] = {
+                "wall_s": round(t_conv, 2), "sweeps_min": int(min(sweeps)), "sweeps_max": int(max(sweeps)),
+                "max_rel_err_vs_closed_form": float(relerr.max()),
+                "ground_energies": [round(float(e), 10) for e in got],
+                "closed_form": [round(float(e), 10) for e in exact],
+            }
 
     # ---- CPU baseline ----------------------------------------------------
     if not args.no_cpu:

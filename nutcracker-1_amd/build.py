@@ -38,6 +38,8 @@ SOURCES = [
     "abi.cpp",
     "chain_abi.cpp",
     "bench_abi.cpp",
+    "formats.cpp",
+    "c_interface.cpp",
 ]
 
 CXXFLAGS = [

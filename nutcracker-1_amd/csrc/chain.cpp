@@ -609,7 +609,8 @@ void Chain::restart_level(uint64_t seed) {
     reset();
 }
 
-std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed) {
+std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed,
+                                                     std::function<void(int)> const& after_level) {
     // solveForMultipleLevels (chain.cpp:301-310): converge (optimizeChain), keep
     // the state as a projector, restart from a new random state orthogonal to
     // the lower levels
@@ -617,38 +618,72 @@ std::vector<double> Chain::solve_for_multiple_levels(int levels, uint64_t seed) 
     for (int k = 0; k + 1 < levels; ++k) {
         optimize_chain();
         energies.push_back(energy);
+        if (after_level) after_level(k);
         add_projector_from_state();
         restart_level(seed + (uint64_t)k + 1);
     }
     if (levels > 0) {
         optimize_chain();
         energies.push_back(energy);
+        if (after_level) after_level(levels - 1);
     }
     return energies;
 }
 
-zd state_overlap(Chain const& a, Chain const& b) {
-    // computeStateOverlap (states.hpp:349-382): <a|b>, contracted from the left
-    // with the overlap sites (conjugates) of a; every site of either chain holds
-    // its part of the state whatever the position of its middle site
-    if (a.N != b.N) throw Error("state overlap: chains of different lengths", -73);
+zd mps_overlap(int N, std::vector<int> const& d, std::vector<int> const& bonds_a,
+               std::vector<const double2*> const& a, std::vector<int> const& bonds_b,
+               std::vector<const double2*> const& b) {
+    // contracted from the left with the overlap sites (conjugates) of a
     hipStream_t s = Context::get().stream;
     ZBuf L(1), ov, next;
     double2 one = make_double2(1, 0);
     h2d(L.ptr, &one, sizeof(double2));
-    for (int i = 0; i < a.N; ++i) {
-        const int d = a.ops[i]->d;
-        if (b.ops[i]->d != d) throw Error("state overlap: physical dimensions differ", -74);
-        const int blo = a.bonds[i], bro = a.bonds[i + 1], bln = b.bonds[i], brn = b.bonds[i + 1];
-        ov.resize((size_t)blo * d * bro);
-        overlap_site(bro, blo, d, a.sites[i].ptr, ov.ptr, s);
+    for (int i = 0; i < N; ++i) {
+        const int blo = bonds_a[i], bro = bonds_a[i + 1], bln = bonds_b[i], brn = bonds_b[i + 1];
+        ov.resize((size_t)blo * d[i] * bro);
+        overlap_site(bro, blo, d[i], a[i], ov.ptr, s);
         next.resize((size_t)brn * bro);
-        vs_left(blo, bro, bln, brn, d, L.ptr, ov.ptr, b.sites[i].ptr, next.ptr, s);
+        vs_left(blo, bro, bln, brn, d[i], L.ptr, ov.ptr, b[i], next.ptr, s);
         std::swap(L, next);
     }
     double2 r;
     d2h(&r, L.ptr, sizeof(double2));
     return zd(r.x, r.y);
+}
+
+zd mps_expectation(std::vector<MpoSite const*> const& ops, std::vector<int> const& bonds,
+                   std::vector<const double2*> const& sites) {
+    hipStream_t s = Context::get().stream;
+    const int N = (int)ops.size();
+    if (ops.front()->cl != 1 || ops.back()->cr != 1) throw Error("expectation: outer operator bonds must be 1", -66);
+    if (bonds.front() != 1 || bonds.back() != 1) throw Error("expectation: outer state bonds must be 1", -67);
+    Workspace ws;
+    ZBuf L(1), next;
+    double2 one = make_double2(1, 0);
+    h2d(L.ptr, &one, sizeof(double2));
+    for (int i = 0; i < N; ++i) {
+        if (i + 1 < N && ops[i]->cr != ops[i + 1]->cl) throw Error("expectation: operator bond mismatch", -65);
+        next.resize((size_t)bonds[i + 1] * bonds[i + 1] * ops[i]->cr);
+        env_left(bonds[i], bonds[i + 1], *ops[i], L.ptr, sites[i], next.ptr, ws, s);
+        std::swap(L, next);
+    }
+    double2 r;
+    d2h(&r, L.ptr, sizeof(double2));
+    return zd(r.x, r.y);
+}
+
+zd state_overlap(Chain const& a, Chain const& b) {
+    // computeStateOverlap (states.hpp:349-382): <a|b>; every site of either chain
+    // holds its part of the state whatever the position of its middle site
+    if (a.N != b.N) throw Error("state overlap: chains of different lengths", -73);
+    for (int i = 0; i < a.N; ++i)
+        if (b.ops[i]->d != a.ops[i]->d) throw Error("state overlap: physical dimensions differ", -74);
+    std::vector<const double2*> pa, pb;
+    for (int i = 0; i < a.N; ++i) {
+        pa.push_back(a.sites[i].ptr);
+        pb.push_back(b.sites[i].ptr);
+    }
+    return mps_overlap(a.N, a.dims(), a.bonds, pa, b.bonds, pb);
 }
 
 void Chain::sweep() {

@@ -94,7 +94,10 @@ struct Chain {
     void clear_projectors();
     double projector_overlap();
     void randomize(uint64_t seed);
-    std::vector<double> solve_for_multiple_levels(int levels, uint64_t seed);
+    // after_level(k), when given, runs once level k has converged (the chain at
+    // its first site, before the level becomes a projector)
+    std::vector<double> solve_for_multiple_levels(int levels, uint64_t seed,
+                                                  std::function<void(int)> const& after_level = nullptr);
     // bond dimensions (utilities.cpp:32-91, utilities.hpp:264-283)
     int chi = 1, initial_chi = 1;
     std::vector<int> dims() const;
@@ -139,5 +142,14 @@ int minimum_bandwidth_for_projectors(std::vector<int> const& dims, int nproj);
 
 // <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382)
 zd state_overlap(Chain const& a, Chain const& b);
+// <a|b> of two matrix product states given as device sites (br, bl, d) with
+// their bond dimensions (N + 1 each) and physical dimensions
+zd mps_overlap(int N, std::vector<int> const& d, std::vector<int> const& bonds_a,
+               std::vector<const double2*> const& a, std::vector<int> const& bonds_b,
+               std::vector<const double2*> const& b);
+// <psi|H|psi> of a matrix product state with an MPO (contract_sos_left from the
+// trivial boundary, core.f95:623-677, as computeExpectationValue does)
+zd mps_expectation(std::vector<MpoSite const*> const& ops, std::vector<int> const& bonds,
+                   std::vector<const double2*> const& sites);
 
 }  // namespace nc

@@ -52,3 +52,28 @@ def phase_overlap(a, b):
     a = np.asarray(a).ravel(order="F")
     b = np.asarray(b).ravel(order="F")
     return abs(np.vdot(a, b)) / (np.linalg.norm(a) * np.linalg.norm(b))
+
+
+def dense_hamiltonian(ops):
+    """The dense matrix of a chain's MPO (small N only): the blocks summed into
+    W[from, to] and contracted along the chain, site 0 the most significant."""
+    tensors = []
+    for o in ops:
+        W = np.zeros((o.cl, o.cr, o.d, o.d), complex)
+        for k in range(o.nmat):
+            f, t = int(o.idx[0][k]) - 1, int(o.idx[1][k]) - 1
+            W[f, t] += np.asarray(o.mats)[:, :, k].T
+        tensors.append(W)
+    H = tensors[0]
+    for W in tensors[1:]:
+        H = np.einsum("aAij,ABkl->aBikjl", H, W).reshape(H.shape[0], W.shape[1], H.shape[2] * W.shape[2], -1)
+    return H[0, 0]
+
+
+def dense_mps(sites):
+    """A matrix product state (site tensors (br, bl, d)) as a dense vector."""
+    v = np.ones((1, 1), complex)
+    for A in sites:
+        v = np.einsum("cl,rls->csr", v, A).reshape(-1, A.shape[0])
+    return v[:, 0]
+
