@@ -536,8 +536,8 @@ void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
 // one size, 3- or 4-multiplication complex product) is timed with HIP events on
 // the plan's own operands (after a warm-up, two interleaved rounds, the minimum
 // per candidate); the fastest is kept for the key when it beats the default
-// choice by more than 3 %.  Accumulating plans are never tuned (their launches
-// are not repeatable).
+// choice by more than 3 %.  Accumulating plans and plans of less than 2e8 flops
+// are never tuned (the first are not repeatable, the second launch-bound).
 // ---------------------------------------------------------------------------
 struct TuneEntry {
     GemmChoice choice;
@@ -574,6 +574,15 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
         for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
         maxk = std::max(maxk, k);
     }
+    // small plans (a few microseconds of matrix-core work) keep the default:
+    // their time is launch overhead, which no choice changes
+    double flops = 0;
+    for (auto const& b : bs) {
+        int64_t k = 0;
+        for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
+        flops += 8.0 * b.M * b.N * k;
+    }
+    if (flops < 2e8) return GemmChoice();
     std::vector<GemmChoice> cands;
     cands.push_back(GemmChoice());
     for (int bk : {16, 32})

@@ -347,6 +347,115 @@ __global__ void bherm_kernel(const BItem* __restrict__ it) {
     }
 }
 
+// ---- device-driven batched iteration: per-job schedule state on the device
+struct NsState {
+    double l;     // lower bound of the singular values
+    double a, b;  // this step's H = a E + b I
+    int phase;    // 0 running, 1 one more step, 2 converged, 3 failed (non-finite)
+    int iters;
+    int live;     // 1 while the job iterates
+    int just;     // 1 in the iteration the job converged (its result is copied out)
+    int m;
+    int _pad;
+};
+
+struct NsItem {
+    double2* G;      // m x m Gram matrix / E / H
+    const double2* X[2];  // the iterate in the two parity buffers
+    double2* Up;     // result
+    int64_t n;       // elements of X (m * ncols)
+};
+
+// E = G - I for the live jobs
+__global__ void ns_eye_kernel(const NsItem* __restrict__ it, const NsState* __restrict__ st) {
+    const int q = blockIdx.y;
+    if (!st[q].live) return;
+    const NsItem d = it[q];
+    const int64_t m = st[q].m, total = m * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+        if (e % m == e / m) d.G[e].x -= 1.0;
+}
+
+// partial sums of |E|^2 per job (fixed block order)
+__global__ void ns_sumsq_kernel(const NsItem* __restrict__ it, const NsState* __restrict__ st,
+                                double* __restrict__ partial) {
+    const int q = blockIdx.y;
+    double acc = 0;
+    if (st[q].live) {
+        const NsItem d = it[q];
+        const int64_t total = (int64_t)st[q].m * st[q].m;
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+            const double2 v = d.G[e];
+            acc += v.x * v.x + v.y * v.y;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// the schedule step of ns_alpha / polar_newton_schulz for every live job
+__global__ void ns_decide_kernel(NsState* __restrict__ st, const double* __restrict__ partial, int nb, int nq,
+                                 double cap) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    NsState s = st[q];
+    s.just = 0;
+    if (!s.live) { st[q] = s; return; }
+    double e2 = 0;
+    for (int b = 0; b < nb; ++b) e2 += partial[(int64_t)q * nb + b];
+    const double err = sqrt(e2);
+    if (!isfinite(err)) {
+        s.phase = 3;
+        s.live = 0;
+    } else if (s.phase == 1 || err < 1e-15 * s.m) {
+        s.phase = 2;
+        s.live = 0;
+        s.just = 1;
+    } else {
+        double l = s.l;
+        if (err < 1.0) l = fmax(l, sqrt(1.0 - err));
+        double alpha = 1.0;
+        if (l < 1.0 - 1e-3) alpha = fmin(cap, sqrt(3.0 / (1.0 + l + l * l)));
+        const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
+        s.l = fmin(fmin(1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l)), p1);
+        // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
+        s.a = -0.5 * alpha * alpha * alpha;
+        s.b = 0.5 * alpha * (3.0 - alpha * alpha);
+        ++s.iters;
+        if (err < 1e-7) s.phase = 1;
+    }
+    st[q] = s;
+}
+
+// H = a E + b I for the live jobs
+__global__ void ns_h_kernel(const NsItem* __restrict__ it, const NsState* __restrict__ st) {
+    const int q = blockIdx.y;
+    if (!st[q].live) return;
+    const NsItem d = it[q];
+    const double a = st[q].a, b = st[q].b;
+    const int64_t m = st[q].m, total = m * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = d.G[e];
+        v.x *= a;
+        v.y *= a;
+        if (e % m == e / m) v.x += b;
+        d.G[e] = v;
+    }
+}
+
+// Up = X (current parity) for the jobs that converged in this iteration
+__global__ void ns_copy_kernel(const NsItem* __restrict__ it, const NsState* __restrict__ st, int parity) {
+    const int q = blockIdx.y;
+    if (!st[q].just) return;
+    const NsItem d = it[q];
+    const double2* __restrict__ X = d.X[parity];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x)
+        d.Up[e] = X[e];
+}
+
 template <typename K>
 void launch_items(K kernel, std::vector<BItem> const& items, int64_t max_elems, hipStream_t s) {
     if (items.empty() || max_elems <= 0) return;
@@ -443,57 +552,83 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
         for (size_t k = 0; k < ids.size(); ++k) { xv.push_back(st[ids[k]].X0.ptr); f.push_back(1.0 / std::sqrt(s2[k])); }
         scale_batch(xv, n0, f, s);
     }
-    for (int it = 0; it < 64; ++it) {
-        std::vector<size_t> act;
-        for (size_t q = 0; q < J; ++q)
-            if (!st[q].done) act.push_back(q);
-        if (act.empty()) break;
-        // G = X X^H, E = G - I, err = ||E||_F
-        GemmPlan g;
-        g.M = g.N = 1; g.opA = OpN; g.opB = OpC;
-        std::vector<BItem> idn;
-        std::vector<const double2*> ev;
-        std::vector<int64_t> en;
-        int64_t maxm = 0;
-        for (size_t q : act) {
-            const int m = jobs[q].m;
-            g.begin_batch(st[q].G.ptr, m, m, m);
-            g.add_seg(st[q].X0.ptr, m, st[q].X0.ptr, m, (int)jobs[q].n);
-            idn.push_back(BItem{st[q].G.ptr, m, 1.0, -1.0});
-            ev.push_back(st[q].G.ptr);
-            en.push_back((int64_t)m * m);
-            maxm = std::max<int64_t>(maxm, (int64_t)m * m);
-        }
-        zgemm_run(g, s);
-        launch_items(bidentity_kernel, idn, maxm, s);
-        std::vector<double> e2 = sumsq_batch(ev, en, s);
-        GemmPlan u;
-        u.M = u.N = 1; u.opA = OpN; u.opB = OpN;
-        std::vector<BItem> hset;
-        std::vector<size_t> upd;
-        for (size_t k = 0; k < act.size(); ++k) {
+    // The iteration runs on the device: every job's schedule (error, alpha, the
+    // lower bound l, convergence) lives in device memory and gates the job's
+    // launches, and the host reads the phases only every few iterations.
+    std::vector<size_t> act;
+    for (size_t q = 0; q < J; ++q)
+        if (!st[q].done) act.push_back(q);
+    if (!act.empty()) {
+        const size_t A = act.size();
+        std::vector<NsItem> items(A);
+        std::vector<NsState> hs(A);
+        int64_t maxmm = 1, maxn = 1;
+        GemmPlan gram[2], upd[2];
+        DevBuf<NsState> dst(A);
+        for (size_t k = 0; k < A; ++k) {
             const size_t q = act[k];
             const int m = jobs[q].m;
-            const double err = std::sqrt(e2[k]);
-            if (!std::isfinite(err)) { st[q].done = true; continue; }  // the SVD below
-            if (st[q].last || err < 1e-15 * m) {
-                d2d(jobs[q].Up, st[q].X0.ptr, (size_t)ns[q] * sizeof(double2));
-                st[q].done = st[q].ok = true;
-                st[q].iters = it;
-                continue;
-            }
-            const double alpha = ns_alpha(st[q].l, err);
-            // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
-            hset.push_back(BItem{st[q].G.ptr, m, -0.5 * alpha * alpha * alpha, 0.5 * alpha * (3.0 - alpha * alpha)});
-            u.begin_batch(st[q].X1.ptr, m, m, (int)jobs[q].n);
-            u.add_seg(st[q].G.ptr, m, st[q].X0.ptr, m, m);
-            upd.push_back(q);
-            if (err < 1e-7) st[q].last = true;
+            items[k] = NsItem{st[q].G.ptr, {st[q].X0.ptr, st[q].X1.ptr}, jobs[q].Up, ns[q]};
+            hs[k] = NsState{1e-16, 0, 0, 0, 0, 1, 0, m, 0};
+            maxmm = std::max<int64_t>(maxmm, (int64_t)m * m);
+            maxn = std::max<int64_t>(maxn, ns[q]);
         }
-        if (upd.empty()) continue;
-        launch_items(bidentity_kernel, hset, maxm, s);
-        zgemm_run(u, s);
-        for (size_t q : upd) std::swap(st[q].X0, st[q].X1);
+        h2d(dst.ptr, hs.data(), A * sizeof(NsState));
+        const int* live0 = &dst.ptr[0].live;
+        for (int p = 0; p < 2; ++p) {
+            gram[p].M = gram[p].N = 1; gram[p].opA = OpN; gram[p].opB = OpC;
+            upd[p].M = upd[p].N = 1; upd[p].opA = OpN; upd[p].opB = OpN;
+            for (size_t k = 0; k < A; ++k) {
+                const size_t q = act[k];
+                const int m = jobs[q].m;
+                const int* lv = reinterpret_cast<const int*>(reinterpret_cast<const char*>(live0) + k * sizeof(NsState));
+                double2* Xp = p == 0 ? st[q].X0.ptr : st[q].X1.ptr;
+                double2* Xo = p == 0 ? st[q].X1.ptr : st[q].X0.ptr;
+                gram[p].begin_batch(st[q].G.ptr, m, m, m, lv);
+                gram[p].add_seg(Xp, m, Xp, m, (int)jobs[q].n);
+                upd[p].begin_batch(Xo, m, m, (int)jobs[q].n, lv);
+                upd[p].add_seg(st[q].G.ptr, m, Xp, m, m);
+            }
+        }
+        std::unique_ptr<PreparedLaunch> pg[2] = {prepare_gemm(gram[0]), prepare_gemm(gram[1])};
+        std::unique_ptr<PreparedLaunch> pu[2] = {prepare_gemm(upd[0]), prepare_gemm(upd[1])};
+        DevBuf<NsItem> ditems(A);
+        h2d(ditems.ptr, items.data(), A * sizeof(NsItem));
+        const unsigned gm = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)A), (maxmm + 255) / 256));
+        const unsigned gs = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(std::max<int64_t>(1, 2048 / (int64_t)A), (maxmm + 255) / 256));
+        const unsigned gc = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)A), (maxn + 255) / 256));
+        DevBuf<double> partial((size_t)A * gs);
+        const double cap = ns_alpha_cap();
+        constexpr int kCheck = 6;  // iterations between host reads of the phases
+        for (int it = 0; it < 64; ++it) {
+            const int p = it & 1;
+            pg[p]->launch(s);
+            hipLaunchKernelGGL(ns_eye_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
+            hipLaunchKernelGGL(ns_sumsq_kernel, dim3(gs, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr,
+                               partial.ptr);
+            hipLaunchKernelGGL(ns_decide_kernel, dim3((unsigned)((A + 63) / 64)), dim3(64), 0, s, dst.ptr,
+                               partial.ptr, (int)gs, (int)A, cap);
+            hipLaunchKernelGGL(ns_copy_kernel, dim3(gc, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr, p);
+            hipLaunchKernelGGL(ns_h_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
+            NC_HIP(hipGetLastError());
+            pu[p]->launch(s);
+            if (it % kCheck == kCheck - 1 || it == 63) {
+                d2h(hs.data(), dst.ptr, A * sizeof(NsState));
+                bool running = false;
+                for (auto const& h : hs) running |= h.live != 0;
+                if (!running) break;
+            }
+        }
+        d2h(hs.data(), dst.ptr, A * sizeof(NsState));
+        for (size_t k = 0; k < A; ++k) {
+            const size_t q = act[k];
+            st[q].done = true;
+            st[q].ok = hs[k].phase == 2;
+            st[q].iters = hs[k].iters;
+        }
     }
     // P = A U_p^H (Hermitian part) for the converged ones; the SVD for the rest
     GemmPlan pg;

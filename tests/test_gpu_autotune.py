@@ -42,8 +42,9 @@ def _heff(h, name, bl, br, cl, cr, d, idx, mats, L, R, x):
 
 
 # (d, chi_left, chi_right, w_left, w_right): a K-split-prone shape (few output
-# tiles, long K), a ragged one, and a spin-1 one
-SHAPES = [(2, 192, 192, 5, 5), (2, 100, 72, 5, 5), (3, 64, 96, 5, 5)]
+# tiles, long K), a ragged one, and a spin-1 one -- each GEMM above the 2e8-flop
+# tuning bound
+SHAPES = [(2, 192, 192, 5, 5), (2, 200, 176, 5, 5), (3, 144, 128, 5, 5)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
