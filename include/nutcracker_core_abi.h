@@ -297,6 +297,12 @@ int nc_set_krylov_graphs(int on);
    including the terminating zero. */
 int nc_set_autotune(int on);
 int64_t nc_autotune_table(char* buf, int64_t len);
+/* Forces one launch choice on every GEMM plan, for tests of each kernel variant
+   (no counterpart in the reference): bk 16 or 32 (K step per LDS stage) or -16
+   (16 with two LDS stages), nsplit 0 (heuristic) or 1/2/4/8, m3 -1 (the
+   precision's default), 0 (4 real products) or 1 (3); bk = 0 ends the forcing.
+   Returns -1 for values outside these. */
+int nc_gemm_force_choice(int bk, int nsplit, int m3);
 /* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,
    sites of the mixed form finished in FP64 (a complex64 solve that came out above
    the previous energy);

@@ -131,13 +131,61 @@ __device__ __forceinline__ void store_tiles(int tid, typename Cplx<T>::type (*As
     }
 }
 
-template <typename T, int OPA, int OPB, bool ACC, bool M3, int BK>
+// the MFMAs of one staged K step: the wave's 2 x 2 blocks of 16 x 16
+template <typename T, bool M3, int BK>
+__device__ __forceinline__ void mma_stage(const typename Cplx<T>::type (*As)[TBM + TPAD],
+                                          const typename Cplx<T>::type (*Bs)[TBN + TPAD],
+                                          typename Cplx<T>::v4 (&acc)[M3 ? 3 : 2][2][2], int wm, int wn, int lrow,
+                                          int lk) {
+    typedef typename Cplx<T>::type CT;
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+        const int k = kq * 4 + lk;
+        CT a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = As[k][wm + i * 16 + lrow];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = Bs[k][wn + j * 16 + lrow];
+        if constexpr (M3) {
+            T as[2], bs[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) as[i] = a[i].x + a[i].y;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bs[j] = b[j].x + b[j].y;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                    acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].y, acc[1][i][j]);
+                    acc[2][i][j] = Cplx<T>::mfma(as[i], bs[j], acc[2][i][j]);
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const T nai = -a[i].y;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                    acc[0][i][j] = Cplx<T>::mfma(nai, b[j].y, acc[0][i][j]);
+                    acc[1][i][j] = Cplx<T>::mfma(a[i].x, b[j].y, acc[1][i][j]);
+                    acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].x, acc[1][i][j]);
+                }
+            }
+        }
+    }
+}
+
+// NBUF = 1: one LDS stage, two barriers per K step.  NBUF = 2: two LDS stages;
+// the next step is stored into the other stage while this one is consumed, one
+// barrier per K step.
+template <typename T, int OPA, int OPB, bool ACC, bool M3, int BK, int NBUF>
 __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict__ batches,
                                                     const SegT<T>* __restrict__ segs) {
     typedef typename Cplx<T>::type CT;
     typedef typename Cplx<T>::v4 V4;
-    __shared__ CT As[BK][TBM + TPAD];
-    __shared__ CT Bs[BK][TBN + TPAD];
+    __shared__ CT As[NBUF][BK][TBM + TPAD];
+    __shared__ CT Bs[NBUF][BK][TBN + TPAD];
 
     // every batch has its own M x N (grouped launch of unequal problems): the
     // grid covers the largest, surplus workgroups of smaller batches exit
@@ -179,53 +227,38 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
         SegT<T> sg = segs[sidx];
         load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
     }
-    while (sidx < bt.seg_end) {
-        store_tiles<T, OPA, OPB, BK>(tid, As, Bs, ra, rb);
-        __syncthreads();
-        // advance and prefetch the next stage
-        k0 += BK;
-        if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
-        if (sidx < bt.seg_end) {
-            SegT<T> sg = segs[sidx];
-            load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
-        }
-#pragma unroll
-        for (int kq = 0; kq < BK / 4; ++kq) {
-            const int k = kq * 4 + lk;
-            CT a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[k][wm + i * 16 + lrow];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[k][wn + j * 16 + lrow];
-            if (M3) {
-                T as[2], bs[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) as[i] = a[i].x + a[i].y;
-#pragma unroll
-                for (int j = 0; j < 2; ++j) bs[j] = b[j].x + b[j].y;
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
-                        acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].y, acc[1][i][j]);
-                        acc[2][i][j] = Cplx<T>::mfma(as[i], bs[j], acc[2][i][j]);
-                    }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const T nai = -a[i].y;
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc[0][i][j] = Cplx<T>::mfma(a[i].x, b[j].x, acc[0][i][j]);
-                        acc[0][i][j] = Cplx<T>::mfma(nai, b[j].y, acc[0][i][j]);
-                        acc[1][i][j] = Cplx<T>::mfma(a[i].x, b[j].y, acc[1][i][j]);
-                        acc[1][i][j] = Cplx<T>::mfma(a[i].y, b[j].x, acc[1][i][j]);
-                    }
-                }
+    if (NBUF == 1) {
+        while (sidx < bt.seg_end) {
+            store_tiles<T, OPA, OPB, BK>(tid, As[0], Bs[0], ra, rb);
+            __syncthreads();
+            // advance and prefetch the next stage
+            k0 += BK;
+            if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
+            if (sidx < bt.seg_end) {
+                SegT<T> sg = segs[sidx];
+                load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
             }
+            mma_stage<T, M3, BK>(As[0], Bs[0], acc, wm, wn, lrow, lk);
+            __syncthreads();
         }
+    } else {
+        int cur = 0;
+        if (sidx < bt.seg_end) store_tiles<T, OPA, OPB, BK>(tid, As[0], Bs[0], ra, rb);
         __syncthreads();
+        while (sidx < bt.seg_end) {
+            k0 += BK;
+            if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
+            const bool more = sidx < bt.seg_end;
+            if (more) {
+                SegT<T> sg = segs[sidx];
+                load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
+            }
+            mma_stage<T, M3, BK>(As[cur], Bs[cur], acc, wm, wn, lrow, lk);
+            // the other stage was last read before the previous barrier
+            if (more) store_tiles<T, OPA, OPB, BK>(tid, As[cur ^ 1], Bs[cur ^ 1], ra, rb);
+            __syncthreads();
+            cur ^= 1;
+        }
     }
 
     CT* __restrict__ C = bt.C;
@@ -329,19 +362,21 @@ Arena& arena() {
     return *a;
 }
 
-template <typename T, int OPA, int OPB, int BK>
+template <typename T, int OPA, int OPB, int BK, int NBUF>
 void launch_bk(bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
 #define NC_L(ACCV, M3V) \
-    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, BK>), grid, dim3(256), 0, st, b, s)
+    hipLaunchKernelGGL((cgemm_kernel<T, OPA, OPB, ACCV, M3V, BK, NBUF>), grid, dim3(256), 0, st, b, s)
     if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
     else { if (m3) NC_L(false, true); else NC_L(false, false); }
 #undef NC_L
 }
 
+// bk: 16 or 32 (one LDS stage), or -16 (16 with two LDS stages)
 template <typename T, int OPA, int OPB>
 void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s, hipStream_t st) {
-    if (bk == 32) launch_bk<T, OPA, OPB, 32>(acc, m3, grid, b, s, st);
-    else launch_bk<T, OPA, OPB, 16>(acc, m3, grid, b, s, st);
+    if (bk == 32) launch_bk<T, OPA, OPB, 32, 1>(acc, m3, grid, b, s, st);
+    else if (bk == -16) launch_bk<T, OPA, OPB, 16, 2>(acc, m3, grid, b, s, st);
+    else launch_bk<T, OPA, OPB, 16, 1>(acc, m3, grid, b, s, st);
 }
 
 template <typename T>
@@ -407,13 +442,16 @@ struct Built {
     bool empty() const { return b.empty() || maxtiles <= 0; }
 };
 
-// Launch choices of one plan: K step staged per LDS round, K split (0 = the
-// heuristic below), complex product form (-1 = the precision's default).
+// Launch choices of one plan: K step staged per LDS round (16, 32, or -16 for
+// 16 with two LDS stages), K split (0 = the heuristic below), complex product
+// form (-1 = the precision's default).
 struct GemmChoice {
     int bk = 16;
     int nsplit = 0;
     int m3 = -1;
 };
+// a choice forced on every plan (tests of each kernel variant); bk = 0: none
+GemmChoice g_forced{0, 0, -1};
 
 // Fills per-batch sizes, chooses split-K and expands it.  `work(bytes)` gives the
 // split-K workspace.
@@ -459,7 +497,7 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
     }
     out.nsplit = nsplit;
     out.m3 = ch.m3 >= 0 ? ch.m3 != 0 : (sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32);
-    out.bk = ch.bk == 32 ? 32 : 16;
+    out.bk = (ch.bk == 32 || ch.bk == -16) ? ch.bk : 16;
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
@@ -554,6 +592,7 @@ std::vector<int> tune_id(int prec, int opA, int opB, TuneKey const& k) {
 template <typename T>
 GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<BatchT<T>> const& batches,
                   std::vector<SegT<T>> const& segs, TuneKey const& key, hipStream_t s) {
+    if (g_forced.bk != 0) return g_forced;
     if (!key.tag || accumulate || !gemm_autotune() || batches.empty()) return GemmChoice();
     const std::vector<int> id = tune_id(sizeof(T) == sizeof(double) ? 64 : 32, opA, opB, key);
     {
@@ -585,7 +624,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
     if (flops < 2e8) return GemmChoice();
     std::vector<GemmChoice> cands;
     cands.push_back(GemmChoice());
-    for (int bk : {16, 32})
+    for (int bk : {16, 32, -16})
         for (int m3 : {0, 1})
             for (int ns : {1, 2, 4, 8}) {
                 if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
@@ -731,6 +770,8 @@ bool& gemm_autotune() {
     return on;
 }
 
+void gemm_force_choice(int bk, int nsplit, int m3) { g_forced = GemmChoice{bk, nsplit, m3}; }
+
 std::string gemm_tune_table() {
     std::lock_guard<std::mutex> g(g_tune_mu);
     std::string out;
@@ -739,8 +780,9 @@ std::string gemm_tune_table() {
         auto const& k = kv.first;
         auto const& e = kv.second;
         snprintf(line, sizeof line,
-                 "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d split %d %s (%.1f us, default %.1f us, %d candidates)\n",
-                 k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9], e.choice.bk, e.choice.nsplit,
+                 "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d%s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
+                 k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9], e.choice.bk == -16 ? 16 : e.choice.bk,
+                 e.choice.bk == -16 ? " 2-stage" : "", e.choice.nsplit,
                  e.choice.m3 < 0 ? "3M/4M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
                  e.candidates);
         out += line;

@@ -134,6 +134,9 @@ struct TuneKey {
 // default off) and a text table of the choices made so far.
 bool& gemm_autotune();
 std::string gemm_tune_table();
+// force one launch choice on every GEMM plan (bk 16, 32 or -16 = 16 with two LDS
+// stages; nsplit 0 = heuristic; m3 -1 = default, 0 = 4M, 1 = 3M); bk = 0 clears
+void gemm_force_choice(int bk, int nsplit, int m3);
 
 // Host-side builder: keeps descriptor lists and uploads them per launch into a
 // small reusable device arena.

@@ -78,3 +78,40 @@ def test_tuned_matvec_against_oracle(lib, shape):
     for ln in mine:
         m = re.search(r"\(([\d.]+) us, default ([\d.]+) us, (\d+) candidates\)", ln)
         assert m and float(m.group(1)) <= float(m.group(2)) and int(m.group(3)) >= 4
+
+
+VARIANTS = [(16, 1, 0), (16, 2, 1), (32, 4, 0), (32, 1, 1), (-16, 1, 0), (-16, 8, 1), (-16, 0, -1)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_every_launch_variant_against_oracle(lib, variant):
+    """Each kernel variant the tuner may pick (K step 16 / 32 / 16 with two LDS
+    stages, K split, 3M / 4M), forced on every plan: the matvec and both
+    environment contractors on ragged shapes keep the drop-in's accuracy."""
+    core, h = lib
+    f = h.nc_gemm_force_choice
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int]
+    d, bl, br, cl, cr = 2, 100, 72, 5, 5
+    rng = np.random.default_rng(4242)
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    x = crand(rng, br, bl, d)
+    ref = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x, oc.iteration_stage_1(bl, cl, cr, d, L, idx, mats))
+    refL = oc.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, x)
+    refR = oc.contract_sos_right(bl, br, cl, cr, d, R, idx, mats, x)
+    assert f(*variant) == 0
+    try:
+        y64 = _heff(h, "nc_bench_heff", bl, br, cl, cr, d, idx, mats, L, R, x)
+        y32 = _heff(h, "nc_bench_heff_c64", bl, br, cl, cr, d, idx, mats, L, R, x)
+        newL = core.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, x)
+        newR = core.contract_sos_right(bl, br, cl, cr, d, R, idx, mats, x)
+    finally:
+        assert f(0, 0, -1) == 0
+    print("variant", variant, "fp64", rel(y64, ref), "c64", rel(y32, ref), flush=True)
+    assert rel(y64, ref) < 1e-12
+    assert 1e-9 < rel(y32, ref) < 2e-6
+    assert rel(newL, refL) < 1e-12
+    assert rel(newR, refR) < 1e-12
+    assert f(24, 0, -1) == -1 and f(16, 3, 0) == -1
