@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--config2-sweeps", type=int, default=6,
+                    help="full sweeps of the chi=256 XXZ chain (config 2), FP64 and mixed; 0 skips them")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form (64 chains) timing")
     ap.add_argument("--batch-chains", type=int, default=64)
@@ -324,6 +326,35 @@ def main():
             "fp64": {"step_s": round(s64, 4), "reset_s": round(w64["reset_s"], 2), "sites": w64["sites"]},
             "mixed": {"step_s": round(smx, 4), "reset_s": round(wmx["reset_s"], 2), "sites": wmx["sites"]},
         }
+
+    # ---- config 2: full sweeps of the chi = 256 XXZ chain, FP64 and mixed ----
+    if args.config2_sweeps > 0:
+        ops2 = xxz_operator(64, 2, 1.0, 1.0)
+        c2 = {}
+        states = {}
+        for mixed in (False, True):
+            name = "mixed" if mixed else "fp64"
+            ch = chain_mod.Chain(ops2, 256, seed=args.seed + 2, ncv=args.ncv, rethrow=True, mixed=mixed)
+            t0 = time.perf_counter()
+            for _ in range(args.config2_sweeps):
+                ch.perform_optimization_sweep()
+            t = time.perf_counter() - t0
+            st = ch.stats()
+            c2[name] = {"sweeps_s": round(t, 2), "energy": ch.energy, "matvecs_fp64": st["matvecs"],
+                        "matvecs_c64": st["matvecs32"], "site_solves": st["site_solves"]}
+            states[name] = ch
+            log("config 2 %s: %d sweeps in %.2f s, E %.12f" % (name, args.config2_sweeps, t, ch.energy))
+        ov = states["fp64"].compute_state_overlap(states["mixed"])
+        for ch in states.values():
+            ch.close()
+        c2["workload"] = ("XXZ chain J=1 Delta=1.0 N=64 d=2 MPO bond 5, chi=256, seeded right-canonical initial "
+                          "state, %d full sweeps (performOptimizationSweep) from the first site, ncv=%d"
+                          % (args.config2_sweeps, args.ncv))
+        c2["mixed_speedup"] = round(c2["fp64"]["sweeps_s"] / c2["mixed"]["sweeps_s"], 3)
+        c2["abs_dE"] = abs(c2["fp64"]["energy"] - c2["mixed"]["energy"])
+        c2["abs_dE_per_site"] = c2["abs_dE"] / 64
+        c2["state_infidelity"] = float(1 - abs(ov) ** 2)
+        result["config2_sweeps"] = c2
 
     # ---- batching form: 64 TFIM chains in lockstep ------------------------
     if not args.no_batch:

@@ -154,6 +154,73 @@ __global__ __launch_bounds__(KT) void kupdate_kernel(const KDesc* __restrict__ d
     }
 }
 
+// The first update fused with the second pass's dot products: w -= sum_{c<=j}
+// V_c coef_c, then the partial dots of V_0..V_j against the updated w and its
+// squared norm (entry j+1), in the layout of kdot_kernel.  The dots re-read the
+// rows of V the update has just read, so they come from the caches, not HBM.
+template <int MC>
+__global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ desc, int j,
+                                                    const double2* __restrict__ coef, int ldp,
+                                                    double2* __restrict__ partial) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    __shared__ double2 sc[MC];
+    __shared__ double sh[KT / 64][2 * MC + 1];
+    const int m = j + 1;
+    for (int c = threadIdx.x; c < m; c += KT) sc[c] = coef[(int64_t)p * ldp + c];
+    __syncthreads();
+    const int64_t n = d.n;
+    double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    double re[MC], im[MC], nrm = 0;
+#pragma unroll
+    for (int c = 0; c < MC; ++c) { re[c] = 0; im[c] = 0; }
+    const int nb = gridDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+        double2 v = w[i];
+        for (int c = 0; c < m; ++c) {
+            const double2 a = d.V[i + (int64_t)c * n];
+            const double2 h = sc[c];
+            v.x -= a.x * h.x - a.y * h.y;
+            v.y -= a.x * h.y + a.y * h.x;
+        }
+        w[i] = v;
+        nrm += v.x * v.x + v.y * v.y;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            if (c < m) {
+                const double2 a = d.V[i + (int64_t)c * n];
+                re[c] += a.x * v.x + a.y * v.y;
+                im[c] += a.x * v.y - a.y * v.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        re[c] = kwarp_sum(re[c]);
+        im[c] = kwarp_sum(im[c]);
+    }
+    nrm = kwarp_sum(nrm);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            sh[threadIdx.x >> 6][2 * c] = re[c];
+            sh[threadIdx.x >> 6][2 * c + 1] = im[c];
+        }
+        sh[threadIdx.x >> 6][2 * MC] = nrm;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= m; c += KT) {
+        double tr = 0, ti = 0;
+        if (c < m) {
+            for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2 * c]; ti += sh[wv][2 * c + 1]; }
+        } else {
+            for (int wv = 0; wv < KT / 64; ++wv) tr += sh[wv][2 * MC];
+        }
+        partial[((int64_t)p * ldp + c) * nb + blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
 // Hessenberg column j = coef1 + coef2, beta_j = ||w||; beta = 0 when the new
 // vector is round-off of an invariant subspace; scale[p] = 1 / beta (or 0)
 __global__ void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv, const double2* __restrict__ coef1,
@@ -459,12 +526,19 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         else hipLaunchKernelGGL(kscatter_kernel<false>, g, dim3(KT), 0, s, ddesc.ptr, iod.ptr, j);
         NC_HIP(hipGetLastError());
     };
-    // the eight Gram-Schmidt kernels of step j
+    // the Gram-Schmidt kernels of step j (classical Gram-Schmidt twice; the
+    // first update and the second pass's dots share one read of the basis)
     auto orthogonalize = [&](int j) {
         hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr);
-        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
-        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+        const int m = j + 1;
+        if (m <= 4) hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
+        else if (m <= 8) hipLaunchKernelGGL(kupdot_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
+        else if (m <= 16) hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
+        else {
+            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
+            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+        }
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr);
         hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr);
         hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr, ldp,
