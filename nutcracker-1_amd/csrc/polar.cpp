@@ -37,38 +37,6 @@ __global__ void scale_cols_kernel(double2* X, const double* sv, int64_t m, int64
 
 // out(i,j) = conj of the Hermitian part of P: 0.5 (conj(P(i,j)) + P(j,i)); one
 // thread per unordered pair, so the update is in place
-__global__ void conj_hermitian_part_kernel(double2* P, int64_t m) {
-    const int64_t total = m * m;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e % m, j = e / m;
-        if (i > j) continue;
-        const double2 a = P[i + j * m], b = P[j + i * m];
-        const double2 nij = make_double2(0.5 * (a.x + b.x), 0.5 * (b.y - a.y));
-        const double2 nji = make_double2(0.5 * (b.x + a.x), 0.5 * (a.y - b.y));
-        P[i + j * m] = nij;
-        P[j + i * m] = nji;
-    }
-}
-
-// X(i, j) += scale * u(i, j) with u uniform in [-1, 1) from a counter hash of
-// the element index (splitmix64), so the perturbation is deterministic
-__global__ void add_hash_noise_kernel(double2* X, int64_t total, double scale) {
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t z = (uint64_t)e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        const double ur = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        z = z * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        const double ui = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        double2 v = X[e];
-        X[e] = make_double2(v.x + scale * ur, v.y + scale * ui);
-    }
-}
-
 namespace {
 PolarStats g_polar_stats;
 
@@ -86,119 +54,20 @@ double ns_alpha_cap() {
     return cap;
 }
 
-// one Newton-Schulz schedule step: the lower bound l of the singular values,
-// raised by the measured error, gives alpha; returns alpha and updates l
-double ns_alpha(double& l, double err) {
-    if (err < 1.0) l = std::max(l, std::sqrt(1.0 - err));
-    double alpha = 1.0;
-    if (l < 1.0 - 1e-3) alpha = std::min(ns_alpha_cap(), std::sqrt(3.0 / (1.0 + l + l * l)));
-    const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
-    l = std::min({1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l), p1});
-    return alpha;
-}
-
-// Scaled Newton-Schulz iteration for the unitary polar factor of a wide
-// matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X  for a lower
-// bound l of the smallest singular value (X is first divided by its Frobenius
-// norm, so every singular value lies in (0, 1]).  The cubic
-// p(x) = (alpha/2) x (3 - alpha^2 x^2) peaks at p(1/alpha) = 1 and maps [l, 1]
-// onto [min(p(l), p(1)), 1].  The equioscillation scaling
-// alpha = sqrt(3 / (1 + l + l^2)) makes p(l) = p(1) ~ 2.6 l, which for small l
-// sends the largest singular values down to the round-off level and loses
-// their singular vectors; alpha is therefore capped so that p(1) >= 0.1
-// (alpha = 1.698, growth 2.55 per step for small singular values instead of
-// 2.60), which keeps every direction at least a tenth of ||X|| in size.  The
-// bound l is also raised to sqrt(1 - ||X X^H - I||_F) whenever that is larger,
-// so iterates near convergence (and nearly unitary input) take plain alpha = 1
-// steps; a scaled step there would map sigma = 1 down to p(1).
-// A site after optimisation is often numerically rank deficient (Schmidt
-// values far below round-off); its polar factor is then not determined in
-// those directions, and the iteration would stall there.  A deterministic
-// perturbation of Frobenius norm 1e-14 ||A|| floors the singular values near
-// 1e-16 ||A||, the schedule starts from l = 1e-16 and converges in about 45
-// steps for any input; the two-site state changes by ~1e-14 relative
-// (P = A U_p^H below, kept Hermitian).
-// The Gram matrix of each step gives ||X X^H - I||_F for free.  Only GEMMs of
-// this library are used.
-bool polar_newton_schulz(int m, int64_t n, const double2* A, double2* Up, hipStream_t s, int* iters_out) {
-    const double nrm = dznrm2(A, (int64_t)m * n, s);
-    if (!(nrm > 0) || !std::isfinite(nrm)) return false;
-    ZBuf X0((size_t)m * n), X1((size_t)m * n), G((size_t)m * m);
-    d2d(X0.ptr, A, (size_t)m * n * sizeof(double2));
-    const int64_t total = (int64_t)m * n;
-    // uniform entries in [-1,1) + i[-1,1) have RMS sqrt(2/3)
-    const double pscale = 1e-14 * nrm / std::sqrt((double)total) * std::sqrt(1.5);
-    {
-        unsigned gsz = (unsigned)std::min<int64_t>(4096, (total + 255) / 256);
-        hipLaunchKernelGGL(add_hash_noise_kernel, dim3(gsz), dim3(256), 0, s, X0.ptr, total, pscale);
-        NC_HIP(hipGetLastError());
-    }
-    const double nrm2 = dznrm2(X0.ptr, total, s);
-    zscal(X0.ptr, zd(1.0 / nrm2, 0), total, s);
-    double l = 1e-16;
-    const int max_iters = 64;
-    bool last = false;
-    for (int it = 0; it < max_iters; ++it) {
-        GemmPlan g;  // G = X X^H
-        g.M = m; g.N = m; g.opA = OpN; g.opB = OpC;
-        g.begin_batch(G.ptr, m);
-        g.add_seg(X0.ptr, m, X0.ptr, m, (int)n);
-        zgemm_run(g, s);
-        add_identity(G.ptr, m, zd(-1, 0), s);              // E = X X^H - I
-        const double err = dznrm2(G.ptr, (int64_t)m * m, s);
-        if (!std::isfinite(err)) return false;
-        if (last || err < 1e-15 * m) {
-            d2d(Up, X0.ptr, (size_t)m * n * sizeof(double2));
-            *iters_out = it;
-            return true;
-        }
-        // every sigma lies in [l, 1], and sigma^2 >= 1 - ||X X^H - I||_F: the
-        // measured error raises the bound, so a nearly unitary iterate takes
-        // plain (alpha = 1) steps, which keep sigma = 1 fixed
-        const double alpha = ns_alpha(l, err);
-        // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
-        zscal(G.ptr, zd(-0.5 * alpha * alpha * alpha, 0), (int64_t)m * m, s);
-        add_identity(G.ptr, m, zd(0.5 * alpha * (3.0 - alpha * alpha), 0), s);
-        GemmPlan u;  // X1 = H X0
-        u.M = m; u.N = (int)n; u.opA = OpN; u.opB = OpN;
-        u.begin_batch(X1.ptr, m);
-        u.add_seg(G.ptr, m, X0.ptr, m, m);
-        zgemm_run(u, s);
-        std::swap(X0.ptr, X1.ptr);
-        // cubic convergence: once every |sigma^2 - 1| < 1e-7 one more step reaches round-off
-        if (err < 1e-7) last = true;
-    }
-    return false;
-}
 }  // namespace
 
 PolarStats polar_stats() { return g_polar_stats; }
 
 int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s) {
-    if (m <= 0 || ncol < m) throw Error("polar_split: need 0 < m <= n", -51);
-    // the SVD is only given finite input: a non-finite matrix is reported as a
-    // failed normalisation (NormalizationError) instead
-    const double anorm = dznrm2(A, (int64_t)m * ncol, s);
-    if (!std::isfinite(anorm)) return 1000;
-    ++g_polar_stats.calls;
-    int iters = 0;
-    if (polar_newton_schulz(m, ncol, A, Up, s, &iters)) {
-        g_polar_stats.iterations += iters;
-        // P = A U_p^H is Hermitian positive semi-definite; keep its Hermitian part
-        GemmPlan p;
-        p.M = m; p.N = m; p.opA = OpN; p.opB = OpC;
-        p.begin_batch(Pconj, m);
-        p.add_seg(A, m, Up, m, (int)ncol);
-        zgemm_run(p, s);
-        unsigned gsz = (unsigned)std::min<int64_t>(4096, ((int64_t)m * m + 255) / 256);
-        hipLaunchKernelGGL(conj_hermitian_part_kernel, dim3(gsz), dim3(256), 0, s, Pconj, (int64_t)m);
-        NC_HIP(hipGetLastError());
-        return 0;
-    }
-    // rank-deficient (or numerically so) input: the SVD completes the polar
-    // factor with an orthonormal basis of the null directions, as zgesdd does
-    ++g_polar_stats.svd_fallbacks;
-    return polar_split_svd(m, ncol, A, Up, Pconj, s);
+    // the batched iteration with one job (its schedule runs on the device)
+    std::vector<PolarJob> jobs(1);
+    jobs[0].m = m;
+    jobs[0].n = ncol;
+    jobs[0].A = A;
+    jobs[0].Up = Up;
+    jobs[0].Pconj = Pconj;
+    polar_split_batch(jobs, s);
+    return jobs[0].info;
 }
 
 int polar_split_svd(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s) {
@@ -396,16 +265,22 @@ __global__ void ns_sumsq_kernel(const NsItem* __restrict__ it, const NsState* __
     if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// the schedule step of ns_alpha / polar_newton_schulz for every live job
-__global__ void ns_decide_kernel(NsState* __restrict__ st, const double* __restrict__ partial, int nb, int nq,
-                                 double cap) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
+// one step of the schedule above (error, lower bound, alpha) for every live job
+// (one 64-lane block per job: the partial sums are added lane-strided, then
+// across the lanes, in a fixed order)
+__global__ __launch_bounds__(64) void ns_decide_kernel(NsState* __restrict__ st, const double* __restrict__ partial,
+                                                       int nb, double cap) {
+    const int q = blockIdx.x;
+    if (!st[q].live) {
+        if (threadIdx.x == 0) st[q].just = 0;
+        return;
+    }
+    double e2 = 0;
+    for (int b = threadIdx.x; b < nb; b += 64) e2 += partial[(int64_t)q * nb + b];
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+    if (threadIdx.x != 0) return;
     NsState s = st[q];
     s.just = 0;
-    if (!s.live) { st[q] = s; return; }
-    double e2 = 0;
-    for (int b = 0; b < nb; ++b) e2 += partial[(int64_t)q * nb + b];
     const double err = sqrt(e2);
     if (!isfinite(err)) {
         s.phase = 3;
@@ -505,8 +380,35 @@ void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, 
     launch_items(bscale_kernel, items, maxn, s);
 }
 
+// Scaled Newton-Schulz iteration for the unitary polar factor of a wide
+// matrix X (m x n, m <= n):  X <- (alpha/2) (3 I - alpha^2 X X^H) X  for a lower
+// bound l of the smallest singular value (X is first divided by its Frobenius
+// norm, so every singular value lies in (0, 1]).  The cubic
+// p(x) = (alpha/2) x (3 - alpha^2 x^2) peaks at p(1/alpha) = 1 and maps [l, 1]
+// onto [min(p(l), p(1)), 1].  The equioscillation scaling
+// alpha = sqrt(3 / (1 + l + l^2)) makes p(l) = p(1) ~ 2.6 l, which for small l
+// sends the largest singular values down to the round-off level and loses
+// their singular vectors; alpha is therefore capped so that p(1) >= 0.1
+// (alpha = 1.698, growth 2.55 per step for small singular values instead of
+// 2.60), which keeps every direction at least a tenth of ||X|| in size.  The
+// bound l is also raised to sqrt(1 - ||X X^H - I||_F) whenever that is larger,
+// so iterates near convergence (and nearly unitary input) take plain alpha = 1
+// steps; a scaled step there would map sigma = 1 down to p(1).
+// A site after optimisation is often numerically rank deficient (Schmidt
+// values far below round-off); its polar factor is then not determined in
+// those directions, and the iteration would stall there.  A deterministic
+// perturbation of Frobenius norm 1e-14 ||A|| floors the singular values near
+// 1e-16 ||A||, the schedule starts from l = 1e-16 and converges in about 45
+// steps for any input; the two-site state changes by ~1e-14 relative
+// (P = A U_p^H below, kept Hermitian).
+// The Gram matrix of each step gives ||X X^H - I||_F for free.  Only GEMMs of
+// this library are used.
+// Non-finite input is refused (info 1000, reported as a normalisation error);
+// a job that does not converge in 64 steps (or has zero norm) is completed by
+// the SVD (rocsolver_zgesvd), which gives the null directions an orthonormal
+// basis as zgesdd does.  P = A U_p^H is Hermitian positive semi-definite; its
+// Hermitian part is kept (conjugated, as the neighbour absorbs it).
 void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
-    // the algorithm of polar_newton_schulz / polar_split, every step for all jobs
     const size_t J = jobs.size();
     struct St {
         ZBuf X0, X1, G;
@@ -609,8 +511,8 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
             hipLaunchKernelGGL(ns_eye_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
             hipLaunchKernelGGL(ns_sumsq_kernel, dim3(gs, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr,
                                partial.ptr);
-            hipLaunchKernelGGL(ns_decide_kernel, dim3((unsigned)((A + 63) / 64)), dim3(64), 0, s, dst.ptr,
-                               partial.ptr, (int)gs, (int)A, cap);
+            hipLaunchKernelGGL(ns_decide_kernel, dim3((unsigned)A), dim3(64), 0, s, dst.ptr, partial.ptr, (int)gs,
+                               cap);
             hipLaunchKernelGGL(ns_copy_kernel, dim3(gc, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr, p);
             hipLaunchKernelGGL(ns_h_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
             NC_HIP(hipGetLastError());
