@@ -477,9 +477,21 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
         }
         h2d(dst.ptr, hs.data(), A * sizeof(NsState));
         const int* live0 = &dst.ptr[0].live;
+        // tuning keys: the largest site of the batch and the job count
+        TuneKey kg;
+        kg.tag = 'g';
+        kg.count = (int)A;
+        for (size_t q : act) {
+            kg.bl = std::max(kg.bl, jobs[q].m);
+            kg.br = std::max<int>(kg.br, (int)jobs[q].n);
+        }
+        TuneKey ku = kg;
+        ku.tag = 'u';
         for (int p = 0; p < 2; ++p) {
             gram[p].M = gram[p].N = 1; gram[p].opA = OpN; gram[p].opB = OpC;
             upd[p].M = upd[p].N = 1; upd[p].opA = OpN; upd[p].opB = OpN;
+            gram[p].key = kg;
+            upd[p].key = ku;
             for (size_t k = 0; k < A; ++k) {
                 const size_t q = act[k];
                 const int m = jobs[q].m;
@@ -492,6 +504,9 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
                 upd[p].add_seg(st[q].G.ptr, m, Xp, m, m);
             }
         }
+        // (a first prepare may tune the plan by running it: gram plans write only
+        // G, upd[0] only X1, both scratch here; upd[1], which writes X0, then
+        // finds the choice made for upd[0] -- same key -- and does not run)
         std::unique_ptr<PreparedLaunch> pg[2] = {prepare_gemm(gram[0]), prepare_gemm(gram[1])};
         std::unique_ptr<PreparedLaunch> pu[2] = {prepare_gemm(upd[0]), prepare_gemm(upd[1])};
         DevBuf<NsItem> ditems(A);
