@@ -276,6 +276,9 @@ typedef struct nc_chain_batch nc_chain_batch;
 nc_chain_batch* nc_chain_batch_create(nc_chain* const* chains, uint32_t n);
 void nc_chain_batch_destroy(nc_chain_batch* batch);
 int nc_chain_batch_sweep(nc_chain_batch* batch);
+/* nc_chain_reset of every chain of the batch with grouped launches (all right
+   normalisations in one batched polar split); the chains hold raw sites. */
+int nc_chain_batch_reset(nc_chain_batch* batch);
 int nc_chain_batch_sweep_until_converged(nc_chain_batch* batch);
 /* wall time (6 values, ms) spent in lockstep moves [0] and site solves [1]; the
    site solves split into per-chain preparation [2], the batched solves [3] and

@@ -196,6 +196,123 @@ void ChainBatch::env_left_batch(std::vector<Chain*> const& cs, hipStream_t s) {
 }
 
 
+// reset (chain.cpp:175-237) of every chain at once: each site i >= 1 is replaced
+// by its right-normalised polar factor (independently of the others, as
+// Chain::reset does), so all of them -- every site of every chain -- are one
+// batched polar split; the right environments then follow site by site with
+// grouped contractions, the first sites are normalised and the energies read
+// with batched reductions.
+void ChainBatch::reset_all() {
+    hipStream_t s = Context::get().stream;
+    bool plain = true;  // complex64 environments take the per-chain path
+    for (Chain* c : chains)
+        if (c->opts.mixed && c->opts.env32) plain = false;
+    if (!plain) {
+        for (Chain* c : chains) c->reset();
+        return;
+    }
+    const double t0 = now_ms();
+    const int N = chains[0]->N;
+    for (Chain* c : chains) {
+        c->right_envs.clear();
+        c->left_envs.clear();
+        c->right_envs.resize(N + 1);
+        c->left_envs.resize(N + 1);
+        c->right_envs[N].resize(1);
+        d2d(c->right_envs[N].ptr, c->trivial.ptr, sizeof(double2));
+        c->left_envs[0].resize(1);
+        d2d(c->left_envs[0].ptr, c->trivial.ptr, sizeof(double2));
+        c->left_ovl.clear();
+        c->right_ovl.clear();
+    }
+    // every right normalisation in one batched polar split
+    std::vector<PolarJob> jobs;
+    std::vector<ZBuf> W, Up, Pc;
+    struct Where { Chain* c; int i; };
+    std::vector<Where> where;
+    for (Chain* c : chains)
+        for (int i = N - 1; i >= 1; --i) {
+            const int bl = c->bonds[i], br = c->bonds[i + 1], d = c->ops[i]->d;
+            if ((int64_t)br * d < bl) throw Error("Not enough degrees of freedom to normalize.", -61);
+            where.push_back(Where{c, i});
+        }
+    W.resize(where.size());
+    Up.resize(where.size());
+    Pc.resize(where.size());
+    for (size_t k = 0; k < where.size(); ++k) {
+        Chain* c = where[k].c;
+        const int i = where[k].i;
+        const int bl = c->bonds[i], br = c->bonds[i + 1], d = c->ops[i]->d;
+        const size_t n = (size_t)bl * br * d;
+        W[k].resize(n);
+        Up[k].resize(n);
+        Pc[k].resize((size_t)bl * bl);
+        permute_102(c->sites[i].ptr, W[k].ptr, br, bl, d, s);
+        jobs.push_back(PolarJob{bl, (int64_t)br * d, W[k].ptr, Up[k].ptr, Pc[k].ptr, 0});
+    }
+    polar_split_batch(jobs, s);
+    for (size_t k = 0; k < where.size(); ++k) {
+        if (jobs[k].info != 0) throw Error("NormalizationError", jobs[k].info);
+        Chain* c = where[k].c;
+        const int i = where[k].i;
+        permute_102(Up[k].ptr, c->sites[i].ptr, c->bonds[i], c->bonds[i + 1], c->ops[i]->d, s);
+    }
+    W.clear();
+    Up.clear();
+    Pc.clear();
+    // right environments, site by site for all chains
+    for (int i = N - 1; i >= 1; --i) {
+        std::vector<EnvRightItem> items;
+        for (Chain* c : chains) {
+            MpoSite const& op = *c->ops[i];
+            c->right_envs[i].resize((size_t)c->bonds[i] * c->bonds[i] * op.cl);
+            items.push_back(EnvRightItem{&op, c->bonds[i], c->bonds[i + 1], c->right_envs[i + 1].ptr,
+                                         c->sites[i].ptr, c->right_envs[i].ptr, &c->ws});
+        }
+        grouped_env_right(items, s);
+    }
+    // first sites divided by their norms (core.f95:2661-2671)
+    std::vector<const double2*> x0;
+    std::vector<double2*> x0w;
+    std::vector<int64_t> n0;
+    for (Chain* c : chains) {
+        x0.push_back(c->sites[0].ptr);
+        x0w.push_back(c->sites[0].ptr);
+        n0.push_back((int64_t)c->site_size(0));
+    }
+    const std::vector<double> ss = sumsq_batch(x0, n0, s);
+    std::vector<double> f;
+    for (double v : ss) f.push_back(1.0 / std::sqrt(v));
+    scale_batch(x0w, n0, f, s);
+    // energies: the expectation value at the first site
+    std::vector<EnvRightItem> items;
+    std::vector<ZBuf> Ns(chains.size());
+    for (size_t q = 0; q < chains.size(); ++q) {
+        Chain* c = chains[q];
+        MpoSite const& op = *c->ops[0];
+        Ns[q].resize((size_t)c->bonds[0] * c->bonds[0] * op.cl);
+        items.push_back(EnvRightItem{&op, c->bonds[0], c->bonds[1], c->right_envs[1].ptr, c->sites[0].ptr,
+                                     Ns[q].ptr, &c->ws});
+    }
+    grouped_env_right(items, s);
+    std::vector<RedItem> tr;
+    for (size_t q = 0; q < chains.size(); ++q) {
+        Chain* c = chains[q];
+        tr.push_back(RedItem{Ns[q].ptr, c->left_envs[0].ptr, (int64_t)c->bonds[0], (int64_t)c->ops[0]->cl, true});
+    }
+    const std::vector<zd> e = reduce_batch(tr, s);
+    const double share = (now_ms() - t0) / chains.size();
+    for (size_t q = 0; q < chains.size(); ++q) {
+        Chain* c = chains[q];
+        c->current = 0;
+        c->optimized = false;
+        const double imag_tol = c->opts.mixed && c->opts.env32 ? 1e-6 : 1e-10;
+        if (std::abs(e[q].imag()) / std::abs(e[q]) > imag_tol) throw Error("InitialChainEnergyNotRealError", -62);
+        c->energy = e[q].real();
+        c->stats.t_reset_ms += share;
+    }
+}
+
 void ChainBatch::env_right_batch(std::vector<Chain*> const& cs, hipStream_t s) {
     // R_i = contract_sos_right(R_{i+1}, site i), i = current + 1 (after the move)
     std::vector<EnvRightItem> items;

@@ -235,6 +235,13 @@ int nc_chain_batch_times(nc_chain_batch* b, double* times_ms) {
     });
 }
 
+int nc_chain_batch_reset(nc_chain_batch* b) {
+    return guarded_chain("nc_chain_batch_reset", [&] {
+        b->batch->reset_all();
+        return 0;
+    });
+}
+
 int nc_chain_batch_sweep(nc_chain_batch* b) {
     return guarded_chain("nc_chain_batch_sweep", [&] { b->batch->sweep_all(); return 0; });
 }

@@ -343,7 +343,19 @@ struct Arena {
             offsets[i] = off;
             off = (off + sizes[i] + 255) & ~size_t(255);
         }
-        if (off > kSlot) throw Error("GEMM descriptor list too long", -70);
+        if (off > kSlot) {
+            // a list larger than a ring slot: one device buffer of its own, written
+            // by a stream-ordered copy (earlier launches reading it have finished
+            // before the copy runs)
+            std::vector<char> h(off);
+            for (int i = 0; i < nparts; ++i)
+                if (sizes[i]) std::memcpy(h.data() + offsets[i], parts[i], sizes[i]);
+            if (big.n < off) big.resize(off);
+            NC_HIP(hipMemcpyAsync(big.ptr, h.data(), off, hipMemcpyHostToDevice, s));
+            NC_HIP(hipStreamSynchronize(s));
+            *slot_out = -1;
+            return big.ptr;
+        }
         int slot = next;
         next = (next + 1) % kSlots;
         NC_HIP(hipEventSynchronize(done[slot]));
@@ -355,7 +367,10 @@ struct Arena {
         *slot_out = slot;
         return d;
     }
-    void release(int slot, hipStream_t s) { NC_HIP(hipEventRecord(done[slot], s)); }
+    void release(int slot, hipStream_t s) {
+        if (slot >= 0) NC_HIP(hipEventRecord(done[slot], s));
+    }
+    DevBuf<char> big;  // lists larger than a slot
 };
 Arena& arena() {
     static Arena* a = new Arena();

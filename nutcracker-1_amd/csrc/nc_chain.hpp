@@ -134,6 +134,9 @@ struct ChainBatch {
     void sweep(std::vector<Chain*> const& active);
     void sweep_all();
     void sweep_until_converged();
+    // Chain::reset of every chain with grouped launches (the chains hold raw
+    // sites, as after nc_chain_set_raw_site)
+    void reset_all();
 };
 
 std::vector<int> bandwidth_sequence(int requested, std::vector<int> const& dims);

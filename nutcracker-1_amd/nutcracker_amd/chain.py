@@ -41,6 +41,7 @@ _CHAIN_SIGS = {
     "nc_chain_batch_sweep": (C.c_int, [C.c_void_p]),
     "nc_chain_batch_sweep_until_converged": (C.c_int, [C.c_void_p]),
     "nc_chain_batch_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "nc_chain_batch_reset": (C.c_int, [C.c_void_p]),
     "nc_chain_set_growth": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "nc_chain_increase_bandwidth": (C.c_int, [C.c_void_p, C.c_uint32]),
     "nc_chain_optimize_chain": (C.c_int, [C.c_void_p]),
@@ -92,7 +93,7 @@ class Chain:
                  sweep_tol: float = 1e-12, sanity: float = 1e-12, max_iterations: int = 10000,
                  which: str = "SR", ncv: int = 3, rethrow: bool = False, mixed: bool = False,
                  env32: bool = False, tol32: float = 1e-7, polish: bool = False, grow: bool = False,
-                 chain_tol: float = 1e-12):
+                 chain_tol: float = 1e-12, reset: bool = True):
         self.ops = list(ops)
         self.n = len(self.ops)
         dims = [o.d for o in self.ops]
@@ -122,7 +123,8 @@ class Chain:
             if a.shape != (self.bonds[i + 1], self.bonds[i], dims[i]):
                 raise ValueError("raw site %d has shape %s" % (i, a.shape))
             self._check(lib.nc_chain_set_raw_site(self._h, i, a.ctypes.data))
-        self.reset()
+        if reset:  # reset=False leaves the raw sites for ChainBatch(..., reset=True)
+            self.reset()
 
     def _check(self, rc, failure_ok=False):
         if rc != 0:
@@ -254,12 +256,20 @@ class ChainBatch:
     solves batched into grouped launches (nc_chain_batch_* in the C-ABI).  The
     chains remain usable on their own; keep them alive while the batch is."""
 
-    def __init__(self, chains: Sequence[Chain]):
+    def __init__(self, chains: Sequence[Chain], reset: bool = False):
+        """``reset=True`` resets every chain with grouped launches (the chains
+        were made with ``Chain(..., reset=False)`` and hold their raw sites)."""
         self.chains = list(chains)
         arr = (C.c_void_p * len(self.chains))(*[c._h for c in self.chains])
         self._h = _lib().nc_chain_batch_create(arr, len(self.chains))
         if not self._h:
             raise RuntimeError(core.last_error())
+        if reset:
+            self.reset()
+
+    def reset(self):
+        """Chain.reset of every chain at once (nc_chain_batch_reset)."""
+        self._check(_lib().nc_chain_batch_reset(self._h))
 
     def _check(self, rc):
         if rc != 0:

@@ -109,3 +109,25 @@ def test_graph_replay_matches_direct_launches(mods, mixed):
             chain.set_krylov_graphs(prev)
     print("graphs", runs[0], "direct", runs[1], flush=True)
     assert runs[0] == runs[1]
+
+
+def test_batched_reset_matches_per_chain_reset(mods):
+    """ChainBatch(..., reset=True): every right normalisation of every chain in one
+    batched polar split gives the states, environments and energies of
+    Chain.reset (chain.cpp:175-237) one chain at a time."""
+    chain, ops_mod = mods
+    n = 12
+    a = _chains(chain, ops_mod, n, SPEC)
+    b = _chains(chain, ops_mod, n, SPEC, reset=False)
+    batch = chain.ChainBatch(b, reset=True)
+    for ca, cb in zip(a, b):
+        assert abs(ca.energy - cb.energy) <= 1e-12 * abs(ca.energy)
+        for i in range(n):
+            assert np.allclose(ca.site(i), cb.site(i), atol=1e-12, rtol=0)
+    # and the batch sweeps on from there like the alone chains
+    batch.sweep_until_converged()
+    for c in a:
+        c.sweep_until_converged()
+    for ca, cb in zip(a, b):
+        assert abs(ca.energy - cb.energy) <= 1e-10 * abs(ca.energy)
+    batch.close()
