@@ -150,6 +150,40 @@ def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
     return ch, {"reset_s": t_reset, "sites": sites}
 
 
+def full_sweep(chain_mod, ops, chi, seed, ncv, mixed):
+    """performOptimizationSweep (chain.cpp:147-163) from the first site, site by
+    site so that every step is timed and printed: right to the last site, then
+    back to the first."""
+    name = "mixed" if mixed else "fp64"
+    t0 = time.perf_counter()
+    ch = chain_mod.Chain(ops, chi, seed=seed, ncv=ncv, rethrow=True, mixed=mixed)
+    t_reset = time.perf_counter() - t0
+    log(name, "full sweep: reset %.2f s" % t_reset)
+    n = ch.n
+    t0 = time.perf_counter()
+    steps = 0
+
+    def opt(tag):
+        ts = time.perf_counter()
+        ch.optimize_site()
+        log(name, "%s site %d: %.3f s, E %.12f" % (tag, ch.stats()["site_solves"], time.perf_counter() - ts,
+                                                   ch.energy))
+
+    opt("right")
+    for _ in range(n - 1):
+        ch.move(1)
+        opt("right")
+        steps += 1
+    for _ in range(n - 1):
+        ch.move(-1)
+        opt("left")
+        steps += 1
+    t = time.perf_counter() - t0
+    st = ch.stats()
+    return ch, {"sweep_s": round(t, 2), "reset_s": round(t_reset, 2), "energy": ch.energy,
+                "site_solves": st["site_solves"], "matvecs_fp64": st["matvecs"], "matvecs_c64": st["matvecs32"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +198,8 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--full-sweep", action="store_true",
+                    help="time one whole chi=1024 sweep in FP64 and in the mixed form (about 7 minutes)")
     ap.add_argument("--config2-sweeps", type=int, default=6,
                     help="full sweeps of the chi=256 XXZ chain (config 2), FP64 and mixed; 0 skips them")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
@@ -326,6 +362,30 @@ def main():
             "fp64": {"step_s": round(s64, 4), "reset_s": round(w64["reset_s"], 2), "sites": w64["sites"]},
             "mixed": {"step_s": round(smx, 4), "reset_s": round(wmx["reset_s"], 2), "sites": wmx["sites"]},
         }
+
+    # ---- optional: one whole chi = 1024 sweep in both forms -----------------
+    if args.full_sweep:
+        ops = xxz_operator(args.sites, d, 1.0, args.delta)
+        res = {}
+        chs = {}
+        for mixed in (False, True):
+            ch, r = full_sweep(chain_mod, ops, chi, args.seed + 1, args.ncv, mixed)
+            res["mixed" if mixed else "fp64"] = r
+            chs["mixed" if mixed else "fp64"] = ch
+        ov = chs["fp64"].compute_state_overlap(chs["mixed"])
+        for ch in chs.values():
+            ch.close()
+        res["mixed_speedup"] = round(res["fp64"]["sweep_s"] / res["mixed"]["sweep_s"], 3)
+        res["abs_dE"] = abs(res["fp64"]["energy"] - res["mixed"]["energy"])
+        res["abs_dE_per_site"] = res["abs_dE"] / args.sites
+        res["state_infidelity"] = float(1 - abs(ov) ** 2)
+        res["workload"] = ("XXZ chain J=1 Delta=%g N=%d d=2 MPO bond 5 at chi=%d: one whole performOptimizationSweep "
+                           "(right to the last site and back) from the seeded right-canonical state, ncv=%d"
+                           % (args.delta, args.sites, chi, args.ncv))
+        log("full sweep: fp64 %.1f s, mixed %.1f s (%.2fx), |dE| %.2e, infidelity %.2e"
+            % (res["fp64"]["sweep_s"], res["mixed"]["sweep_s"], res["mixed_speedup"], res["abs_dE"],
+               res["state_infidelity"]))
+        result["full_sweep"] = res
 
     # ---- config 2: full sweeps of the chi = 256 XXZ chain, FP64 and mixed ----
     if args.config2_sweeps > 0:
