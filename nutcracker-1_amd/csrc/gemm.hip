@@ -18,6 +18,7 @@
 // (neighbouring tiles share rows of A or columns of B in that XCD's L2).
 #include "nc_internal.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -516,6 +517,13 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
+        if (!uniform) {
+            // largest outputs first: workgroups are dispatched in batch order, so
+            // the tail of the launch is made of the small batches' tiles
+            std::stable_sort(out.b.begin(), out.b.end(), [](BatchT<T> const& x, BatchT<T> const& y) {
+                return (int64_t)x.M * x.N > (int64_t)y.M * y.N;
+            });
+        }
         return;
     }
     // split-K: partial products into a workspace, then a reduction into C
