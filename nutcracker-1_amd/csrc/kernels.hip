@@ -197,46 +197,6 @@ std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<float2>> const& 
     return prepare_mix_t(items, mixw32);
 }
 
-template <typename A, typename B>
-struct ConvDesc {
-    const A* src;
-    B* dst;
-    int64_t n;
-};
-__device__ __forceinline__ float2 conv(double2 v) { return make_float2((float)v.x, (float)v.y); }
-__device__ __forceinline__ double2 conv(float2 v) { return make_double2((double)v.x, (double)v.y); }
-template <typename A, typename B>
-__global__ void convert_batch_kernel(const ConvDesc<A, B>* __restrict__ ds) {
-    const ConvDesc<A, B> c = ds[blockIdx.y];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (int64_t)gridDim.x * blockDim.x)
-        c.dst[i] = conv(c.src[i]);
-}
-template <typename A, typename B>
-static void convert_batch(std::vector<const A*> const& src, std::vector<B*> const& dst, std::vector<int64_t> const& n,
-                          hipStream_t s) {
-    if (src.empty()) return;
-    std::vector<ConvDesc<A, B>> ds(src.size());
-    int64_t maxn = 0;
-    for (size_t q = 0; q < src.size(); ++q) {
-        ds[q] = ConvDesc<A, B>{src[q], dst[q], n[q]};
-        maxn = std::max(maxn, n[q]);
-    }
-    if (maxn == 0) return;
-    int slot;
-    auto* dd = (const ConvDesc<A, B>*)stage_descriptors(ds.data(), ds.size() * sizeof(ConvDesc<A, B>), s, &slot);
-    const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, 4096 / (int64_t)src.size()), (maxn + 255) / 256);
-    hipLaunchKernelGGL((convert_batch_kernel<A, B>), dim3(g, (unsigned)src.size()), dim3(256), 0, s, dd);
-    NC_HIP(hipGetLastError());
-    release_descriptors(slot, s);
-}
-void to_c64_batch(std::vector<const double2*> const& src, std::vector<float2*> const& dst,
-                  std::vector<int64_t> const& n, hipStream_t s) {
-    convert_batch<double2, float2>(src, dst, n, s);
-}
-void to_c128_batch(std::vector<const float2*> const& src, std::vector<double2*> const& dst,
-                   std::vector<int64_t> const& n, hipStream_t s) {
-    convert_batch<float2, double2>(src, dst, n, s);
-}
 
 __global__ void to_c64_kernel(const double2* __restrict__ a, float2* __restrict__ b, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -427,13 +387,6 @@ __global__ void scale_rows_kernel(double2* __restrict__ X, const double* __restr
     }
 }
 
-__global__ void add_identity_kernel(double2* __restrict__ X, int64_t m, double2 a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) {
-        double2 v = X[i + i * m];
-        X[i + i * m] = make_double2(v.x + a.x, v.y + a.y);
-    }
-}
 
 static unsigned grid_for(int64_t n) {
     int64_t g = (n + 255) / 256;
@@ -583,11 +536,6 @@ void scale_rows(double2* X, const double* sv, int64_t m, int64_t ncol, hipStream
     NC_HIP(hipGetLastError());
 }
 
-void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s) {
-    hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, X, m,
-                       make_double2(alpha.real(), alpha.imag()));
-    NC_HIP(hipGetLastError());
-}
 
 // uniform entries in [-1, 1) + i [-1, 1) from a counter-based hash (splitmix64)
 __global__ void fill_random_kernel(double2* x, int64_t n, uint64_t key) {

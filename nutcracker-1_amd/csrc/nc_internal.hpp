@@ -244,11 +244,6 @@ std::unique_ptr<PreparedLaunch> prepare_mix(std::vector<MixItem<float2>> const& 
 // b_q = 0 does not read y_q)
 void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::vector<const double2*> const& x,
                  std::vector<zd> const& b, std::vector<int64_t> const& n, hipStream_t s);
-// element-wise precision changes of several vectors in one launch
-void to_c64_batch(std::vector<const double2*> const& src, std::vector<float2*> const& dst,
-                  std::vector<int64_t> const& n, hipStream_t s);
-void to_c128_batch(std::vector<const float2*> const& src, std::vector<double2*> const& dst,
-                   std::vector<int64_t> const& n, hipStream_t s);
 
 void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q,
                 int64_t rs, hipStream_t s);
@@ -330,7 +325,6 @@ void conj_inplace(double2* X, int64_t n, hipStream_t s);
 // deterministic pseudo-random entries, uniform in [-1, 1) + i [-1, 1), from `key`
 void fill_random(double2* x, int64_t n, uint64_t key, hipStream_t s);
 // add alpha * I to an (m x m) matrix
-void add_identity(double2* X, int64_t m, zd alpha, hipStream_t s);
 // sum_k sum_ij A(i, j, k) B(j, i, k)
 zd trace_pair(const double2* A, const double2* B, int64_t b, int64_t c, hipStream_t s);
 
