@@ -71,6 +71,33 @@ template <typename T, int OPA, int OPB, int BK>
 __device__ __forceinline__ void load_tiles(const SegT<T>& sg, int k0, int m0, int n0, int M, int N, int tid,
                                            typename Cplx<T>::type (&ra)[BK / 4], typename Cplx<T>::type (&rb)[BK / 4]) {
     typedef typename Cplx<T>::type CT;
+    // global memory (no flat addressing), loaded as a native two-element vector
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    typedef const V2 __attribute__((address_space(1)))* GP;
+    if (m0 + 64 <= M && n0 + 64 <= N && k0 + BK <= sg.K) {
+        // interior tile: every element in range, no per-element checks
+        const GP A = (GP)sg.A;
+        const GP B = (GP)sg.B;
+#pragma unroll
+        for (int it = 0; it < BK / 4; ++it) {
+            const int id = tid + 256 * it;
+            V2 v;
+            if (OPA == OpN) v = A[(int64_t)(m0 + (id & 63)) + (int64_t)(k0 + (id >> 6)) * sg.lda];
+            else v = A[(int64_t)(k0 + id % BK) + (int64_t)(m0 + id / BK) * sg.lda];
+            ra[it].x = v.x;
+            ra[it].y = OPA == OpC ? -v.y : v.y;
+        }
+#pragma unroll
+        for (int it = 0; it < BK / 4; ++it) {
+            const int id = tid + 256 * it;
+            V2 v;
+            if (OPB == OpN) v = B[(int64_t)(k0 + id % BK) + (int64_t)(n0 + id / BK) * sg.ldb];
+            else v = B[(int64_t)(n0 + (id & 63)) + (int64_t)(k0 + (id >> 6)) * sg.ldb];
+            rb[it].x = v.x;
+            rb[it].y = OPB == OpC ? -v.y : v.y;
+        }
+        return;
+    }
 #pragma unroll
     for (int it = 0; it < BK / 4; ++it) {
         const int id = tid + 256 * it;
@@ -224,8 +251,11 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
     CT ra[BK / 4], rb[BK / 4];
     int sidx = bt.seg_begin;
     int k0 = 0;
+    // the current segment stays in registers; it is re-read only when K moves
+    // on to the next one
+    SegT<T> sg;
     if (sidx < bt.seg_end) {
-        SegT<T> sg = segs[sidx];
+        sg = segs[sidx];
         load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
     }
     if (NBUF == 1) {
@@ -234,11 +264,12 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
             __syncthreads();
             // advance and prefetch the next stage
             k0 += BK;
-            if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
-            if (sidx < bt.seg_end) {
-                SegT<T> sg = segs[sidx];
-                load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
+            if (k0 >= sg.K) {
+                ++sidx;
+                k0 = 0;
+                if (sidx < bt.seg_end) sg = segs[sidx];
             }
+            if (sidx < bt.seg_end) load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
             mma_stage<T, M3, BK>(As[0], Bs[0], acc, wm, wn, lrow, lk);
             __syncthreads();
         }
@@ -248,12 +279,13 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
         __syncthreads();
         while (sidx < bt.seg_end) {
             k0 += BK;
-            if (k0 >= segs[sidx].K) { ++sidx; k0 = 0; }
-            const bool more = sidx < bt.seg_end;
-            if (more) {
-                SegT<T> sg = segs[sidx];
-                load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
+            if (k0 >= sg.K) {
+                ++sidx;
+                k0 = 0;
+                if (sidx < bt.seg_end) sg = segs[sidx];
             }
+            const bool more = sidx < bt.seg_end;
+            if (more) load_tiles<T, OPA, OPB, BK>(sg, k0, m0, n0, M, N, tid, ra, rb);
             mma_stage<T, M3, BK>(As[cur], Bs[cur], acc, wm, wn, lrow, lk);
             // the other stage was last read before the previous barrier
             if (more) store_tiles<T, OPA, OPB, BK>(tid, As[cur ^ 1], Bs[cur ^ 1], ra, rb);
