@@ -19,6 +19,7 @@
 #include "nc_internal.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -65,6 +66,8 @@ struct BatchT {
     int32_t seg_begin, seg_end;
     int32_t M, N;      // output size of this batch (filled in on the host)
     const int* live;   // null, or a device flag: 0 = skip this batch in this launch
+    int32_t upper = 0;  // 1: only the tiles on and above the diagonal (Hermitian C)
+    int32_t _pad = 0;
 };
 
 template <typename T, int OPA, int OPB, int BK>
@@ -226,9 +229,12 @@ __global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict_
     if (w >= ntiles) return;
     // XCD-aware tile order: workgroup w runs on XCD w % 8; give each XCD a
     // contiguous range of tiles (row-major over the tile grid)
+    // (not for an upper-triangle launch: its working tiles crowd the first rows,
+    // and the plain order spreads them over the XCDs)
     int tile = w;
-    if ((ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    if (!bt.upper && (ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
     const int tm = tile / tiles_n, tn = tile % tiles_n;
+    if (bt.upper && tm > tn) return;
 
     const int m0 = tm * TBM, n0 = tn * TBN;
     const int tid = threadIdx.x;
@@ -573,6 +579,7 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
             x.M = M;
             x.N = N;
             x.live = batches[b].live;
+            x.upper = batches[b].upper;
             x.seg_begin = (int32_t)out.sg.size();
             for (auto const& sg : chunks[q]) out.sg.push_back(sg);
             x.seg_end = (int32_t)out.sg.size();
@@ -856,6 +863,7 @@ void release_descriptors(int slot, hipStream_t s) { arena().release(slot, s); }
 void zgemm_run(GemmPlan const& plan, hipStream_t s) {
     static_assert(sizeof(BatchT<double>) == sizeof(GemmBatch), "descriptor layout");
     static_assert(sizeof(SegT<double>) == sizeof(GemmSeg), "descriptor layout");
+    static_assert(offsetof(BatchT<double>, upper) == offsetof(GemmBatch, upper), "descriptor layout");
     std::vector<BatchT<double>> b(plan.batches.size());
     std::vector<SegT<double>> sg(plan.segs.size());
     std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
@@ -864,7 +872,8 @@ void zgemm_run(GemmPlan const& plan, hipStream_t s) {
 }
 
 static void to_float_lists(CGemmPlan const& plan, std::vector<BatchT<float>>& b, std::vector<SegT<float>>& sg) {
-    for (auto const& x : plan.batches) b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N, x.live});
+    for (auto const& x : plan.batches)
+        b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N, x.live, x.upper, 0});
     for (auto const& x : plan.segs) sg.push_back(SegT<float>{x.A, x.B, x.lda, x.ldb, x.K, 0});
 }
 

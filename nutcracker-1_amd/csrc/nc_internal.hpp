@@ -107,6 +107,8 @@ struct GemmBatch {
     int32_t seg_begin, seg_end;
     int32_t M, N;             // this batch's output size; 0 = the plan's M, N
     const int* live = nullptr;  // device flag read at launch: 0 = skip this batch
+    int32_t upper = 0;          // 1: C is Hermitian, only its tiles on and above the
+    int32_t _pad = 0;           //    diagonal are computed (the rest is left as is)
 };
 
 enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
@@ -173,6 +175,8 @@ struct CGemmBatch {
     int32_t seg_begin, seg_end;
     int32_t M, N;               // 0 = the plan's M, N
     const int* live = nullptr;  // device flag read at launch: 0 = skip this batch
+    int32_t upper = 0;          // as in GemmBatch
+    int32_t _pad = 0;
 };
 struct CGemmPlan {
     std::vector<CGemmBatch> batches;
