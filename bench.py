@@ -142,11 +142,14 @@ def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
         t2 = time.perf_counter()
         s1 = ch.stats()
         rec = {"site": start + w, "optimize_s": t1 - t0, "move_s": t2 - t1,
+               "move_polar_ms": s1["t_move_ms"] - s0["t_move_ms"], "move_env_ms": s1["t_env_ms"] - s0["t_env_ms"],
                "matvecs_fp64": s1["matvecs"] - s0["matvecs"], "matvecs_c64": s1["matvecs32"] - s0["matvecs32"],
                "energy": ch.energy}
         sites.append(rec)
-        log(name, "site %d: optimize %.3f s, move %.3f s, matvecs fp64 %d c64 %d, E %.15f"
-            % (rec["site"], rec["optimize_s"], rec["move_s"], rec["matvecs_fp64"], rec["matvecs_c64"], rec["energy"]))
+        log(name, "site %d: optimize %.3f s, move %.3f s (polar split %.1f ms, environment %.1f ms), "
+            "matvecs fp64 %d c64 %d, E %.15f"
+            % (rec["site"], rec["optimize_s"], rec["move_s"], rec["move_polar_ms"], rec["move_env_ms"],
+               rec["matvecs_fp64"], rec["matvecs_c64"], rec["energy"]))
     return ch, {"reset_s": t_reset, "sites": sites}
 
 
