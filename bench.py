@@ -163,6 +163,7 @@ def full_sweep(chain_mod, ops, chi, seed, ncv, mixed):
     t_reset = time.perf_counter() - t0
     log(name, "full sweep: reset %.2f s" % t_reset)
     n = ch.n
+    st0 = ch.stats()
     t0 = time.perf_counter()
     steps = 0
 
@@ -183,7 +184,12 @@ def full_sweep(chain_mod, ops, chi, seed, ncv, mixed):
         steps += 1
     t = time.perf_counter() - t0
     st = ch.stats()
+    part = {k: round((st[k] - st0[k]) / 1e3, 2) for k in ("t_optimize_ms", "t_move_ms", "t_env_ms")}
+    log(name, "full sweep %.1f s: site solves %.1f s, moves %.1f s (polar splits %.1f s, environments %.1f s)"
+        % (t, part["t_optimize_ms"], part["t_move_ms"] + part["t_env_ms"], part["t_move_ms"], part["t_env_ms"]))
     return ch, {"sweep_s": round(t, 2), "reset_s": round(t_reset, 2), "energy": ch.energy,
+                "site_solves_s": part["t_optimize_ms"], "moves_polar_s": part["t_move_ms"],
+                "moves_env_s": part["t_env_ms"],
                 "site_solves": st["site_solves"], "matvecs_fp64": st["matvecs"], "matvecs_c64": st["matvecs32"]}
 
 
