@@ -26,7 +26,10 @@ dimension chi = 1024.
     python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-batch] [--no-cpu]
 
 Prints one JSON line.  The roofline object describes the dominant kernel of the
-FP64 path (the grouped complex FP64 GEMM, two launches per matvec);
+FP64 path (the grouped complex FP64 GEMM, two launches per matvec): `achieved`
+counts the algorithmic 8 real flops per complex multiply-add; the launches run
+Gauss's 3-multiply product, which executes 3/4 of them on the matrix cores, so
+`mfma_tflops` / `mfma_frac` give the work the matrix cores actually do;
 cpu_baseline times the oracle's restatement of the reference's strategy-3 matvec
 (dense stage-1 tensor + two zgemms through OpenBLAS) on a bounded sample on the
 host cores.
@@ -119,6 +122,21 @@ def time_matvec(core, lib, name, op, L, R, x, warmup, steps):
     if rc != 0:
         raise RuntimeError(core.last_error())
     return total.value / steps, list(phases), y
+
+
+def product_form(table, prec, chi, d):
+    """The complex product form ("3M" or "4M") of the step-A and step-C plans of
+    a single chain at this shape, from the autotuner's table (3M is the default
+    of both precisions; a table without the shape means the default ran)."""
+    key = "%s op00 step %%s chi_l %d chi_r %d d %d" % (prec, chi, chi, d)
+    forms = set()
+    for step in "AC":
+        for ln in table:
+            if ln.startswith(key % step) and "chains 1:" in ln:
+                forms.add("4M" if " 4M (" in ln else "3M")
+    if len(forms) > 1:
+        return "3M+4M"
+    return forms.pop() if forms else "3M"
 
 
 def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
@@ -273,6 +291,15 @@ def main():
     g64_ms, g64_tf, g64_frac = roof(ph64, FP64_MFMA_PEAK_TFLOPS)
     g32_ms, g32_tf, g32_frac = roof(ph32, FP32_MFMA_PEAK_TFLOPS)
     traffic = measured_traffic()
+    # complex product form the launches ran (3M executes 3 of the 4 real products)
+    tune_lines = core.autotune_table()
+    form64 = product_form(tune_lines, "fp64", chi, d)
+    form32 = product_form(tune_lines, "fp32", chi, d)
+
+    def mfma_fields(form, tf, peak):
+        share = {"3M": 0.75, "4M": 1.0, "3M+4M": 0.875}[form]
+        return {"complex_product": form, "mfma_flops_per_launch": int(share * f_gemm_launch),
+                "mfma_tflops": round(share * tf, 3), "mfma_frac": round(share * tf / peak, 4)}
 
     result = {
         "metric": METRIC,
@@ -296,7 +323,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(g64_tf, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(g64_frac, 4),
                      "traffic": traffic, "kernel": "cgemm_kernel (complex128, FP64 MFMA)",
-                     "flops_per_launch": int(f_gemm_launch), "avg_launch_ms": round(g64_ms, 4)},
+                     "flops_per_launch": int(f_gemm_launch), "avg_launch_ms": round(g64_ms, 4),
+                     **mfma_fields(form64, g64_tf, FP64_MFMA_PEAK_TFLOPS)},
         "mixed": {
             "matvecs_per_s": round(1000.0 / ms32, 3), "ms_per_matvec": round(ms32, 4),
             "matvec_speedup_vs_fp64": round(ms64 / ms32, 3),
@@ -305,7 +333,8 @@ def main():
                          "gemm_step_c": round(ph32[2], 4)},
             "roofline": {"bound": "mfma", "achieved": round(g32_tf, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(g32_frac, 4),
-                         "kernel": "cgemm_kernel (complex64, FP32 MFMA)", "avg_launch_ms": round(g32_ms, 4)},
+                         "kernel": "cgemm_kernel (complex64, FP32 MFMA)", "avg_launch_ms": round(g32_ms, 4),
+                         **mfma_fields(form32, g32_tf, FP32_MFMA_PEAK_TFLOPS)},
         },
         "copy_calibration": {"bytes": copy_bytes, "avg_ms": round(copy_ms.value, 4),
                              "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},

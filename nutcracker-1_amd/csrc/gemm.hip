@@ -210,8 +210,14 @@ __device__ __forceinline__ void mma_stage(const typename Cplx<T>::type (*As)[TBM
 // NBUF = 1: one LDS stage, two barriers per K step.  NBUF = 2: two LDS stages;
 // the next step is stored into the other stage while this one is consumed, one
 // barrier per K step.
+// The complex128 3-multiply form holds three FP64 accumulator sets (96
+// registers); left to itself the compiler also keeps a whole K stage of LDS
+// operands live and ends at one wave per SIMD, so it is asked for two.
+template <typename T, bool M3, int BK>
+constexpr int cgemm_min_waves() { return (sizeof(T) == sizeof(double) && M3 && BK == 16) ? 2 : 1; }
+
 template <typename T, int OPA, int OPB, bool ACC, bool M3, int BK, int NBUF>
-__global__ __launch_bounds__(256) void cgemm_kernel(const BatchT<T>* __restrict__ batches,
+__global__ __launch_bounds__(256, (cgemm_min_waves<T, M3, BK>())) void cgemm_kernel(const BatchT<T>* __restrict__ batches,
                                                     const SegT<T>* __restrict__ segs) {
     typedef typename Cplx<T>::type CT;
     typedef typename Cplx<T>::v4 V4;
@@ -845,7 +851,7 @@ std::string gemm_tune_table() {
                  "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d%s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
                  k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9], e.choice.bk == -16 ? 16 : e.choice.bk,
                  e.choice.bk == -16 ? " 2-stage" : "", e.choice.nsplit,
-                 e.choice.m3 < 0 ? "3M/4M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
+                 e.choice.m3 < 0 ? "3M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
                  e.candidates);
         out += line;
     }

@@ -108,16 +108,17 @@ struct GemmBatch {
     int32_t M, N;             // this batch's output size; 0 = the plan's M, N
     const int* live = nullptr;  // device flag read at launch: 0 = skip this batch
     int32_t upper = 0;          // 1: C is Hermitian, only its tiles on and above the
-    int32_t _pad = 0;           //    diagonal are computed (the rest is left as is)
+    int32_t _pad = 0;           //    diagonal are computed; the elements below them are
+                                //    undefined afterwards (a split-K launch overwrites them)
 };
 
 enum Op : int { OpN = 0, OpT = 1, OpC = 2 };
 
 // Launch-time choices of the GEMM (see gemm.hip).
 struct GemmTuning {
-    // Gauss's three-multiplication complex product: faster for FP32 (matrix-core
-    // bound), slower for FP64 (measured on the MI355X), hence per precision
-    bool use_3m_f64 = false;
+    // Gauss's three-multiplication complex product (3/4 of the matrix-core
+    // work): the default in both precisions; the autotuner also times 4M
+    bool use_3m_f64 = true;
     bool use_3m_f32 = true;
     bool split_k = true;            // split K when a launch has few output tiles
     int64_t split_k_tiles = 1024;   // ... fewer than this many (tiles x batches)
