@@ -302,8 +302,10 @@ int nc_set_autotune(int on);
 int64_t nc_autotune_table(char* buf, int64_t len);
 /* Forces one launch choice on every GEMM plan, for tests of each kernel variant
    (no counterpart in the reference): bk 16 or 32 (K step per LDS stage) or -16
-   (16 with two LDS stages), nsplit 0 (heuristic) or 1/2/4/8, m3 -1 (the
-   precision's default), 0 (4 real products) or 1 (3); bk = 0 ends the forcing.
+   (16 with two LDS stages), or 6 (complex64 untransposed operands on the bf16
+   matrix cores, each FP32 value split into three bf16 parts; other plans take
+   bk 16), nsplit 0 (heuristic) or 1/2/4/8, m3 -1 (the precision's default), 0
+   (4 real products) or 1 (3); bk = 0 ends the forcing.
    Returns -1 for values outside these. */
 int nc_gemm_force_choice(int bk, int nsplit, int m3);
 /* counts (7): FP64 matvecs, site solves, moves, sweeps, failures, complex64 matvecs,

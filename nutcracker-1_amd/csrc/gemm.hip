@@ -338,6 +338,244 @@ __global__ __launch_bounds__(256, (cgemm_min_waves<T, M3, BK>())) void cgemm_ker
         }
 }
 
+// ---- complex64 GEMM on the bf16 matrix cores (both operands untransposed)
+//
+// Every FP32 operand value v is split into three bf16 parts, v = hi + mid + lo
+// + O(2^-27 |v|) (each part the round-to-nearest bf16 of what the previous
+// ones leave; the residuals are exact in FP32), and a real product x y is
+// formed from the six partial products whose size is above 2^-24 |x y|:
+// hi.hi, hi.mid, mid.hi, hi.lo, mid.mid, lo.hi, all accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16.  The dropped terms are below 3 2^-27 |x y|, so the
+// result has FP32 accuracy, at 6/16 of the FP32 matrix-core time per product
+// (the bf16 rate is 16x the FP32 one).  The complex product is Gauss's 3M
+// form, so the three real matrices re, im, re + im of each operand are split.
+//
+// Layout: 64 x 64 output tile per 256-thread workgroup, K staged 32 deep.  The
+// staging thread (row r = lane, K group g = wave) loads 8 consecutive K values
+// of row r, splits them and writes one 16-byte chunk per (matrix, part) into
+// LDS planes [9][64 rows][4 chunks of 8 bf16], chunk g stored at g ^ ((r >> 2)
+// & 3) so that the 16-byte reads of 16 consecutive rows hit distinct banks.
+// MFMA operands (measured lane layout): lane l holds row l % 16 of the block
+// and K values 8 (l / 16) .. 8 (l / 16) + 7; the accumulator holds rows
+// 4 (l / 16) + v of column l % 16, as the FP32 16x16x4 one.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int B6K = 32;
+constexpr int kBf16x6 = 6;  // launch choice code of this kernel (GemmChoice::bk)
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// two FP32 values rounded to nearest bf16, packed (a in the low half)
+__device__ __forceinline__ unsigned bf16_pair(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2{a, b}), bf16x2));
+}
+// the packed bf16 values as FP32 (a bf16 is the upper half of an FP32)
+__device__ __forceinline__ float bf16_lo(unsigned u) { return __builtin_bit_cast(float, u << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned u) { return __builtin_bit_cast(float, u & 0xffff0000u); }
+
+// v = h + m + l + O(2^-24 |v|), each part round-to-nearest bf16 of what the
+// previous ones leave (the residuals are exact in FP32)
+__device__ __forceinline__ void split3(const float (&v)[8], bf16x8& h, bf16x8& m, bf16x8& l) {
+    u32x4 H, Mid, L;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        float a = v[2 * e], b = v[2 * e + 1];
+        const unsigned ph = bf16_pair(a, b);
+        a -= bf16_lo(ph);
+        b -= bf16_hi(ph);
+        const unsigned pm = bf16_pair(a, b);
+        a -= bf16_lo(pm);
+        b -= bf16_hi(pm);
+        H[e] = ph;
+        Mid[e] = pm;
+        L[e] = bf16_pair(a, b);
+    }
+    h = __builtin_bit_cast(bf16x8, H);
+    m = __builtin_bit_cast(bf16x8, Mid);
+    l = __builtin_bit_cast(bf16x8, L);
+}
+
+// the 8 complex values of row r (M index for A, N index for B) at K positions
+// k0 + 8 g .. + 7 of the current segment; zero outside the matrix
+template <bool IS_A>
+__device__ __forceinline__ void b6_load(const SegT<float>& sg, int k0, int r0, int R, int r, int g, float2 (&v)[8]) {
+    // global memory (not flat: flat loads would share the LDS wait counter)
+    typedef float V2 __attribute__((ext_vector_type(2)));
+    typedef const V2 __attribute__((address_space(1)))* GP;
+    const int row = r0 + r;
+    const int kb = k0 + 8 * g;
+    const GP P = (GP)(IS_A ? sg.A : sg.B);
+    const int64_t ld = IS_A ? sg.lda : sg.ldb;
+    // A(m, k) at A[m + k lda] (lanes consecutive in m); B(k, n) at B[k + n ldb]
+    // (the 8 values of a lane contiguous)
+    const int64_t base = IS_A ? (int64_t)row + (int64_t)kb * ld : (int64_t)kb + (int64_t)row * ld;
+    const int64_t step = IS_A ? ld : 1;
+    if (row < R && kb + 8 <= sg.K) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const V2 t = P[base + e * step];
+            v[e] = make_float2(t.x, t.y);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            v[e] = make_float2(0.f, 0.f);
+            if (row < R && kb + e < sg.K) {
+                const V2 t = P[base + e * step];
+                v[e] = make_float2(t.x, t.y);
+            }
+        }
+    }
+}
+
+// splits re, im and re + im of the 8 values into the 9 planes of row r, chunk g
+__device__ __forceinline__ void b6_store(bf16x8 (*S)[64][4], int r, int g, const float2 (&v)[8]) {
+    const int c = g ^ ((r >> 2) & 3);
+    float x[8];
+    bf16x8 h, m, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = v[e].x;
+    split3(x, h, m, l);
+    S[0][r][c] = h; S[1][r][c] = m; S[2][r][c] = l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = v[e].y;
+    split3(x, h, m, l);
+    S[3][r][c] = h; S[4][r][c] = m; S[5][r][c] = l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = v[e].x + v[e].y;
+    split3(x, h, m, l);
+    S[6][r][c] = h; S[7][r][c] = m; S[8][r][c] = l;
+}
+
+template <bool ACC>
+__global__ __launch_bounds__(256) void cgemm_b6_kernel(const BatchT<float>* __restrict__ batches,
+                                                       const SegT<float>* __restrict__ segs) {
+    __shared__ bf16x8 As[9][64][4];
+    __shared__ bf16x8 Bs[9][64][4];
+
+    const BatchT<float> bt = batches[blockIdx.y];
+    if (bt.live && *bt.live == 0) return;
+    const int M = bt.M, N = bt.N;
+    const int tiles_m = (M + TBM - 1) / TBM, tiles_n = (N + TBN - 1) / TBN;
+    const int ntiles = tiles_m * tiles_n;
+    const int w = blockIdx.x;
+    if (w >= ntiles) return;
+    int tile = w;
+    if (!bt.upper && (ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    if (bt.upper && tm > tn) return;
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave & 1) * 32;
+    const int wn = (wave >> 1) * 32;
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+
+    f32x4 acc[3][2][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = f32x4{0, 0, 0, 0};
+
+    // K stages of this batch (each segment in stages of 32, its tail zero-filled)
+    int nst = 0;
+    for (int i = bt.seg_begin; i < bt.seg_end; ++i) nst += segs[i].K > 0 ? (segs[i].K + B6K - 1) / B6K : 0;
+    // loads run two stages ahead (two register sets): one stage's MFMAs are too
+    // short to hide the load latency at the two workgroups per CU that the LDS
+    // planes allow
+    // (segments with K = 0 have no stage and are stepped over)
+    int lsidx = bt.seg_begin, lk0 = 0;
+    SegT<float> lsg;
+    auto next_seg = [&]() {
+        while (lsidx < bt.seg_end && (lsg = segs[lsidx]).K <= 0) ++lsidx;
+    };
+    next_seg();
+    auto load_next = [&](float2 (&va)[8], float2 (&vb)[8]) {
+        b6_load<true>(lsg, lk0, m0, M, lane, wave, va);
+        b6_load<false>(lsg, lk0, n0, N, lane, wave, vb);
+        lk0 += B6K;
+        if (lk0 >= lsg.K) {
+            ++lsidx;
+            lk0 = 0;
+            next_seg();
+        }
+    };
+    auto stage = [&](float2 (&va)[8], float2 (&vb)[8], int st) {
+        b6_store(As, lane, wave, va);
+        b6_store(Bs, lane, wave, vb);
+        __syncthreads();
+        if (st + 2 < nst) load_next(va, vb);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            bf16x8 a[2][3], b[2][3];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = wm + i * 16 + lrow;
+                const int c = lk ^ ((r >> 2) & 3);
+#pragma unroll
+                for (int p = 0; p < 3; ++p) a[i][p] = As[3 * q + p][r][c];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int r = wn + j * 16 + lrow;
+                const int c = lk ^ ((r >> 2) & 3);
+#pragma unroll
+                for (int p = 0; p < 3; ++p) b[j][p] = Bs[3 * q + p][r][c];
+            }
+            // partial products from the smallest up (lo.hi, mid.mid, hi.lo,
+            // mid.hi, hi.mid, hi.hi), the four accumulators interleaved
+            constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
+            constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][pa[t]], b[j][pb[t]], acc[q][i][j],
+                                                                               0, 0, 0);
+        }
+        __syncthreads();
+    };
+    float2 va0[8], vb0[8], va1[8], vb1[8];
+    if (nst > 0) load_next(va0, vb0);
+    if (nst > 1) load_next(va1, vb1);
+    for (int st = 0; st < nst; st += 2) {
+        stage(va0, vb0, st);
+        if (st + 1 < nst) stage(va1, vb1, st + 1);
+    }
+
+    float2* __restrict__ C = bt.C;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 16 + lrow;
+            if (col >= N) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + 4 * lk + v;
+                if (row < M) {
+                    const float t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
+                    float2 o = make_float2(t1 - t2, t3 - t1 - t2);
+                    float2* p = C + (int64_t)row + (int64_t)col * bt.ldc;
+                    if (ACC) {
+                        const float2 c = *p;
+                        o.x += c.x;
+                        o.y += c.y;
+                    }
+                    *p = o;
+                }
+            }
+        }
+}
+
 // C_b = sum_c W_{b,c} (+ C_b) for the split-K partial products
 template <typename CT>
 __global__ void splitk_reduce_kernel(const CT* __restrict__ W, int64_t slice, int nsplit, CT* const* __restrict__ Cptr,
@@ -442,6 +680,16 @@ void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, cons
 template <typename T>
 void launch_any(int opA, int opB, int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
                 hipStream_t st) {
+    if (bk == kBf16x6) {
+        if constexpr (sizeof(T) == sizeof(float)) {
+            if (opA == OpN && opB == OpN) {
+                if (acc) hipLaunchKernelGGL(cgemm_b6_kernel<true>, grid, dim3(256), 0, st, b, s);
+                else hipLaunchKernelGGL(cgemm_b6_kernel<false>, grid, dim3(256), 0, st, b, s);
+                return;
+            }
+        }
+        bk = 16;  // complex128 and transposed operands: the FP MFMA kernel
+    }
 #define NC_O(a, bb) \
     if (opA == a && opB == bb) { launch_typed<T, a, bb>(bk, acc, m3, grid, b, s, st); return; }
     NC_O(OpN, OpN) NC_O(OpN, OpT) NC_O(OpN, OpC) NC_O(OpT, OpN) NC_O(OpC, OpN) NC_O(OpC, OpC) NC_O(OpT, OpT)
@@ -557,7 +805,7 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
     }
     out.nsplit = nsplit;
     out.m3 = ch.m3 >= 0 ? ch.m3 != 0 : (sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32);
-    out.bk = (ch.bk == 32 || ch.bk == -16) ? ch.bk : 16;
+    out.bk = (ch.bk == 32 || ch.bk == -16 || ch.bk == kBf16x6) ? ch.bk : 16;
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
@@ -698,6 +946,12 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
                 if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
                 cands.push_back(GemmChoice{bk, ns, m3});
             }
+    // complex64 untransposed operands: the bf16 split kernel (always 3M)
+    if (sizeof(T) == sizeof(float) && opA == OpN && opB == OpN && g_tuning.bf16x6)
+        for (int ns : {1, 2, 4, 8}) {
+            if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
+            cands.push_back(GemmChoice{kBf16x6, ns, 1});
+        }
     // every candidate built once (own descriptors; the split-K workspace is
     // shared, candidates run one after another)
     struct Cand {
@@ -849,8 +1103,9 @@ std::string gemm_tune_table() {
         auto const& e = kv.second;
         snprintf(line, sizeof line,
                  "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d%s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
-                 k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9], e.choice.bk == -16 ? 16 : e.choice.bk,
-                 e.choice.bk == -16 ? " 2-stage" : "", e.choice.nsplit,
+                 k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9],
+                 e.choice.bk == -16 ? 16 : (e.choice.bk == kBf16x6 ? 32 : e.choice.bk),
+                 e.choice.bk == -16 ? " 2-stage" : (e.choice.bk == kBf16x6 ? " bf16x6" : ""), e.choice.nsplit,
                  e.choice.m3 < 0 ? "3M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
                  e.candidates);
         out += line;

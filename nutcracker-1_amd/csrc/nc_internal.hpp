@@ -13,6 +13,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -120,6 +121,12 @@ struct GemmTuning {
     // work): the default in both precisions; the autotuner also times 4M
     bool use_3m_f64 = true;
     bool use_3m_f32 = true;
+    // complex64 plans may also be tuned onto the bf16 split kernel (FP32
+    // accuracy from six bf16 partial products); NC_BF16X6=0 takes it out
+    bool bf16x6 = [] {
+        const char* e = std::getenv("NC_BF16X6");
+        return !(e && e[0] == '0');
+    }();
     bool split_k = true;            // split K when a launch has few output tiles
     int64_t split_k_tiles = 1024;   // ... fewer than this many (tiles x batches)
 };

@@ -80,14 +80,16 @@ def test_tuned_matvec_against_oracle(lib, shape):
         assert m and float(m.group(1)) <= float(m.group(2)) and int(m.group(3)) >= 4
 
 
-VARIANTS = [(16, 1, 0), (16, 2, 1), (32, 4, 0), (32, 1, 1), (-16, 1, 0), (-16, 8, 1), (-16, 0, -1)]
+VARIANTS = [(16, 1, 0), (16, 2, 1), (32, 4, 0), (32, 1, 1), (-16, 1, 0), (-16, 8, 1), (-16, 0, -1), (6, 1, 1),
+            (6, 4, 1)]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_every_launch_variant_against_oracle(lib, variant):
     """Each kernel variant the tuner may pick (K step 16 / 32 / 16 with two LDS
-    stages, K split, 3M / 4M), forced on every plan: the matvec and both
-    environment contractors on ragged shapes keep the drop-in's accuracy."""
+    stages, K split, 3M / 4M, the bf16 split kernel for complex64), forced on
+    every plan: the matvec and both environment contractors on ragged shapes
+    keep the drop-in's accuracy, and the complex64 matvec FP32 accuracy."""
     core, h = lib
     f = h.nc_gemm_force_choice
     f.restype = C.c_int
@@ -114,4 +116,4 @@ def test_every_launch_variant_against_oracle(lib, variant):
     assert 1e-9 < rel(y32, ref) < 2e-6
     assert rel(newL, refL) < 1e-12
     assert rel(newR, refR) < 1e-12
-    assert f(24, 0, -1) == -1 and f(16, 3, 0) == -1
+    assert f(24, 0, -1) == -1 and f(16, 3, 0) == -1 and f(7, 1, 1) == -1
