@@ -120,3 +120,28 @@ def test_mixed_with_fp64_polish(mods):
     b.sweep_until_converged()
     print("E64", a.energy, "Emixed+polish", b.energy, b.stats(), flush=True)
     assert abs(a.energy - b.energy) <= 1e-10 * abs(a.energy)
+
+
+@pytest.mark.parametrize("model", ["xxz", "spin1"])
+def test_mixed_chain_on_bf16_split_kernel(mods, model):
+    """The complex64 contractions forced onto the bf16 matrix-core kernel (each
+    FP32 operand split into three bf16 parts): the mixed chain still converges
+    to the plain FP64 chain's energy within the drop-in bar of 1e-10."""
+    chain, _, ops_mod, lib = mods
+    f = lib.nc_gemm_force_choice
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int]
+    ops, chi = _model(ops_mod, model)
+    print("model", model, flush=True)
+    a = chain.Chain(ops, chi, seed=7, rethrow=True)
+    a.sweep_until_converged()
+    assert f(6, 1, 1) == 0
+    try:
+        b = chain.Chain(ops, chi, seed=7, rethrow=True, mixed=True)
+        b.sweep_until_converged()
+    finally:
+        assert f(0, 0, -1) == 0
+    de = abs(a.energy - b.energy)
+    print("E64", a.energy, "Emixed(bf16 split)", b.energy, "dE", de, flush=True)
+    assert de <= 1e-10 * abs(a.energy)
+    assert b.stats()["matvecs32"] > 0
