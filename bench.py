@@ -53,6 +53,7 @@ sys.path.insert(0, os.path.join(ROOT, "nutcracker-1_amd"))
 METRIC = "matvecs/s + sweep sec at chi=1024, mixed vs FP64 on 1 GPU; |dE| vs FP64"
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cgemm_heff_traffic_chi1024.md")
 
@@ -133,9 +134,11 @@ def product_form(table, prec, chi, d):
     for step in "AC":
         for ln in table:
             if ln.startswith(key % step) and "chains 1:" in ln:
-                forms.add("4M" if " 4M (" in ln else "3M")
-    if len(forms) > 1:
+                forms.add("bf16x6" if " bf16x6 " in ln else ("4M" if " 4M (" in ln else "3M"))
+    if forms == {"3M", "4M"}:
         return "3M+4M"
+    if len(forms) > 1:
+        return "mixed forms"
     return forms.pop() if forms else "3M"
 
 
@@ -297,9 +300,14 @@ def main():
     form32 = product_form(tune_lines, "fp32", chi, d)
 
     def mfma_fields(form, tf, peak):
-        share = {"3M": 0.75, "4M": 1.0, "3M+4M": 0.875}[form]
+        # matrix-core work executed per algorithmic flop, and the peak it runs against
+        # (bf16x6: 3M on the bf16 matrix cores, six bf16 partial products per real one)
+        if form not in ("3M", "4M", "3M+4M", "bf16x6"):
+            return {"complex_product": form}
+        share, mpeak = {"3M": (0.75, peak), "4M": (1.0, peak), "3M+4M": (0.875, peak),
+                        "bf16x6": (4.5, BF16_MFMA_PEAK_TFLOPS)}[form]
         return {"complex_product": form, "mfma_flops_per_launch": int(share * f_gemm_launch),
-                "mfma_tflops": round(share * tf, 3), "mfma_frac": round(share * tf / peak, 4)}
+                "mfma_tflops": round(share * tf, 3), "mfma_peak": mpeak, "mfma_frac": round(share * tf / mpeak, 4)}
 
     result = {
         "metric": METRIC,
