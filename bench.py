@@ -341,7 +341,9 @@ def main():
                          "gemm_step_c": round(ph32[2], 4)},
             "roofline": {"bound": "mfma", "achieved": round(g32_tf, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(g32_frac, 4),
-                         "kernel": "cgemm_kernel (complex64, FP32 MFMA)", "avg_launch_ms": round(g32_ms, 4),
+                         "kernel": ("cgemm_b6_kernel (complex64 split into bf16 parts, bf16 MFMA)"
+                                    if form32 == "bf16x6" else "cgemm_kernel (complex64, FP32 MFMA)"),
+                         "avg_launch_ms": round(g32_ms, 4),
                          **mfma_fields(form32, g32_tf, FP32_MFMA_PEAK_TFLOPS)},
         },
         "copy_calibration": {"bytes": copy_bytes, "avg_ms": round(copy_ms.value, 4),
