@@ -222,14 +222,21 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
 }
 
 // Hessenberg column j = coef1 + coef2, beta_j = ||w||; beta = 0 when the new
-// vector is round-off of an invariant subspace; scale[p] = 1 / beta (or 0)
-__global__ void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv, const double2* __restrict__ coef1,
-                               const double2* __restrict__ coef2, int ldp, const double* __restrict__ pnorm, int nblk,
-                               double2* __restrict__ H, double* __restrict__ betas, double* __restrict__ scale) {
+// vector is round-off of an invariant subspace; scale[p] = 1 / beta (or 0).
+// One 64-lane block per problem: the partial norms are added lane-strided, then
+// across the lanes, in a fixed order.
+__global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv,
+                                                     const double2* __restrict__ coef1,
+                                                     const double2* __restrict__ coef2, int ldp,
+                                                     const double* __restrict__ pnorm, int nblk,
+                                                     double2* __restrict__ H, double* __restrict__ betas,
+                                                     double* __restrict__ scale) {
     const int p = blockIdx.x;
-    if (j < desc[p].kstart || threadIdx.x != 0) return;
+    if (j < desc[p].kstart) return;
     double b2 = 0;
-    for (int b = 0; b < nblk; ++b) b2 += pnorm[(int64_t)p * nblk + b];
+    for (int b = threadIdx.x; b < nblk; b += 64) b2 += pnorm[(int64_t)p * nblk + b];
+    for (int off = 32; off > 0; off >>= 1) b2 += __shfl_xor(b2, off, 64);
+    if (threadIdx.x != 0) return;
     double beta = sqrt(b2);
     const double wn = sqrt(fmax(0.0, coef1[(int64_t)p * ldp + j + 1].x));
     if (!(beta >= 1e-14 * wn)) beta = 0.0;
