@@ -156,8 +156,8 @@ __global__ __launch_bounds__(KT) void kupdate_kernel(const KDesc* __restrict__ d
 
 // The first update fused with the second pass's dot products: w -= sum_{c<=j}
 // V_c coef_c, then the partial dots of V_0..V_j against the updated w and its
-// squared norm (entry j+1), in the layout of kdot_kernel.  The dots re-read the
-// rows of V the update has just read, so they come from the caches, not HBM.
+// squared norm (entry j+1), in the layout of kdot_kernel.  Each row's basis
+// entries are loaded once, into registers, and serve both.
 template <int MC>
 __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ desc, int j,
                                                     const double2* __restrict__ coef, int ldp,
@@ -177,21 +177,27 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
     for (int c = 0; c < MC; ++c) { re[c] = 0; im[c] = 0; }
     const int nb = gridDim.x;
     for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+        // the row's basis entries, all loads issued before the first use
+        double2 a[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c)
+            if (c < m) a[c] = d.V[i + (int64_t)c * n];
         double2 v = w[i];
-        for (int c = 0; c < m; ++c) {
-            const double2 a = d.V[i + (int64_t)c * n];
-            const double2 h = sc[c];
-            v.x -= a.x * h.x - a.y * h.y;
-            v.y -= a.x * h.y + a.y * h.x;
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            if (c < m) {
+                const double2 h = sc[c];
+                v.x -= a[c].x * h.x - a[c].y * h.y;
+                v.y -= a[c].x * h.y + a[c].y * h.x;
+            }
         }
         w[i] = v;
         nrm += v.x * v.x + v.y * v.y;
 #pragma unroll
         for (int c = 0; c < MC; ++c) {
             if (c < m) {
-                const double2 a = d.V[i + (int64_t)c * n];
-                re[c] += a.x * v.x + a.y * v.y;
-                im[c] += a.x * v.y - a.y * v.x;
+                re[c] += a[c].x * v.x + a[c].y * v.y;
+                im[c] += a[c].x * v.y - a[c].y * v.x;
             }
         }
     }
