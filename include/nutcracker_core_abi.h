@@ -148,9 +148,9 @@ void form_overlap_vector_(const uint32_t* b_left_old, const uint32_t* b_right_ol
 /* core.cpp:371-378; core.f95:1341-1435.  n = vector length, m = number of
    vectors; vectors(n, m) is overwritten by its column-pivoted Householder QR,
    coefficients and swaps have min(n, m) entries, *rank = number of non-null
-   reflectors. */
-void convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
-                                    nc_zcomplex* coefficients, uint32_t* swaps);
+   reflectors (also returned, as the reference's declaration has it). */
+uint32_t convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
+                                        nc_zcomplex* coefficients, uint32_t* swaps);
 
 /* core.cpp:498-504; core.f95:3299-3313.  overlap(bl, d, br) = conj(state(br, bl, d)). */
 void form_overlap_site_tensor_(const uint32_t* br, const uint32_t* bl, const uint32_t* d,
@@ -165,11 +165,91 @@ void form_norm_overlap_tensors_(const uint32_t* bl, const uint32_t* bm, const ui
                                 nc_zcomplex* unnormalized_overlap_tensor_1, nc_zcomplex* unnormalized_state_tensor_2,
                                 nc_zcomplex* right_norm_overlap_tensor_2);
 
+/* ---- random tensors, normalisation, bond growth, boundaries --------------
+   The random routines draw from a device hash keyed by a per-process call
+   counter instead of gfortran's rand() stream (core.f95:2617-2632): the
+   numbers differ, the properties (normalisation, orthonormality, rank) hold. */
+
+/* core.cpp:735-750; core.f95:2634-2659.  Random site (br, bl, d) with
+   sum_{r,s} psi(r,l,s) conj(psi(r,l',s)) = delta(l,l'); needs br*d >= bl. */
+void rand_norm_state_site_tensor_(const uint32_t* br, const uint32_t* bl, const uint32_t* d,
+                                  nc_zcomplex* state_site_tensor);
+
+/* core.cpp:752-767; core.f95:2661-2671.  Random site of unit Frobenius norm. */
+void rand_unnorm_state_site_tensor_(const uint32_t* br, const uint32_t* bl, const uint32_t* d,
+                                    nc_zcomplex* state_site_tensor);
+
+/* core.cpp:769-783; core.f95:2673-2706.  Random vectors (length
+   projector_length, number_of_projectors of them) converted to Householder
+   reflectors as convert_vectors_to_reflectors_ does. */
+void random_projector_matrix_(const uint32_t* projector_length, const uint32_t* number_of_projectors,
+                              uint32_t* rank, nc_zcomplex* reflectors, nc_zcomplex* coefficients, uint32_t* swaps);
+
+/* core.cpp:638-655; core.f95:2880-2917.  The unitary polar factor of the site
+   as a (br x bl*d) matrix.  Returns the SVD info (0 on success). */
+int32_t norm_for_left_(const uint32_t* br, const uint32_t* bl, const uint32_t* d, const nc_zcomplex* site_tensor,
+                       nc_zcomplex* normalized_site_tensor);
+
+/* core.cpp:657-674; core.f95:2919-2960.  The same for the (bl x br*d) matrix
+   of the transposed site. */
+int32_t norm_for_right_(const uint32_t* br, const uint32_t* bl, const uint32_t* d, const nc_zcomplex* site_tensor,
+                        nc_zcomplex* normalized_site_tensor);
+
+/* core.cpp:548-578; core.f95:3090-3144.  Widens the bond between a left site
+   (bm, bl, dl) and a right site (br, bm, dr) to new_bm with a random isometry
+   and re-normalises the right site (norm_denorm_going_left).  Outputs, in
+   argument order: the left site (new_bm, bl, dl), the right site
+   (br, new_bm, dr).  Returns the SVD info. */
+int32_t increase_bandwidth_between_(const uint32_t* bl, const uint32_t* bm, const uint32_t* br, const uint32_t* dl,
+                                    const uint32_t* dr, const uint32_t* new_bm,
+                                    const nc_zcomplex* site_tensor_to_normalize,
+                                    const nc_zcomplex* site_tensor_to_denormalize,
+                                    nc_zcomplex* normalized_site_tensor, nc_zcomplex* denormalized_site_tensor);
+
+/* core.cpp:785-819; core.f95:3246-3297.  For a random new_b x b isometry M:
+   state_s -> conj(M) state_s M^H, left_k -> M left_k M^H,
+   right_k -> conj(M) right_k M^T. */
+void increase_bandwidth_with_environment_(const uint32_t* b, const uint32_t* c, const uint32_t* d,
+                                          const uint32_t* new_b, const nc_zcomplex* old_state_site_tensor,
+                                          const nc_zcomplex* old_left_exp_environment,
+                                          const nc_zcomplex* old_right_exp_environment,
+                                          nc_zcomplex* new_state_site_tensor, nc_zcomplex* new_left_exp_environment,
+                                          nc_zcomplex* new_right_exp_environment);
+
+/* core.cpp:139-159; core.f95:3402-3418.  L(i,j,k) = s(i) conj(s(j)) o(k). */
+void construct_left_exp_boundary_(const uint32_t* b, const uint32_t* c, const nc_zcomplex* state_boundary,
+                                  const nc_zcomplex* operator_boundary, nc_zcomplex* left_expectation_boundary);
+
+/* core.cpp:161-181; core.f95:3420-3436.  R(i,j,k) = conj(s(i)) s(j) o(k). */
+void construct_right_exp_boundary_(const uint32_t* b, const uint32_t* c, const nc_zcomplex* state_boundary,
+                                   const nc_zcomplex* operator_boundary, nc_zcomplex* right_expectation_boundary);
+
+/* core.cpp:210-233; core.f95:1195-1234.  new_left(br, br) = X L X^H for
+   X = matrix (br, bl), L = left_environment (bl, bl). */
+void contract_matrix_left_(const uint32_t* bl, const uint32_t* br, const nc_zcomplex* left_environment,
+                           const nc_zcomplex* matrix, nc_zcomplex* new_left_environment);
+
+/* core.cpp:400-429; core.f95:1163-1193.
+   new(r, s, l) = sum_m site(r, m, s) old(m, l), new (br, dr, dl). */
+void extend_state_vector_fragment_(const uint32_t* bm, const uint32_t* br, const uint32_t* dl, const uint32_t* dr,
+                                   const nc_zcomplex* old_state_vector_fragment, const nc_zcomplex* site_tensor,
+                                   nc_zcomplex* new_state_vector_fragment);
+
 /* ---- entry points with no counterpart in the reference --------------------- */
 
 /* Message of the last failed call on this thread (the reference's C API keeps
    a thread-local error string, sources/c-interface/errors.cpp:7-25). */
 const char* nc_last_error(void);
+
+/* Krylov basis size used by optimize_ (the reference hard-wires 3 at
+   core.f95:2567; 3 is the default here too).  Accepts 3..62; returns the
+   previous value, or -1 for a value outside that range. */
+int nc_set_optimizer_ncv(int ncv);
+
+/* Counters of the site optimizer since load (5 values): site solves, first
+   Krylov runs that ended with -14, ncv x 3/2 retries run (core.f95:2588-2602),
+   dense strategy-1 fallbacks (core.f95:2569-2587), solves returned with -14. */
+int nc_optimizer_stats(int64_t* out);
 
 /* Architecture name, compute-unit count, clock and memory of the current device. */
 int nc_device_info(char* name, int name_len, int* cu_count, double* clock_ghz, size_t* total_mem);
@@ -234,6 +314,9 @@ int nc_chain_set_mixed(nc_chain* chain, int mixed, int env32, double tol32, int 
 /* raw (unnormalised) site tensor (br, bl, d); nc_chain_reset canonicalises */
 int nc_chain_set_raw_site(nc_chain* chain, uint32_t site, const nc_zcomplex* data);
 int nc_chain_get_site(nc_chain* chain, uint32_t site, nc_zcomplex* out);
+/* The environment of the current site: side 0 the left one (bl, bl, cl), side 1
+   the right one (br, br, cr). */
+int nc_chain_get_environment(nc_chain* chain, uint32_t site, int side, nc_zcomplex* out);
 int nc_chain_reset(nc_chain* chain);
 int nc_chain_optimize_site(nc_chain* chain);
 int nc_chain_move(nc_chain* chain, int direction);

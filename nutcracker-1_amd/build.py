@@ -36,6 +36,7 @@ SOURCES = [
     "chain.cpp",
     "batch.cpp",
     "abi.cpp",
+    "abi_support.cpp",
     "chain_abi.cpp",
     "bench_abi.cpp",
     "formats.cpp",

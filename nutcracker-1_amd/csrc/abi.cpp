@@ -8,7 +8,7 @@
 // Failures never abort the process: the entry point records a message
 // (nc_last_error) and returns an error code, or leaves zeroed outputs for the
 // reference's void routines.
-#include "nc_internal.hpp"
+#include "nc_abi.hpp"
 #include "../../include/nutcracker_core_abi.h"
 
 #include <algorithm>
@@ -17,33 +17,6 @@
 using namespace nc;
 
 namespace {
-
-struct HostIn {
-    ZBuf buf;
-    HostIn(const void* host, size_t count) : buf(std::max<size_t>(1, count)) {
-        if (count) h2d(buf.ptr, host, count * sizeof(double2));
-    }
-    double2* ptr() { return buf.ptr; }
-};
-
-void fill_zero(void* p, size_t bytes) {
-    if (p && bytes) memset(p, 0, bytes);
-}
-
-template <class F>
-int guarded(const char* name, F&& f) {
-    try {
-        return f();
-    } catch (Error const& e) {
-        set_last_error(std::string(name) + ": " + e.what());
-        fprintf(stderr, "[nutcracker-hip] %s: %s\n", name, e.what());
-        return e.code != 0 ? e.code : -1;
-    } catch (std::exception const& e) {
-        set_last_error(std::string(name) + ": " + e.what());
-        fprintf(stderr, "[nutcracker-hip] %s: %s\n", name, e.what());
-        return -1;
-    }
-}
 
 Workspace& ws() {
     static thread_local Workspace w;
@@ -59,6 +32,23 @@ void build_op(MpoSite& op, uint32_t d, uint32_t cl, uint32_t cr, uint32_t nmat, 
 extern "C" {
 
 const char* nc_last_error(void) { return last_error().c_str(); }
+
+int nc_set_optimizer_ncv(int ncv) {
+    if (ncv < 3 || ncv > kMaxNcv) return -1;
+    const int prev = optimizer_ncv();
+    optimizer_ncv() = ncv;
+    return prev;
+}
+
+int nc_optimizer_stats(int64_t* out) {
+    const OptimizerStats st = optimizer_stats();
+    out[0] = st.solves;
+    out[1] = st.first_run_failures;
+    out[2] = st.ncv_retries;
+    out[3] = st.dense_fallbacks;
+    out[4] = st.unconverged;
+    return 0;
+}
 
 int nc_device_info(char* name, int name_len, int* cu_count, double* clock_ghz, size_t* total_mem) {
     return guarded("nc_device_info", [&] {
@@ -310,9 +300,13 @@ uint32_t optimize_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, c
             proj.n = (int)n;
             proj.orth = (int)*orthogonal_subspace_dimension;
         }
+        // core.f95:1962-1983: info 5 when the orthogonal space is larger than the
+        // full space, 10 when it is empty, whatever the projector count
         if (nproj == 0 && (size_t)*orthogonal_subspace_dimension > n) return 5;
+        if (nproj == 0 && *orthogonal_subspace_dimension == 0) return 10;
         OptimizeArgs a{(int)*bl, (int)*br, (int)*cl, (int)*cr, (int)*d, L.ptr(), R.ptr(), &op, &proj, which, *tol,
                        (int)*number_of_iterations, g.ptr(), res.ptr};
+        a.ncv = optimizer_ncv();
         OptimizeOut o = optimize_site(a, ws(), s);
         if (o.info == 5 || o.info == 10 || o.info == 11) return o.info;
         d2h(result, res.ptr, n * sizeof(double2));
@@ -550,8 +544,8 @@ void form_norm_overlap_tensors_(const uint32_t* bl, const uint32_t* bm, const ui
     }
 }
 
-void convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
-                                    nc_zcomplex* coefficients, uint32_t* swaps) {
+uint32_t convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zcomplex* vectors, uint32_t* rank,
+                                        nc_zcomplex* coefficients, uint32_t* swaps) {
     // n: vector length (rows), m: number of vectors (the Fortran routine's m, n)
     const int64_t rows = *n;
     const int cols = (int)*m;
@@ -573,6 +567,7 @@ void convert_vectors_to_reflectors_(const uint32_t* n, const uint32_t* m, nc_zco
         return 0;
     });
     if (rc) *rank = 0;
+    return *rank;
 }
 
 }  // extern "C"

@@ -111,6 +111,21 @@ int nc_chain_get_site(nc_chain* c, uint32_t i, nc_zcomplex* out) {
     });
 }
 
+int nc_chain_get_environment(nc_chain* c, uint32_t i, int side, nc_zcomplex* out) {
+    return guarded_chain("nc_chain_get_environment", [&] {
+        Chain& ch = *c->chain;
+        if ((int)i >= ch.N) throw Error("site index out of range", -69);
+        if (side == 0) {
+            const size_t n = (size_t)ch.bonds[i] * ch.bonds[i] * ch.ops[i]->cl;
+            d2h(out, ch.left_envs[i].ptr, n * sizeof(double2));
+        } else {
+            const size_t n = (size_t)ch.bonds[i + 1] * ch.bonds[i + 1] * ch.ops[i]->cr;
+            d2h(out, ch.right_envs[i + 1].ptr, n * sizeof(double2));
+        }
+        return 0;
+    });
+}
+
 int nc_chain_reset(nc_chain* c) {
     return guarded_chain("nc_chain_reset", [&] { c->chain->reset(); return 0; });
 }

@@ -336,15 +336,28 @@ struct Pinned {
     T const& operator[](size_t i) const { return ptr[i]; }
 };
 
-// instantiated graphs of one solve, destroyed with it
+// Instantiated graphs of one solve, destroyed with it.  The stream they run on
+// is drained first (an exception may leave a replay in flight), and each
+// captured graph lives as long as its executable.
 struct GraphCache {
-    std::vector<std::pair<int, hipGraphExec_t>> execs;
+    struct Entry {
+        int key;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> entries;
+    hipStream_t stream = nullptr;
     ~GraphCache() {
-        for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+        if (entries.empty()) return;
+        (void)hipStreamSynchronize(stream);
+        for (auto& e : entries) {
+            (void)hipGraphExecDestroy(e.exec);
+            (void)hipGraphDestroy(e.graph);
+        }
     }
     hipGraphExec_t find(int key) const {
-        for (auto const& e : execs)
-            if (e.first == key) return e.second;
+        for (auto const& e : entries)
+            if (e.key == key) return e.exec;
         return nullptr;
     }
 };
@@ -600,9 +613,10 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                     }
                     NC_HIP(hipStreamEndCapture(s, &gph));
                     const hipError_t e = hipGraphInstantiate(&ge, gph, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(gph);
+                    if (e != hipSuccess) (void)hipGraphDestroy(gph);
                     NC_HIP(e);
-                    graphs.execs.emplace_back(jmin, ge);
+                    graphs.stream = s;
+                    graphs.entries.push_back(GraphCache::Entry{jmin, gph, ge});
                 }
                 NC_HIP(hipGraphLaunch(ge, s));
             } else {

@@ -21,6 +21,38 @@ struct MixDesc {
     int64_t rs;
 };
 
+// One thread, one slab element e, outputs b0 .. b0 + NB - 1 of destination dst
+// (those below d).  D > 0 fixes the physical dimension at compile time.
+template <typename CT, int D, int NB>
+__device__ __forceinline__ void mix_element(const CT* __restrict__ P, CT* __restrict__ Q,
+                                            const int* __restrict__ dst_ptr, const int* __restrict__ blk_src,
+                                            const CT* __restrict__ W, int d_rt, int64_t rs, int64_t e, int dst,
+                                            int b0) {
+    const int d = (D > 0) ? D : d_rt;
+    CT q[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) { q[b].x = 0; q[b].y = 0; }
+    for (int t = dst_ptr[dst]; t < dst_ptr[dst + 1]; ++t) {
+        const int src = blk_src[t];
+        const CT* Wn = W + (int64_t)t * d * d;
+        for (int a = 0; a < d; ++a) {
+            const CT p = P[((int64_t)src * d + a) * rs + e];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (b0 + b < d) {
+                    const CT w = Wn[a + (b0 + b) * d];
+                    q[b].x += w.x * p.x - w.y * p.y;
+                    q[b].y += w.x * p.y + w.y * p.x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        if (b0 + b < d) Q[((int64_t)dst * d + b0 + b) * rs + e] = q[b];
+}
+
+// D = 0: any physical dimension, four outputs per pass over the blocks
 template <typename CT, int D>
 __global__ __launch_bounds__(256) void mix_kernel(const CT* __restrict__ P, CT* __restrict__ Q,
                                                   const int* __restrict__ dst_ptr,
@@ -29,28 +61,17 @@ __global__ __launch_bounds__(256) void mix_kernel(const CT* __restrict__ P, CT* 
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int dst = blockIdx.y;
     if (e >= md.rs) return;
-    const int d = (D > 0) ? D : md.d;
-    CT q[4];
-    for (int b = 0; b < 4; ++b) { q[b].x = 0; q[b].y = 0; }
-    for (int t = dst_ptr[dst]; t < dst_ptr[dst + 1]; ++t) {
-        const int src = blk_src[t];
-        const CT* Wn = W + (int64_t)t * d * d;
-        for (int a = 0; a < d; ++a) {
-            const CT p = P[((int64_t)src * d + a) * md.rs + e];
-            for (int b = 0; b < d; ++b) {
-                const CT w = Wn[a + b * d];
-                q[b].x += w.x * p.x - w.y * p.y;
-                q[b].y += w.x * p.y + w.y * p.x;
-            }
-        }
+    if (D > 0) {
+        mix_element<CT, D, (D > 0 ? D : 1)>(P, Q, dst_ptr, blk_src, W, D, md.rs, e, dst, 0);
+    } else {
+        for (int b0 = 0; b0 < md.d; b0 += 4) mix_element<CT, 0, 4>(P, Q, dst_ptr, blk_src, W, md.d, md.rs, e, dst, b0);
     }
-    for (int b = 0; b < d; ++b) Q[((int64_t)dst * d + b) * md.rs + e] = q[b];
 }
 
 template <typename CT>
 static void mix_blocks_t(MpoSite const& op, bool src_is_from, const CT* P, CT* Q, int64_t rs, const CT* W,
                          hipStream_t s) {
-    if (op.d > 4) throw Error("mix_blocks: physical dimension above 4 is not supported");
+    if (op.d < 1) throw Error("mix_blocks: physical dimension below 1", -4);
     const int dir = src_is_from ? 0 : 1;
     const int ndst = (int)(src_is_from ? op.tos.size() : op.froms.size());
     if (ndst == 0 || rs == 0) return;
@@ -61,6 +82,7 @@ static void mix_blocks_t(MpoSite const& op, bool src_is_from, const CT* P, CT* Q
     switch (op.d) {
         case 2: hipLaunchKernelGGL((mix_kernel<CT, 2>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
         case 3: hipLaunchKernelGGL((mix_kernel<CT, 3>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
+        case 4: hipLaunchKernelGGL((mix_kernel<CT, 4>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
         default: hipLaunchKernelGGL((mix_kernel<CT, 0>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
     }
     NC_HIP(hipGetLastError());
@@ -93,21 +115,12 @@ __global__ __launch_bounds__(256) void mix_batch_kernel(const MixBatchDesc<CT>* 
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int dst = blockIdx.y;
     if (dst >= md.ndst || e >= md.rs) return;
-    CT q[D];
-    for (int b = 0; b < D; ++b) { q[b].x = 0; q[b].y = 0; }
-    for (int t = md.dst_ptr[dst]; t < md.dst_ptr[dst + 1]; ++t) {
-        const int src = md.blk_src[t];
-        const CT* Wn = md.W + (int64_t)t * D * D;
-        for (int a = 0; a < D; ++a) {
-            const CT p = md.P[((int64_t)src * D + a) * md.rs + e];
-            for (int b = 0; b < D; ++b) {
-                const CT w = Wn[a + b * D];
-                q[b].x += w.x * p.x - w.y * p.y;
-                q[b].y += w.x * p.y + w.y * p.x;
-            }
-        }
+    if (D > 0) {
+        mix_element<CT, D, (D > 0 ? D : 1)>(md.P, md.Q, md.dst_ptr, md.blk_src, md.W, D, md.rs, e, dst, 0);
+    } else {
+        for (int b0 = 0; b0 < md.d; b0 += 4)
+            mix_element<CT, 0, 4>(md.P, md.Q, md.dst_ptr, md.blk_src, md.W, md.d, md.rs, e, dst, b0);
     }
-    for (int b = 0; b < D; ++b) md.Q[((int64_t)dst * D + b) * md.rs + e] = q[b];
 }
 
 template <typename CT>
@@ -135,7 +148,7 @@ static bool mix_descs(std::vector<MixItem<CT>> const& items, const CT* (*W)(MpoS
         maxdst = std::max(maxdst, m.ndst);
     }
     if (maxrs == 0 || maxdst == 0) return false;
-    if (d < 2 || d > 4) throw Error("mix_blocks_batch: physical dimension above 4 is not supported", -4);
+    if (d < 1) throw Error("mix_blocks_batch: physical dimension below 1", -4);
     grid = dim3((unsigned)((maxrs + 255) / 256), (unsigned)maxdst, (unsigned)ds.size());
     return true;
 }
@@ -145,7 +158,8 @@ static void mix_launch(const MixBatchDesc<CT>* dd, dim3 grid, int d, hipStream_t
     switch (d) {
         case 2: hipLaunchKernelGGL((mix_batch_kernel<CT, 2>), grid, dim3(256), 0, s, dd); break;
         case 3: hipLaunchKernelGGL((mix_batch_kernel<CT, 3>), grid, dim3(256), 0, s, dd); break;
-        default: hipLaunchKernelGGL((mix_batch_kernel<CT, 4>), grid, dim3(256), 0, s, dd); break;
+        case 4: hipLaunchKernelGGL((mix_batch_kernel<CT, 4>), grid, dim3(256), 0, s, dd); break;
+        default: hipLaunchKernelGGL((mix_batch_kernel<CT, 0>), grid, dim3(256), 0, s, dd); break;
     }
     NC_HIP(hipGetLastError());
 }

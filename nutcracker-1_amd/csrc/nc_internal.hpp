@@ -394,6 +394,10 @@ struct IramResult {
     zd eigenvalue{0, 0};
 };
 
+// Largest Krylov basis the solver takes (its per-step coefficient tables hold
+// ncv + 2 entries in 64 lanes).
+constexpr int kMaxNcv = 62;
+
 // which: "SR", "LR", "LM".  guess / result are device vectors of length op.n.
 IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_restarts,
                 const double2* guess, double2* result, hipStream_t s);
@@ -504,6 +508,15 @@ struct OptimizeOut {
 void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::vector<Workspace*> const& wss,
                             std::vector<OptimizeOut>& outs, hipStream_t s, double* krylov_ms = nullptr);
 OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s);
+// Counters of optimize_site since load: solves, first Krylov runs that ended
+// with -14, ncv retries run (core.f95:2588-2602), dense strategy-1 fallbacks
+// (2569-2587), solves returned with -14.
+struct OptimizerStats {
+    int64_t solves = 0, first_run_failures = 0, ncv_retries = 0, dense_fallbacks = 0, unconverged = 0;
+};
+OptimizerStats optimizer_stats();
+// Krylov basis size of the drop-in optimize_ (core.f95:2567 hard-wires 3)
+int& optimizer_ncv();
 
 // ----------------------------------------------------------------------------
 // Site move: polar split of an (m x ncol) matrix, m <= ncol.

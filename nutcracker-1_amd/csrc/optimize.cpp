@@ -4,6 +4,7 @@
 // 1266-1339, 1513-1664).
 #include "nc_internal.hpp"
 
+#include <atomic>
 #include <chrono>
 
 #include <algorithm>
@@ -283,8 +284,28 @@ static void strategy_1(OptimizeArgs const& a, HeffOperator& op, OptimizeOut& out
     out.iterations = 0;
 }
 
+namespace {
+std::atomic<int64_t> g_opt_stats[5];
+}
+
+int& optimizer_ncv() {
+    static int ncv = 3;
+    return ncv;
+}
+
+OptimizerStats optimizer_stats() {
+    OptimizerStats st;
+    st.solves = g_opt_stats[0].load();
+    st.first_run_failures = g_opt_stats[1].load();
+    st.ncv_retries = g_opt_stats[2].load();
+    st.dense_fallbacks = g_opt_stats[3].load();
+    st.unconverged = g_opt_stats[4].load();
+    return st;
+}
+
 OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
     OptimizeOut out;
+    g_opt_stats[0].fetch_add(1);
     const int64_t nfull = (int64_t)a.d * a.bl * a.br;
     const int orth = a.proj->active() ? a.proj->orth : (int)nfull;
     out.iterations = a.max_iterations;
@@ -319,19 +340,27 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
             else out.matvecs += rr.matvecs;
         };
         // a Krylov basis cannot be larger than the (projected) space
-        int ncv = std::min(a.ncv, orth);
+        int ncv = std::min(a.ncv, std::min(orth, kMaxNcv));
         int iterations = a.max_iterations;
         IramResult r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
         count(r);
         iterations = r.restarts;
         bool done = false;
         if (r.info == -14) {
+            g_opt_stats[1].fetch_add(1);
             if (nfull < iterations) {
+                g_opt_stats[3].fetch_add(1);
                 strategy_1(a, op, out, s);
                 done = true;
             } else {
-                while (ncv <= iterations && ncv < orth) {
-                    ncv = std::min(ncv * 3 / 2, orth);
+                // core.f95:2588-2602: grow ncv by 3/2 while it does not exceed
+                // the restart count.  The basis is capped at the projected space
+                // and at the solver's largest basis; once the cap is reached
+                // without convergence the site keeps info -14.
+                const int cap = std::min(orth, kMaxNcv);
+                while (ncv <= iterations && ncv < cap) {
+                    ncv = std::min(std::max(ncv * 3 / 2, ncv + 1), cap);
+                    g_opt_stats[2].fetch_add(1);
                     r = iram(kop, ncv, a.which, ktol, iterations, pguess.ptr, pres.ptr, s);
                     count(r);
                     iterations = r.restarts;
@@ -344,7 +373,8 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
                 // continue in FP64 from the complex64 solution
                 ZBuf start((size_t)orth);
                 d2d(start.ptr, pres.ptr, (size_t)orth * sizeof(double2));
-                IramResult r2 = iram(op, a.ncv, a.which, a.tol, a.max_iterations, start.ptr, pres.ptr, s);
+                IramResult r2 = iram(op, std::min(a.ncv, std::min(orth, kMaxNcv)), a.which, a.tol, a.max_iterations,
+                                     start.ptr, pres.ptr, s);
                 out.matvecs += r2.matvecs;
                 if (r2.info == 0) r = r2;
             } else {
@@ -359,6 +389,7 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
             }
         }
         if (!done) {
+            if (r.info == -14) g_opt_stats[4].fetch_add(1);
             out.info = r.info;
             out.iterations = iterations;
             out.eigenvalue = r.eigenvalue;
@@ -590,7 +621,8 @@ void optimize_sites_batched(std::vector<OptimizeArgs const*> const& args, std::v
                 if (a.polish) {
                     ZBuf start((size_t)nfull);
                     d2d(start.ptr, a.result, (size_t)nfull * sizeof(double2));
-                    IramResult r2 = iram(*op64[q], a.ncv, a.which, a.tol, a.max_iterations, start.ptr, a.result, s);
+                    IramResult r2 = iram(*op64[q], std::min<int64_t>(a.ncv, std::min<int64_t>(nfull, kMaxNcv)), a.which,
+                                         a.tol, a.max_iterations, start.ptr, a.result, s);
                     o.matvecs += r2.matvecs;
                     if (r2.info == 0) r = r2;
                 } else {

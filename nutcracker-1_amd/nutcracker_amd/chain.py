@@ -192,6 +192,18 @@ class Chain:
         self._check(_lib().nc_chain_get_site(self._h, i, out.ctypes.data))
         return out
 
+    def environments(self, i: int):
+        """(left (bl, bl, cl), right (br, br, cr)) environments of the current site i."""
+        h = _lib()
+        f = h.nc_chain_get_environment
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+        L = np.zeros((self.bonds[i], self.bonds[i], self.ops[i].cl), CPLX, order="F")
+        R = np.zeros((self.bonds[i + 1], self.bonds[i + 1], self.ops[i].cr), CPLX, order="F")
+        self._check(f(self._h, i, 0, L.ctypes.data))
+        self._check(f(self._h, i, 1, R.ctypes.data))
+        return L, R
+
     # -- excited levels (chain.cpp:60-73, 301-310) ----------------------------
     def construct_and_add_projector_from_state(self):
         """Keep the current state (chain at its first site) as a lower level."""

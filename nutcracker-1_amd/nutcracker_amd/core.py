@@ -42,7 +42,18 @@ _SIGNATURES = {
     "contract_vs_left_": (None, [_u32p] * 5 + [_zp] * 4),
     "contract_vs_right_": (None, [_u32p] * 5 + [_zp] * 4),
     "form_overlap_vector_": (None, [_u32p] * 5 + [_zp] * 4),
-    "convert_vectors_to_reflectors_": (None, [_u32p, _u32p, _zp, _u32p, _zp, _u32p]),
+    "convert_vectors_to_reflectors_": (C.c_uint32, [_u32p, _u32p, _zp, _u32p, _zp, _u32p]),
+    "rand_norm_state_site_tensor_": (None, [_u32p] * 3 + [_zp]),
+    "rand_unnorm_state_site_tensor_": (None, [_u32p] * 3 + [_zp]),
+    "random_projector_matrix_": (None, [_u32p, _u32p, _u32p, _zp, _zp, _u32p]),
+    "norm_for_left_": (C.c_int32, [_u32p] * 3 + [_zp] * 2),
+    "norm_for_right_": (C.c_int32, [_u32p] * 3 + [_zp] * 2),
+    "increase_bandwidth_between_": (C.c_int32, [_u32p] * 6 + [_zp] * 4),
+    "increase_bandwidth_with_environment_": (None, [_u32p] * 4 + [_zp] * 6),
+    "construct_left_exp_boundary_": (None, [_u32p] * 2 + [_zp] * 3),
+    "construct_right_exp_boundary_": (None, [_u32p] * 2 + [_zp] * 3),
+    "contract_matrix_left_": (None, [_u32p] * 2 + [_zp] * 3),
+    "extend_state_vector_fragment_": (None, [_u32p] * 4 + [_zp] * 3),
     "form_overlap_site_tensor_": (None, [_u32p] * 3 + [_zp] * 2),
     "form_norm_overlap_tensors_": (None, [_u32p] * 5 + [_zp] * 6),
     "nc_last_error": (C.c_char_p, []),
@@ -89,6 +100,29 @@ def autotune_table() -> List[str]:
     buf = C.create_string_buffer(int(n))
     f(buf, n)
     return [ln for ln in buf.value.decode().splitlines() if ln]
+
+
+def set_optimizer_ncv(ncv: int) -> int:
+    """Krylov basis size of optimize_ (the reference's 3 by default); returns the
+    previous value."""
+    f = lib().nc_set_optimizer_ncv
+    f.restype = C.c_int
+    f.argtypes = [C.c_int]
+    prev = f(int(ncv))
+    if prev < 0:
+        raise ValueError("ncv must be in 3..62")
+    return prev
+
+
+def optimizer_stats() -> dict:
+    """Site-optimizer counters since load (see nc_optimizer_stats)."""
+    f = lib().nc_optimizer_stats
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(C.c_int64)]
+    out = (C.c_int64 * 5)()
+    f(out)
+    keys = ("solves", "first_run_failures", "ncv_retries", "dense_fallbacks", "unconverged")
+    return {k: int(out[i]) for i, k in enumerate(keys)}
 
 
 def last_error() -> str:
@@ -409,3 +443,117 @@ def form_norm_overlap_tensors(bl, bm, br, dl, dr, unnorm_1, right_norm_2):
         lib().form_norm_overlap_tensors_(_u(bl), _u(bm), _u(br), _u(dl), _u(dr), _z(U1), _z(R2), _z(lo1), _z(uo1),
                                          _z(u2), _z(ro2))
     return lo1, uo1, u2, ro2
+
+
+def rand_norm_state_site_tensor(br, bl, d) -> np.ndarray:
+    """Core::rand_norm_state_site_tensor (core.cpp:735-750): a random (br, bl, d)
+    site with sum_{r,s} psi(r,l,s) conj(psi(r,l',s)) = delta(l,l')."""
+    out = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        lib().rand_norm_state_site_tensor_(_u(br), _u(bl), _u(d), _z(out))
+    return out
+
+
+def rand_unnorm_state_site_tensor(br, bl, d) -> np.ndarray:
+    """Core::rand_unnorm_state_site_tensor (core.cpp:752-767): unit Frobenius norm."""
+    out = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        lib().rand_unnorm_state_site_tensor_(_u(br), _u(bl), _u(d), _z(out))
+    return out
+
+
+def random_projector_matrix(n, nproj):
+    """Core::random_projector_matrix (core.cpp:769-783): returns
+    (rank, reflectors(n, nproj), coefficients, swaps)."""
+    k = min(n, nproj)
+    refl = np.zeros((n, max(1, nproj)), CPLX, order="F")
+    coeffs = np.zeros(max(1, k), CPLX)
+    swaps = np.zeros(max(1, k), np.uint32)
+    rank = C.c_uint32(0)
+    with _ErrorWatch():
+        lib().random_projector_matrix_(_u(n), _u(nproj), C.byref(rank), _z(refl), _z(coeffs),
+                                       swaps.ctypes.data_as(_u32p))
+    return int(rank.value), refl[:, :nproj], coeffs[:k], swaps[:k]
+
+
+def norm_for_left(br, bl, d, site):
+    """Core::norm_for_left (core.cpp:638-655): (info, normalized (br, bl, d))."""
+    S = _zin(site, br * bl * d)
+    out = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        info = lib().norm_for_left_(_u(br), _u(bl), _u(d), _z(S), _z(out))
+    return int(info), out
+
+
+def norm_for_right(br, bl, d, site):
+    """Core::norm_for_right (core.cpp:657-674): (info, normalized (br, bl, d))."""
+    S = _zin(site, br * bl * d)
+    out = np.zeros((br, bl, d), CPLX, order="F")
+    with _ErrorWatch():
+        info = lib().norm_for_right_(_u(br), _u(bl), _u(d), _z(S), _z(out))
+    return int(info), out
+
+
+def increase_bandwidth_between(bl, bm, br, dl, dr, new_bm, left_site, right_site):
+    """Core::increase_bandwidth_between (core.cpp:548-578) for a left site
+    (bm, bl, dl) and a right site (br, bm, dr): (info, new left (new_bm, bl, dl),
+    new right (br, new_bm, dr))."""
+    A = _zin(left_site, bm * bl * dl)
+    B = _zin(right_site, br * bm * dr)
+    o1 = np.zeros((new_bm, bl, dl), CPLX, order="F")
+    o2 = np.zeros((br, new_bm, dr), CPLX, order="F")
+    with _ErrorWatch():
+        info = lib().increase_bandwidth_between_(_u(bl), _u(bm), _u(br), _u(dl), _u(dr), _u(new_bm), _z(A), _z(B),
+                                                 _z(o1), _z(o2))
+    return int(info), o1, o2
+
+
+def increase_bandwidth_with_environment(b, c, d, new_b, site, L, R):
+    """Core::increase_bandwidth_with_environment (core.cpp:785-819):
+    (new site (new_b, new_b, d), new left (new_b, new_b, c), new right)."""
+    S = _zin(site, b * b * d)
+    Lz = _zin(L, b * b * c)
+    Rz = _zin(R, b * b * c)
+    so = np.zeros((new_b, new_b, d), CPLX, order="F")
+    lo = np.zeros((new_b, new_b, c), CPLX, order="F")
+    ro = np.zeros((new_b, new_b, c), CPLX, order="F")
+    with _ErrorWatch():
+        lib().increase_bandwidth_with_environment_(_u(b), _u(c), _u(d), _u(new_b), _z(S), _z(Lz), _z(Rz), _z(so),
+                                                   _z(lo), _z(ro))
+    return so, lo, ro
+
+
+def construct_left_exp_boundary(b, c, state_boundary, operator_boundary) -> np.ndarray:
+    s = _zin(state_boundary, b)
+    o = _zin(operator_boundary, c)
+    out = np.zeros((b, b, c), CPLX, order="F")
+    with _ErrorWatch():
+        lib().construct_left_exp_boundary_(_u(b), _u(c), _z(s), _z(o), _z(out))
+    return out
+
+
+def construct_right_exp_boundary(b, c, state_boundary, operator_boundary) -> np.ndarray:
+    s = _zin(state_boundary, b)
+    o = _zin(operator_boundary, c)
+    out = np.zeros((b, b, c), CPLX, order="F")
+    with _ErrorWatch():
+        lib().construct_right_exp_boundary_(_u(b), _u(c), _z(s), _z(o), _z(out))
+    return out
+
+
+def contract_matrix_left(bl, br, L, matrix) -> np.ndarray:
+    Lz = _zin(L, bl * bl)
+    X = _zin(matrix, br * bl)
+    out = np.zeros((br, br), CPLX, order="F")
+    with _ErrorWatch():
+        lib().contract_matrix_left_(_u(bl), _u(br), _z(Lz), _z(X), _z(out))
+    return out
+
+
+def extend_state_vector_fragment(bm, br, dl, dr, old, site) -> np.ndarray:
+    O = _zin(old, bm * dl)
+    S = _zin(site, br * bm * dr)
+    out = np.zeros((br, dr, dl), CPLX, order="F")
+    with _ErrorWatch():
+        lib().extend_state_vector_fragment_(_u(bm), _u(br), _u(dl), _u(dr), _z(O), _z(S), _z(out))
+    return out

@@ -70,6 +70,23 @@ def dense_hamiltonian(ops):
     return H[0, 0]
 
 
+def tfim_zz_x_ground_energy(n, h):
+    """Ground energy of H = -sum Z_i Z_{i+1} - h sum X_i on an open chain of n
+    sites: free fermions (Lieb, Schultz, Mattis), E0 = (tr A - sum_k eps_k)/2 - n h
+    with eps_k^2 the eigenvalues of (A - B)(A + B); pinned against exact
+    diagonalisation in test_oracle.py."""
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 2 * h
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = -1.0
+        B[i, i + 1] = -1.0
+        B[i + 1, i] = 1.0
+    lam = np.sqrt(np.clip(np.linalg.eigvalsh((A - B) @ (A + B)), 0, None))
+    return 0.5 * (np.trace(A) - lam.sum()) - n * h
+
+
 def dense_mps(sites):
     """A matrix product state (site tensors (br, bl, d)) as a dense vector."""
     v = np.ones((1, 1), complex)

@@ -304,3 +304,13 @@ def test_config1_exact_levels_closed_form():
     e0 = 0.5 * (np.trace(A) - lam.sum()) - n * hx
     levels = sorted([e0, e0 + lam[0], e0 + lam[1], e0 + lam[0] + lam[1], e0 + lam[2]])[:4]
     assert np.allclose(levels, [-20.400217867027, -20.400216436515, -19.373376103501, -19.373374672990], atol=1e-9)
+
+
+def test_tfim_zz_x_closed_form_against_ed():
+    """The closed form used for the 64-chain batch against exact diagonalisation
+    of the chain's own MPO (H = -sum ZZ - h sum X)."""
+    from helpers import tfim_zz_x_ground_energy
+    from nutcracker_amd.operators import tfim_zz_x_operator
+    for n, h in [(4, 0.5), (6, 1.0), (8, 1.5), (7, 0.77)]:
+        ed = _exact_levels(tfim_zz_x_operator(n, h), 1)[0]
+        assert abs(tfim_zz_x_ground_energy(n, h) - ed) <= 1e-12 * abs(ed)
