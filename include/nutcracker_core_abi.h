@@ -241,6 +241,24 @@ void extend_state_vector_fragment_(const uint32_t* bm, const uint32_t* br, const
    a thread-local error string, sources/c-interface/errors.cpp:7-25). */
 const char* nc_last_error(void);
 
+/* contract_sos_left_ for callers that cannot allocate device memory (code under
+   HIP graph capture, serving loops that forbid hipMalloc).  No counterpart in
+   the reference.  left_environment (bl, bl, cl), state_site_tensor (br, bl, d)
+   and new_left_environment (br, br, cr) are DEVICE pointers; the block list
+   (indices, matrices) is read from host memory during the call.  The work is
+   queued on `stream` (a hipStream_t; NULL is HIP's null stream) and the call
+   returns without waiting.  The work space lives in a fixed 64 MiB device array
+   per GPU (about br*bl*d*(n_from + n_to)*16 bytes are needed: bl = br = 448 at
+   d = 2 with 5 + 5 distinct MPO bonds); larger inputs return -90 with a
+   message in nc_last_error.  No hipMalloc, hipFree or allocating library call
+   is made.  Every call on one GPU uses that GPU's array: calls that overlap on
+   one GPU race.  Returns 0 or an error code. */
+int nc_contract_sos_left_noalloc(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d,
+                                 const nc_zcomplex* left_environment, uint32_t number_of_matrices,
+                                 const uint32_t* sparse_operator_indices, const nc_zcomplex* sparse_operator_matrices,
+                                 const nc_zcomplex* state_site_tensor, nc_zcomplex* new_left_environment,
+                                 void* stream);
+
 /* Krylov basis size used by optimize_ (the reference hard-wires 3 at
    core.f95:2567; 3 is the default here too).  Accepts 3..62; returns the
    previous value, or -1 for a value outside that range. */

@@ -37,6 +37,7 @@ SOURCES = [
     "batch.cpp",
     "abi.cpp",
     "abi_support.cpp",
+    "abi_noalloc.cpp",
     "chain_abi.cpp",
     "bench_abi.cpp",
     "formats.cpp",

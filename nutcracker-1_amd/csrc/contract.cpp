@@ -15,44 +15,61 @@
 
 namespace nc {
 
+void mpo_mix_tables(int d, int nmat, const uint32_t* idx1, const zd* mats, int cl, int cr, MpoTables& t) {
+    if (d <= 0 || cl <= 0 || cr <= 0 || nmat <= 0) throw Error("operator site: bad dimensions", -2);
+    t.froms.clear();
+    t.tos.clear();
+    for (int n = 0; n < nmat; ++n) {
+        const int f = (int)idx1[2 * n] - 1, to = (int)idx1[2 * n + 1] - 1;
+        if (f < 0 || f >= cl || to < 0 || to >= cr) throw Error("operator site: block index out of range", -3);
+        t.froms.push_back(f);
+        t.tos.push_back(to);
+    }
+    std::sort(t.froms.begin(), t.froms.end());
+    t.froms.erase(std::unique(t.froms.begin(), t.froms.end()), t.froms.end());
+    std::sort(t.tos.begin(), t.tos.end());
+    t.tos.erase(std::unique(t.tos.begin(), t.tos.end()), t.tos.end());
+    t.blk_from.resize(nmat);
+    t.blk_to.resize(nmat);
+    for (int n = 0; n < nmat; ++n) {
+        const int f = (int)idx1[2 * n] - 1, to = (int)idx1[2 * n + 1] - 1;
+        t.blk_from[n] = (int)(std::lower_bound(t.froms.begin(), t.froms.end(), f) - t.froms.begin());
+        t.blk_to[n] = (int)(std::lower_bound(t.tos.begin(), t.tos.end(), to) - t.tos.begin());
+    }
+    for (int dir = 0; dir < 2; ++dir) {
+        const bool src_is_from = dir == 0;
+        const int ndst = (int)(src_is_from ? t.tos.size() : t.froms.size());
+        t.ptr[dir].assign(ndst + 1, 0);
+        t.src[dir].clear();
+        t.W[dir].clear();
+        for (int dst = 0; dst < ndst; ++dst) {
+            for (int n = 0; n < nmat; ++n) {
+                const int bd = src_is_from ? t.blk_to[n] : t.blk_from[n];
+                if (bd != dst) continue;
+                t.src[dir].push_back(src_is_from ? t.blk_from[n] : t.blk_to[n]);
+                for (int k = 0; k < d * d; ++k) t.W[dir].push_back(mats[(size_t)n * d * d + k]);
+            }
+            t.ptr[dir][dst + 1] = (int)t.src[dir].size();
+        }
+    }
+}
+
 void MpoSite::build(int d_, int cl_, int cr_, int nmat_, const uint32_t* idx1, const zd* mats_) {
     if (d_ <= 0 || cl_ <= 0 || cr_ <= 0 || nmat_ <= 0) throw Error("operator site: bad dimensions", -2);
     d = d_; cl = cl_; cr = cr_; nmat = nmat_;
     idx.assign(idx1, idx1 + 2 * (size_t)nmat);
     mats.assign(mats_, mats_ + (size_t)d * d * nmat);
-    froms.clear(); tos.clear();
-    for (int n = 0; n < nmat; ++n) {
-        int f = (int)idx[2 * n] - 1, t = (int)idx[2 * n + 1] - 1;
-        if (f < 0 || f >= cl || t < 0 || t >= cr) throw Error("operator site: block index out of range", -3);
-        froms.push_back(f);
-        tos.push_back(t);
-    }
-    std::sort(froms.begin(), froms.end());
-    froms.erase(std::unique(froms.begin(), froms.end()), froms.end());
-    std::sort(tos.begin(), tos.end());
-    tos.erase(std::unique(tos.begin(), tos.end()), tos.end());
-    blk_from.resize(nmat);
-    blk_to.resize(nmat);
-    for (int n = 0; n < nmat; ++n) {
-        int f = (int)idx[2 * n] - 1, t = (int)idx[2 * n + 1] - 1;
-        blk_from[n] = (int)(std::lower_bound(froms.begin(), froms.end(), f) - froms.begin());
-        blk_to[n] = (int)(std::lower_bound(tos.begin(), tos.end(), t) - tos.begin());
-    }
+    MpoTables t;
+    mpo_mix_tables(d, nmat, idx1, mats_, cl, cr, t);
+    froms = t.froms;
+    tos = t.tos;
+    blk_from = t.blk_from;
+    blk_to = t.blk_to;
     // mix tables, grouped by destination
     for (int dir = 0; dir < 2; ++dir) {
-        const bool src_is_from = dir == 0;
-        const int ndst = (int)(src_is_from ? tos.size() : froms.size());
-        std::vector<int> ptr(ndst + 1, 0), src;
-        std::vector<zd> w;
-        for (int dst = 0; dst < ndst; ++dst) {
-            for (int n = 0; n < nmat; ++n) {
-                int bd = src_is_from ? blk_to[n] : blk_from[n];
-                if (bd != dst) continue;
-                src.push_back(src_is_from ? blk_from[n] : blk_to[n]);
-                for (int k = 0; k < d * d; ++k) w.push_back(mats[(size_t)n * d * d + k]);
-            }
-            ptr[dst + 1] = (int)src.size();
-        }
+        std::vector<int> const& ptr = t.ptr[dir];
+        std::vector<int> const& src = t.src[dir];
+        std::vector<zd> const& w = t.W[dir];
         std::vector<float2> w32(w.size());
         for (size_t i = 0; i < w.size(); ++i) w32[i] = make_float2((float)w[i].real(), (float)w[i].imag());
         mix_ptr[dir].resize(ptr.size());

@@ -1132,6 +1132,49 @@ void zgemm_run(GemmPlan const& plan, hipStream_t s) {
     run_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, s, plan.key);
 }
 
+namespace {
+// the plan built with the default launch choice and no K split
+Built<double> fixed_plan(GemmPlan const& plan) {
+    std::vector<BatchT<double>> b(plan.batches.size());
+    std::vector<SegT<double>> sg(plan.segs.size());
+    std::memcpy(b.data(), plan.batches.data(), b.size() * sizeof(GemmBatch));
+    std::memcpy(sg.data(), plan.segs.data(), sg.size() * sizeof(GemmSeg));
+    Built<double> bp;
+    build_plan<double>(plan.opA, plan.opB, plan.accumulate, plan.M, plan.N, b, sg, bp,
+                       [](size_t) -> void* { throw Error("GEMM: a fixed launch takes no K split", -72); },
+                       GemmChoice{16, 1, -1});
+    return bp;
+}
+void fixed_layout(Built<double> const& bp, const void* parts[5], size_t sizes[5], size_t offs[5], size_t& total) {
+    parts_of(bp, parts, sizes);
+    total = 0;
+    for (int i = 0; i < 5; ++i) {
+        offs[i] = total;
+        total = (total + sizes[i] + 255) & ~size_t(255);
+    }
+}
+}  // namespace
+
+size_t zgemm_desc_bytes(GemmPlan const& plan) {
+    Built<double> bp = fixed_plan(plan);
+    const void* parts[5];
+    size_t sizes[5], offs[5], total;
+    fixed_layout(bp, parts, sizes, offs, total);
+    return total;
+}
+
+void zgemm_run_at(GemmPlan const& plan, void* desc, size_t cap, hipStream_t s) {
+    Built<double> bp = fixed_plan(plan);
+    if (bp.empty()) return;
+    const void* parts[5];
+    size_t sizes[5], offs[5], total;
+    fixed_layout(bp, parts, sizes, offs, total);
+    if (total > cap) throw Error("GEMM: descriptor lists larger than the space given", -73);
+    for (int i = 0; i < 5; ++i)
+        if (sizes[i]) store_bytes(static_cast<char*>(desc) + offs[i], parts[i], sizes[i], s);
+    launch_built(bp, dev_of<double>(static_cast<char*>(desc), offs), s);
+}
+
 static void to_float_lists(CGemmPlan const& plan, std::vector<BatchT<float>>& b, std::vector<SegT<float>>& sg) {
     for (auto const& x : plan.batches)
         b.push_back(BatchT<float>{x.C, x.ldc, x.seg_begin, x.seg_end, x.M, x.N, x.live, x.upper, 0});

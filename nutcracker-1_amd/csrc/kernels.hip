@@ -6,6 +6,7 @@
 //    the Householder projectors.
 #include "nc_internal.hpp"
 
+#include <algorithm>
 #include <cstring>
 
 namespace nc {
@@ -93,6 +94,61 @@ void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* 
 }
 void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q, int64_t rs, hipStream_t s) {
     mix_blocks_t<float2>(op, src_is_from, P, Q, rs, op.mix_W32[src_is_from ? 0 : 1].ptr, s);
+}
+
+void mix_blocks_at(int d, int ndst, const int* dst_ptr, const int* blk_src, const double2* W, const double2* P,
+                   double2* Q, int64_t rs, hipStream_t s) {
+    if (ndst == 0 || rs == 0) return;
+    MixDesc md{d, ndst, rs};
+    dim3 grid((unsigned)((rs + 255) / 256), (unsigned)ndst);
+    switch (d) {
+        case 2: hipLaunchKernelGGL((mix_kernel<double2, 2>), grid, dim3(256), 0, s, P, Q, dst_ptr, blk_src, W, md); break;
+        case 3: hipLaunchKernelGGL((mix_kernel<double2, 3>), grid, dim3(256), 0, s, P, Q, dst_ptr, blk_src, W, md); break;
+        case 4: hipLaunchKernelGGL((mix_kernel<double2, 4>), grid, dim3(256), 0, s, P, Q, dst_ptr, blk_src, W, md); break;
+        default: hipLaunchKernelGGL((mix_kernel<double2, 0>), grid, dim3(256), 0, s, P, Q, dst_ptr, blk_src, W, md); break;
+    }
+    NC_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------
+// Host bytes into device memory through kernel arguments: the launch copies the
+// argument block, so the host buffer may change as soon as the call returns,
+// and nothing is allocated or staged in pinned memory.
+// ----------------------------------------------------------------------------
+namespace {
+constexpr int kArgBlob = 2048;
+struct ArgBlob {
+    unsigned char bytes[kArgBlob];
+};
+__global__ __launch_bounds__(256) void store_blob_kernel(ArgBlob b, unsigned char* __restrict__ dst, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = b.bytes[i];
+}
+}  // namespace
+
+void store_bytes(void* dst, const void* src, size_t n, hipStream_t s) {
+    const unsigned char* p = static_cast<const unsigned char*>(src);
+    unsigned char* q = static_cast<unsigned char*>(dst);
+    for (size_t off = 0; off < n; off += kArgBlob) {
+        ArgBlob b;
+        const int len = (int)std::min<size_t>(kArgBlob, n - off);
+        std::memcpy(b.bytes, p + off, (size_t)len);
+        hipLaunchKernelGGL(store_blob_kernel, dim3(1), dim3(256), 0, s, b, q + off, len);
+        NC_HIP(hipGetLastError());
+    }
+}
+
+namespace {
+__global__ __launch_bounds__(256) void zero_fill_kernel(double2* __restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = make_double2(0.0, 0.0);
+}
+}  // namespace
+
+void zero_fill(double2* x, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(zero_fill_kernel, dim3(g), dim3(256), 0, s, x, n);
+    NC_HIP(hipGetLastError());
 }
 
 // the same mixing for several chains: blockIdx.z selects the chain

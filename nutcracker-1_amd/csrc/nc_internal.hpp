@@ -169,6 +169,18 @@ struct GemmPlan {
 
 // Upload-and-launch (descriptor arena is stream ordered).
 void zgemm_run(GemmPlan const& plan, hipStream_t s);
+// The same plan with its descriptor lists written into caller-provided device
+// memory (`cap` bytes at `desc`, zgemm_desc_bytes of them needed) by
+// stream-ordered store kernels; the default launch choice, no autotuning and
+// no K split, so nothing is allocated.
+size_t zgemm_desc_bytes(GemmPlan const& plan);
+void zgemm_run_at(GemmPlan const& plan, void* desc, size_t cap, hipStream_t s);
+
+// n host bytes into device memory at dst through kernel arguments (stream
+// ordered; the host buffer may be reused as soon as the call returns)
+void store_bytes(void* dst, const void* src, size_t n, hipStream_t s);
+// x[0 .. n) = 0 with a fill kernel
+void zero_fill(double2* x, int64_t n, hipStream_t s);
 
 // The same for complex64 operands (FP32 MFMA), used by the mixed-precision form.
 struct CGemmSeg {
@@ -231,6 +243,15 @@ struct MpoSite {
     void build(int d, int cl, int cr, int nmat, const uint32_t* idx1, const zd* mats);
 };
 
+// The host side of MpoSite::build: distinct from / to indices, each block's
+// position in them, and per direction the destination-grouped mix tables.
+struct MpoTables {
+    std::vector<int> froms, tos, blk_from, blk_to;
+    std::vector<int> ptr[2], src[2];
+    std::vector<zd> W[2];
+};
+void mpo_mix_tables(int d, int nmat, const uint32_t* idx1, const zd* mats, int cl, int cr, MpoTables& t);
+
 // Mix kernel: Q(e, b, dst) = sum_{n : dst(n) = dst} sum_a W_n(a, b) P(e, a, src(n))
 // where e runs over an element slab of size rs.  P has d * nsrc slabs, Q d * ndst.
 // Small descriptor lists for a launch: copied through a pinned ring with a
@@ -259,6 +280,10 @@ void axpby_batch(std::vector<double2*> const& y, std::vector<zd> const& a, std::
 
 void mix_blocks(MpoSite const& op, bool src_is_from, const double2* P, double2* Q,
                 int64_t rs, hipStream_t s);
+// the same with the destination-grouped tables (ndst + 1 offsets, source slab
+// per block, blocks W) given as device pointers
+void mix_blocks_at(int d, int ndst, const int* dst_ptr, const int* blk_src, const double2* W, const double2* P,
+                   double2* Q, int64_t rs, hipStream_t s);
 void mix_blocks(MpoSite const& op, bool src_is_from, const float2* P, float2* Q,
                 int64_t rs, hipStream_t s);
 
