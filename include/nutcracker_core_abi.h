@@ -363,6 +363,10 @@ int nc_chain_add_projector(nc_chain* chain);
 int nc_chain_clear_projectors(nc_chain* chain);
 int nc_chain_projector_overlap(nc_chain* chain, double* overlap);
 int nc_chain_randomize(nc_chain* chain, uint64_t seed);
+/* reset (chain.cpp:175-237) for the next level: the bond dimension goes back to
+   the initial one (with growth) raised to the minimum that leaves room outside
+   the lower levels, then new raw sites from `seed` are canonicalised. */
+int nc_chain_restart_level(nc_chain* chain, uint64_t seed);
 int nc_chain_solve_for_multiple_levels(nc_chain* chain, uint32_t levels, uint64_t seed, double* energies);
 /* <a|b> of two chains' states (computeStateOverlap, states.hpp:349-382): the
    chains need the same length and physical dimensions, not the same bonds. */

@@ -206,6 +206,10 @@ int nc_chain_randomize(nc_chain* c, uint64_t seed) {
     return guarded_chain("nc_chain_randomize", [&] { c->chain->randomize(seed); return 0; });
 }
 
+int nc_chain_restart_level(nc_chain* c, uint64_t seed) {
+    return guarded_chain("nc_chain_restart_level", [&] { c->chain->restart_level(seed); return 0; });
+}
+
 int nc_chain_solve_for_multiple_levels(nc_chain* c, uint32_t levels, uint64_t seed, double* energies) {
     return guarded_chain("nc_chain_solve_for_multiple_levels", [&] {
         std::vector<double> e = c->chain->solve_for_multiple_levels((int)levels, seed);

@@ -4,7 +4,10 @@
 // mode 1, iparam(1) = 1, info = 1 so the start vector is the caller's guess):
 //   * Arnoldi steps with classical Gram-Schmidt applied twice (CGS2; znaitr
 //     applies the second pass when the first loses more than 0.717 of the
-//     norm -- here it always runs, which is never less orthogonal);
+//     norm -- here it always runs, which is never less orthogonal), a third
+//     pass when the second loses as much, and znaitr's rule for the residual:
+//     zero (an invariant subspace) only when the last pass taken also lost
+//     that much, otherwise kept as the next direction however small;
 //   * Ritz pairs of the small Hessenberg H on the host; convergence when the
 //     Ritz estimate |beta e_m^T s| <= tol max(eps^(2/3), |theta|) (znconv);
 //   * restart by ncv - 1 shifted QR steps with the unwanted Ritz values
@@ -45,12 +48,14 @@ __device__ __forceinline__ double kwarp_sum(double v) {
 }
 
 // partial dots of columns 0..j+1 of V against w = column j+1 (the last one is
-// ||w||^2): partial[((p * ldp) + c) * nblk + b]
+// ||w||^2): partial[((p * ldp) + c) * nblk + b].  gate (optional): problems
+// whose flag is 0 skip the step.
 __global__ __launch_bounds__(KT) void kdot_kernel(const KDesc* __restrict__ desc, int j, double2* __restrict__ partial,
-                                                  int ldp) {
+                                                  int ldp, const int* __restrict__ gate) {
     const int p = blockIdx.y;
     const KDesc d = desc[p];
     if (j < d.kstart) return;
+    if (gate && gate[p] == 0) return;
     const int m = j + 2;
     const int64_t n = d.n;
     const double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
@@ -97,9 +102,10 @@ __global__ __launch_bounds__(KT) void kdot_kernel(const KDesc* __restrict__ desc
 // out[p * ldp + c] = sum_b partial[((p * ldp) + c) * nblk + b], c = 0..j+1
 __global__ __launch_bounds__(KT) void kreduce_kernel(const KDesc* __restrict__ desc, int j,
                                                      const double2* __restrict__ partial, int ldp, int nblk,
-                                                     double2* __restrict__ out) {
+                                                     double2* __restrict__ out, const int* __restrict__ gate) {
     const int p = blockIdx.y, c = blockIdx.x;
     if (j < desc[p].kstart || c > j + 1) return;
+    if (gate && gate[p] == 0) return;
     __shared__ double sre[KT / 64], sim[KT / 64];
     double re = 0, im = 0;
     for (int b = threadIdx.x; b < nblk; b += KT) {
@@ -121,10 +127,11 @@ __global__ __launch_bounds__(KT) void kreduce_kernel(const KDesc* __restrict__ d
 // w -= sum_{c <= j} V_c coef_c, and partial ||w||^2 per block
 __global__ __launch_bounds__(KT) void kupdate_kernel(const KDesc* __restrict__ desc, int j,
                                                      const double2* __restrict__ coef, int ldp,
-                                                     double* __restrict__ pnorm) {
+                                                     double* __restrict__ pnorm, const int* __restrict__ gate) {
     const int p = blockIdx.y;
     const KDesc d = desc[p];
     if (j < d.kstart) return;
+    if (gate && gate[p] == 0) return;
     __shared__ double2 sc[64];
     __shared__ double sn[KT / 64];
     const int m = j + 1;
@@ -227,29 +234,69 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
     }
 }
 
-// Hessenberg column j = coef1 + coef2, beta_j = ||w||; beta = 0 when the new
-// vector is round-off of an invariant subspace; scale[p] = 1 / beta (or 0).
+// Hessenberg column j = coef1 + coef2, beta_j = ||w||; scale[p] = 1 / beta (or 0).
+// The new vector lies in the span of the basis (beta = 0) when the second
+// Gram-Schmidt pass removes more than 1 - 0.717 of what the first pass left
+// (znaitr's test, steps 4-5: ARPACK then tries a third pass and zeroes the
+// residual when that one loses as much).  A residual that survives the second
+// pass is kept as the next direction however small it is, as znaitr keeps it:
+// when the start vector is an exact eigenvector, that round-off residual is the
+// next Krylov direction.
 // One 64-lane block per problem: the partial norms are added lane-strided, then
 // across the lanes, in a fixed order.
-__global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv,
-                                                     const double2* __restrict__ coef1,
-                                                     const double2* __restrict__ coef2, int ldp,
-                                                     const double* __restrict__ pnorm, int nblk,
-                                                     double2* __restrict__ H, double* __restrict__ betas,
-                                                     double* __restrict__ scale) {
+// After the second pass: ||w|| (beta2, kept in b2s) and whether a third pass
+// is needed (the second one removed more than 1 - 0.717 of what the first left;
+// znaitr step 5 then refines once more).
+__global__ __launch_bounds__(64) void kcheck_kernel(const KDesc* __restrict__ desc, int j,
+                                                    const double2* __restrict__ coef2, int ldp,
+                                                    const double* __restrict__ pnorm, int nblk,
+                                                    int* __restrict__ need3, double* __restrict__ b2s) {
     const int p = blockIdx.x;
     if (j < desc[p].kstart) return;
     double b2 = 0;
     for (int b = threadIdx.x; b < nblk; b += 64) b2 += pnorm[(int64_t)p * nblk + b];
     for (int off = 32; off > 0; off >>= 1) b2 += __shfl_xor(b2, off, 64);
     if (threadIdx.x != 0) return;
-    double beta = sqrt(b2);
-    const double wn = sqrt(fmax(0.0, coef1[(int64_t)p * ldp + j + 1].x));
-    if (!(beta >= 1e-14 * wn)) beta = 0.0;
+    const double beta2 = sqrt(b2);
+    const double beta1 = sqrt(fmax(0.0, coef2[(int64_t)p * ldp + j + 1].x));
+    b2s[p] = beta2;
+    need3[p] = (beta2 > 0.0 && !(beta2 > 0.717 * beta1)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ desc, int j, int ncv,
+                                                     const double2* __restrict__ coef1,
+                                                     const double2* __restrict__ coef2,
+                                                     const double2* __restrict__ coef3, int ldp,
+                                                     const double* __restrict__ pnorm, int nblk,
+                                                     const int* __restrict__ need3, const double* __restrict__ b2s,
+                                                     double2* __restrict__ H, double* __restrict__ betas,
+                                                     double* __restrict__ scale) {
+    const int p = blockIdx.x;
+    if (j < desc[p].kstart) return;
+    const bool third = need3[p] != 0;
+    double b2 = 0;
+    if (third) {
+        for (int b = threadIdx.x; b < nblk; b += 64) b2 += pnorm[(int64_t)p * nblk + b];
+        for (int off = 32; off > 0; off >>= 1) b2 += __shfl_xor(b2, off, 64);
+    }
+    if (threadIdx.x != 0) return;
+    // beta = ||w|| after the last pass taken; zero when that pass also removed
+    // more than 1 - 0.717 of its input (the residual lies in the basis span)
+    double beta = b2s[p];
+    if (third) {
+        const double beta3 = sqrt(b2);
+        beta = (beta3 > 0.717 * beta) ? beta3 : 0.0;
+    }
     double2* Hp = H + (int64_t)p * (ncv + 1) * ncv;
     for (int r = 0; r <= j; ++r) {
         const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
-        Hp[r + (int64_t)j * (ncv + 1)] = make_double2(a.x + c.x, a.y + c.y);
+        double2 h = make_double2(a.x + c.x, a.y + c.y);
+        if (third) {
+            const double2 e = coef3[(int64_t)p * ldp + r];
+            h.x += e.x;
+            h.y += e.y;
+        }
+        Hp[r + (int64_t)j * (ncv + 1)] = h;
     }
     Hp[(j + 1) + (int64_t)j * (ncv + 1)] = make_double2(beta, 0.0);
     betas[(int64_t)p * ncv + j] = beta;
@@ -507,7 +554,9 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((nmax + 4 * KT - 1) / (4 * KT),
                                                                    std::max(1, 2048 / P)));
     const int ldp = ncv + 2;
-    ZBuf partial((size_t)P * ldp * nblk), coef1((size_t)P * ldp), coef2((size_t)P * ldp);
+    ZBuf partial((size_t)P * ldp * nblk), coef1((size_t)P * ldp), coef2((size_t)P * ldp), coef3((size_t)P * ldp);
+    DevBuf<int> need3(P);
+    DevBuf<double> b2s(P);
     // Hessenberg columns and subdiagonals of all problems in one buffer, read
     // back with one copy per extension
     const size_t nH = (size_t)P * (ncv + 1) * ncv, nB = (size_t)P * ncv;
@@ -555,20 +604,28 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     // the Gram-Schmidt kernels of step j (classical Gram-Schmidt twice; the
     // first update and the second pass's dots share one read of the basis)
     auto orthogonalize = [&](int j) {
-        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
-        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr);
+        const int* ng = nullptr;
+        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, ng);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
         const int m = j + 1;
         if (m <= 4) hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
         else if (m <= 8) hipLaunchKernelGGL(kupdot_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
         else if (m <= 16) hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
         else {
-            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr);
-            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+            hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
+            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, ng);
         }
-        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr);
-        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr);
-        hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr, ldp,
-                           pnorm.ptr, nblk, Hd_ptr, betas_ptr, scale.ptr);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr, ng);
+        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, ng);
+        // znaitr's third pass, only for the problems whose second pass lost too much
+        hipLaunchKernelGGL(kcheck_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, nblk,
+                           need3.ptr, b2s.ptr);
+        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, need3.ptr);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef3.ptr,
+                           need3.ptr);
+        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
+        hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr,
+                           coef3.ptr, ldp, pnorm.ptr, nblk, need3.ptr, b2s.ptr, Hd_ptr, betas_ptr, scale.ptr);
         hipLaunchKernelGGL(kscale_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, ncv, scale.ptr);
         NC_HIP(hipGetLastError());
     };

@@ -203,6 +203,16 @@ __global__ void bnoise_kernel(const BItem* __restrict__ it) {
     }
 }
 
+// X(i, i) += a for i < m, X an m x n matrix (n >= m, leading dimension m)
+__global__ void bdiag_kernel(const BItem* __restrict__ it) {
+    const BItem d = it[blockIdx.y];
+    const int64_t m = d.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = d.x[i + i * m];
+        d.x[i + i * m] = make_double2(v.x + d.a, v.y);
+    }
+}
+
 __global__ void bherm_kernel(const BItem* __restrict__ it) {
     const BItem d = it[blockIdx.y];
     const int64_t m = d.n, total = m * m;
@@ -404,12 +414,19 @@ void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, 
 // so iterates near convergence (and nearly unitary input) take plain alpha = 1
 // steps; a scaled step there would map sigma = 1 down to p(1).
 // A site after optimisation is often numerically rank deficient (Schmidt
-// values far below round-off); its polar factor is then not determined in
-// those directions, and the iteration would stall there.  A deterministic
-// perturbation of Frobenius norm 1e-14 ||A|| floors the singular values near
-// 1e-16 ||A||, the schedule starts from l = 1e-16 and converges in about 45
-// steps for any input; the two-site state changes by ~1e-14 relative
-// (P = A U_p^H below, kept Hermitian).
+// values far below round-off), and right after a bond grows it is exactly so;
+// its polar factor is then not determined in those directions, and the
+// iteration would stall there.  A perturbation of Frobenius norm 1e-14 ||A||
+// along the leading diagonal, E = eps [I 0], floors the singular values near
+// 1e-16 ||A|| and completes the null directions with the projections of the
+// low-index unit vectors, as LAPACK's zgesdd completes them with the first
+// basis vectors its reflectors leave (the reference's mysvd, core.f95:1-55):
+// after a bond grows, the new directions are low-index basis states, not
+// random ones.  A hashed perturbation a hundred times smaller keeps the floor
+// where the unit vectors have no component in the null space.  The schedule
+// starts from l = 1e-16 and converges in about 45 steps for any input; the
+// two-site state changes by ~1e-14 relative (P = A U_p^H below, kept
+// Hermitian).
 // The Gram matrix of each step gives ||X X^H - I||_F for free; being
 // Hermitian, only its upper tiles are computed (a step costs 1.5 GEMMs of
 // m x m x n instead of 2).  Only GEMMs of
@@ -436,8 +453,8 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
         ns.push_back((int64_t)j.m * j.n);
     }
     std::vector<double> a2 = sumsq_batch(xs, ns, s);
-    std::vector<BItem> noise;
-    int64_t maxn = 0;
+    std::vector<BItem> noise, diag;
+    int64_t maxn = 0, maxrows = 0;
     for (size_t q = 0; q < J; ++q) {
         const double nrm = std::sqrt(a2[q]);
         jobs[q].info = 0;
@@ -449,9 +466,12 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
         st[q].X1.resize(total);
         st[q].G.resize((size_t)jobs[q].m * jobs[q].m);
         d2d(st[q].X0.ptr, jobs[q].A, (size_t)total * sizeof(double2));
-        noise.push_back(BItem{st[q].X0.ptr, total, 1e-14 * nrm / std::sqrt((double)total) * std::sqrt(1.5), 0});
+        noise.push_back(BItem{st[q].X0.ptr, total, 1e-16 * nrm / std::sqrt((double)total) * std::sqrt(1.5), 0});
+        diag.push_back(BItem{st[q].X0.ptr, (int64_t)jobs[q].m, 1e-14 * nrm / std::sqrt((double)jobs[q].m), 0});
         maxn = std::max(maxn, total);
+        maxrows = std::max(maxrows, (int64_t)jobs[q].m);
     }
+    launch_items(bdiag_kernel, diag, maxrows, s);
     launch_items(bnoise_kernel, noise, maxn, s);
     {
         std::vector<const double2*> x0;

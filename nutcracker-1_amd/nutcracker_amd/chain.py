@@ -222,6 +222,16 @@ class Chain:
         self._check(_lib().nc_chain_randomize(self._h, seed))
         self._refresh_bonds()
 
+    def restart_level(self, seed: int):
+        """reset (chain.cpp:175-237) for the next level: the initial bond
+        dimension (with growth) raised to the minimum that leaves room outside the
+        lower levels, and a new random state from ``seed``."""
+        f = _lib().nc_chain_restart_level
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_uint64]
+        self._check(f(self._h, seed))
+        self._refresh_bonds()
+
     def solve_for_multiple_levels(self, levels: int, seed: int = 1) -> List[float]:
         """Converge ``levels`` levels in turn at the chain's bond dimension, each
         orthogonal to the ones before; returns their energies."""

@@ -56,7 +56,9 @@ def test_sweeps_match_oracle(mods, model):
     dims = [o.d for o in ops]
     raw = fixtures.raw_state(11, dims, ch.bonds)
     oc_ = OracleChain(ops, ch.bonds, raw, rethrow=True)
-    assert abs(ch.energy - oc_.energy) <= 1e-12 * abs(oc_.energy)
+    # a random state's energy can lie near zero: its error is measured against
+    # the operator's scale (one unit per site) there
+    assert abs(ch.energy - oc_.energy) <= 1e-12 * max(abs(oc_.energy), len(ops))
     for sweep in range(3):
         ch.perform_optimization_sweep()
         oc_.sweep()

@@ -183,20 +183,28 @@ def test_tfim_four_levels(mods, case):
 
 @pytest.mark.parametrize("d,n,levels", [(2, 1, [0, 1]), (2, 2, [0, 1, 1, 2]), (2, 4, [0, 1, 1, 1]),
                                         (3, 1, [0, 1, 2]), (3, 2, [0, 1, 1, 2, 2]), (3, 3, [0, 1, 1, 1, 2])])
-def test_external_field_levels(mods, d, n, levels):
+@pytest.mark.parametrize("grow", [True, False])
+def test_external_field_levels(mods, d, n, levels, grow):
     """tests/chain.cpp:496-517: external field diag(0, 1, ..., d-1) on every site,
-    levels to 1e-12 (the reference loosens the sanity threshold 10x for d = n = 3).
+    levels to 1e-12 (the reference loosens the sanity threshold 10x for d = n = 3),
+    with the reference's hard-wired ncv = 3, and either its bond-dimension schedule
+    (chi0 = 1, grown by one per optimizeChain step) or the exact bond dimension.
 
-    The levels are exactly degenerate multiplets.  Once a sweep's guess is an exact
-    eigenvector of a higher multiplet, an ARPACK-style solve with ncv = 3 can stop
-    on it (its Ritz estimate is zero and the two other Krylov vectors may all lie
-    above it); measured over 12 seeds, ncv = 3 missed the fourth level of N = 4,
-    d = 2 for 6 of them, ncv = 6 for none.  The chain runs at its exact bond
-    dimension with ncv = 6."""
+    The levels are exactly degenerate multiplets, so a sweep often holds an exact
+    eigenvector of some multiplet.  Two behaviours of the reference decide which
+    level the chain reaches from there: znaitr keeps a residual that survives the
+    second Gram-Schmidt pass as the next Krylov direction (however small; with
+    the guess an exact eigenvector that round-off direction is how the solve
+    leaves a higher multiplet), and zgesdd completes the null directions of a
+    rank-deficient site (right after the bond grows) with low-index basis
+    vectors, which for this operator are its low-energy states."""
     chain, ops_mod = mods
     ops = ops_mod.external_field_operator(n, np.diag(np.arange(d)).astype(complex))
     sanity = 1e-11 if (d == 3 and n == 3) else 1e-12
-    ch = chain.Chain(ops, d ** n, seed=5, rethrow=True, sanity=sanity, ncv=6)
+    if grow:
+        ch = chain.Chain(ops, 1, seed=5, rethrow=True, sanity=sanity, ncv=3, grow=True)
+    else:
+        ch = chain.Chain(ops, d ** n, seed=5, rethrow=True, sanity=sanity, ncv=3)
     got = ch.solve_for_multiple_levels(len(levels), seed=9)
     print("levels", got, flush=True)
     assert np.allclose(got, levels, atol=1e-12, rtol=0)
