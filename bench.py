@@ -13,26 +13,26 @@ dimension chi = 1024.
   the kernels run on.
 * Sweep: two chains on the same seeded right-canonical initial state, one FP64
   and one mixed (complex64 matvecs, FP64 environments, Krylov vectors, Rayleigh
-  quotient and site moves).  Each is walked to the middle of the chain and then
-  optimises a window of bulk sites (optimise + move right), timed on the host
-  around synchronised library calls.  Sweep seconds are EXTRAPOLATED as
-  2 (N - 1) x the mean bulk site step; |dE| is the difference of the two chains'
-  energies after each site solve of the window, and the state infidelity
-  1 - |<psi_mixed|psi_FP64>|^2 is taken after the window.
+  quotient and site moves), each swept once end to end (performOptimizationSweep:
+  right to the last site and back), timed on the host around synchronised
+  library calls; |dE| is the difference of the two chains' energies after the
+  sweep and the state infidelity 1 - |<psi_mixed|psi_FP64>|^2.  Per level, the
+  same two forms are compared on a TFIM chain with three levels.
 * Batching: 64 TFIM chains (N = 32, h and chi drawn from a seed) sweep once in
   lockstep with grouped launches; a sample of them also sweeps one chain after
   another, for the batching speedup.
 
-    python bench.py [--steps K] [--warmup W] [--chi 1024] [--window 3] [--no-sweep] [--no-batch] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--chi 1024] [--no-sweep] [--window-sweep] [--no-batch] [--no-cpu]
 
 Prints one JSON line.  The roofline object describes the dominant kernel of the
 FP64 path (the grouped complex FP64 GEMM, two launches per matvec): `achieved`
-counts the algorithmic 8 real flops per complex multiply-add; the launches run
-Gauss's 3-multiply product, which executes 3/4 of them on the matrix cores, so
-`mfma_tflops` / `mfma_frac` give the work the matrix cores actually do;
-cpu_baseline times the oracle's restatement of the reference's strategy-3 matvec
-(dense stage-1 tensor + two zgemms through OpenBLAS) on a bounded sample on the
-host cores.
+counts the real flops the launch's complex product form performs (Gauss's
+3-multiply form: 6 per complex multiply-add), so frac is the share of the FP64
+matrix-core peak in use; the reference zgemm's 8-flop count of the same work is
+given beside it.  cpu_baseline times the oracle's restatement of the
+reference's strategy-3 matvec (dense stage-1 tensor + two zgemms through
+OpenBLAS on every host core), its environment updates and its zgesdd site move
+on a bounded sample, and extrapolates the sweep from them.
 """
 
 from __future__ import annotations
@@ -227,9 +227,11 @@ def main():
     ap.add_argument("--ncv", type=int, default=16)
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-sweep", action="store_true")
-    ap.add_argument("--full-sweep", action="store_true",
-                    help="time one whole chi=1024 sweep in FP64 and in the mixed form (about 7 minutes)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the whole chi=1024 sweeps")
+    ap.add_argument("--window-sweep", action="store_true",
+                    help="also time a window of bulk site steps (optimise + move) in both forms")
+    ap.add_argument("--levels", type=int, default=3,
+                    help="levels of the per-level precision comparison (XXZ N=20, chi=64); 0 skips it")
     ap.add_argument("--config2-sweeps", type=int, default=6,
                     help="full sweeps of the chi=256 XXZ chain (config 2), FP64 and mixed; 0 skips them")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
@@ -286,28 +288,29 @@ def main():
     f_gemm_c = 8 * br * br * bl * d * n_to
     f_gemm_launch = (f_gemm_a + f_gemm_c) / 2
 
-    def roof(ph, peak):
-        avg_ms = (ph[0] + ph[2]) / 2
-        ach = f_gemm_launch / (avg_ms * 1e-3) / 1e12
-        return avg_ms, ach, ach / peak
-
-    g64_ms, g64_tf, g64_frac = roof(ph64, FP64_MFMA_PEAK_TFLOPS)
-    g32_ms, g32_tf, g32_frac = roof(ph32, FP32_MFMA_PEAK_TFLOPS)
     traffic = measured_traffic()
     # complex product form the launches ran (3M executes 3 of the 4 real products)
     tune_lines = core.autotune_table()
     form64 = product_form(tune_lines, "fp64", chi, d)
     form32 = product_form(tune_lines, "fp32", chi, d)
 
-    def mfma_fields(form, tf, peak):
-        # matrix-core work executed per algorithmic flop, and the peak it runs against
-        # (bf16x6: 3M on the bf16 matrix cores, six bf16 partial products per real one)
-        if form not in ("3M", "4M", "3M+4M", "bf16x6"):
-            return {"complex_product": form}
+    def roofline_of(form, ph, peak, kernel):
+        """The dominant GEMM launch against the peak of the matrix cores it runs
+        on.  `achieved` counts the real flops the launch's complex product form
+        performs: 6 per complex multiply-add for Gauss's 3-multiply form (3 real
+        multiply-adds), 8 for the 4-multiply form, and for the bf16 split kernel
+        36 bf16 flops (3 real products x 6 bf16 partial products); the
+        4-multiply count of the same work (the reference zgemm's 8 flops) is
+        given beside it."""
         share, mpeak = {"3M": (0.75, peak), "4M": (1.0, peak), "3M+4M": (0.875, peak),
-                        "bf16x6": (4.5, BF16_MFMA_PEAK_TFLOPS)}[form]
-        return {"complex_product": form, "mfma_flops_per_launch": int(share * f_gemm_launch),
-                "mfma_tflops": round(share * tf, 3), "mfma_peak": mpeak, "mfma_frac": round(share * tf / mpeak, 4)}
+                        "bf16x6": (4.5, BF16_MFMA_PEAK_TFLOPS)}.get(form, (1.0, peak))
+        avg_ms = (ph[0] + ph[2]) / 2
+        tf4 = f_gemm_launch / (avg_ms * 1e-3) / 1e12
+        ach = share * tf4
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": mpeak, "unit": "TFLOP/s",
+                "frac": round(ach / mpeak, 4), "kernel": kernel, "complex_product": form,
+                "flops_per_launch": int(share * f_gemm_launch), "avg_launch_ms": round(avg_ms, 4),
+                "flops_per_launch_4m_count": int(f_gemm_launch), "tflops_4m_count": round(tf4, 3)}
 
     result = {
         "metric": METRIC,
@@ -323,28 +326,22 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "XXZ chain J=1 Delta=%g N=%d d=2 MPO bond 5 at chi=%d: bulk-site H_eff matvec "
-                               "(value) and a bulk sweep window" % (args.delta, args.sites, chi),
+                               "(value) and one whole sweep in each form" % (args.delta, args.sites, chi),
                    "chi": chi, "d": d, "mpo_bond": op.cl, "mpo_blocks": op.nmat,
                    "form": "value = plain FP64 complex drop-in; 'mixed' = complex64 matvec on FP32 MFMA"},
         "matvec_tflops": round(f_mv / (ms64 * 1e-3) / 1e12, 3),
         "phase_ms": {"gemm_step_a": round(ph64[0], 4), "mix": round(ph64[1], 4), "gemm_step_c": round(ph64[2], 4)},
-        "roofline": {"bound": "mfma", "achieved": round(g64_tf, 3), "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(g64_frac, 4),
-                     "traffic": traffic, "kernel": "cgemm_kernel (complex128, FP64 MFMA)",
-                     "flops_per_launch": int(f_gemm_launch), "avg_launch_ms": round(g64_ms, 4),
-                     **mfma_fields(form64, g64_tf, FP64_MFMA_PEAK_TFLOPS)},
+        "roofline": {**roofline_of(form64, ph64, FP64_MFMA_PEAK_TFLOPS, "cgemm_kernel (complex128, FP64 MFMA)"),
+                     "traffic": traffic},
         "mixed": {
             "matvecs_per_s": round(1000.0 / ms32, 3), "ms_per_matvec": round(ms32, 4),
             "matvec_speedup_vs_fp64": round(ms64 / ms32, 3),
             "matvec_rel_err_vs_fp64": err32,
             "phase_ms": {"gemm_step_a": round(ph32[0], 4), "mix": round(ph32[1], 4),
                          "gemm_step_c": round(ph32[2], 4)},
-            "roofline": {"bound": "mfma", "achieved": round(g32_tf, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(g32_frac, 4),
-                         "kernel": ("cgemm_b6_kernel (complex64 split into bf16 parts, bf16 MFMA)"
+            "roofline": roofline_of(form32, ph32, FP32_MFMA_PEAK_TFLOPS,
+                                    "cgemm_b6_kernel (complex64 split into bf16 parts, bf16 MFMA)"
                                     if form32 == "bf16x6" else "cgemm_kernel (complex64, FP32 MFMA)"),
-                         "avg_launch_ms": round(g32_ms, 4),
-                         **mfma_fields(form32, g32_tf, FP32_MFMA_PEAK_TFLOPS)},
         },
         "copy_calibration": {"bytes": copy_bytes, "avg_ms": round(copy_ms.value, 4),
                              "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},
@@ -375,8 +372,8 @@ def main():
             del cL, cR, cx
         result["other_configs_matvec"] = others
 
-    # ---- sweep window ----------------------------------------------------
-    if not args.no_sweep:
+    # ---- sweep window (optional) ------------------------------------------
+    if args.window_sweep:
         ops = xxz_operator(args.sites, d, 1.0, args.delta)
         start = min(args.window_start, args.sites - 1 - args.window)
         c64, w64 = sweep_window(chain_mod, ops, chi, args.seed, args.ncv, start, args.window, mixed=False)
@@ -411,8 +408,8 @@ def main():
             "mixed": {"step_s": round(smx, 4), "reset_s": round(wmx["reset_s"], 2), "sites": wmx["sites"]},
         }
 
-    # ---- optional: one whole chi = 1024 sweep in both forms -----------------
-    if args.full_sweep:
+    # ---- one whole chi = 1024 sweep in both forms (measured, not extrapolated)
+    if not args.no_sweep:
         ops = xxz_operator(args.sites, d, 1.0, args.delta)
         res = {}
         chs = {}
@@ -434,6 +431,39 @@ def main():
             % (res["fp64"]["sweep_s"], res["mixed"]["sweep_s"], res["mixed_speedup"], res["abs_dE"],
                res["state_infidelity"]))
         result["full_sweep"] = res
+        result["sweep"] = {"fp64_sweep_s": res["fp64"]["sweep_s"], "mixed_sweep_s": res["mixed"]["sweep_s"],
+                           "mixed_speedup": res["mixed_speedup"], "abs_dE_per_site": res["abs_dE_per_site"],
+                           "state_infidelity": res["state_infidelity"], "extrapolated": False}
+
+    # ---- precision form per level: |E - E_FP64| and infidelity ------------
+    if args.levels > 0:
+        from nutcracker_amd.operators import tfim_zz_x_operator
+        # TFIM in its paramagnetic phase: every level is non-degenerate, so the
+        # state of a level (and the infidelity between the forms) is defined
+        lops = tfim_zz_x_operator(20, 1.5)
+        pair = [chain_mod.Chain(lops, 32, seed=args.seed + 5, ncv=args.ncv, rethrow=True, mixed=m)
+                for m in (False, True)]
+        per = []
+        t0 = time.perf_counter()
+        for k in range(args.levels):
+            if k:
+                for ch in pair:
+                    ch.construct_and_add_projector_from_state()
+                    ch.restart_level(args.seed + 50 + k)
+            for ch in pair:
+                ch.sweep_until_converged()
+            e64, emx = pair[0].energy, pair[1].energy
+            ov = pair[0].compute_state_overlap(pair[1])
+            per.append({"level": k, "energy_fp64": e64, "energy_mixed": emx, "abs_dE": abs(e64 - emx),
+                        "state_infidelity": float(1 - abs(ov) ** 2)})
+            log("level %d: E fp64 %.12f mixed %.12f |dE| %.2e infidelity %.2e"
+                % (k, e64, emx, abs(e64 - emx), per[-1]["state_infidelity"]))
+        for ch in pair:
+            ch.close()
+        result["precision_per_level"] = {
+            "workload": "TFIM H = -sum ZZ - 1.5 sum X, N=20, chi=32, %d levels through projectors, FP64 and mixed "
+                        "chains from the same seeded states, each swept to convergence" % args.levels,
+            "wall_s": round(time.perf_counter() - t0, 2), "levels": per}
 
     # ---- config 2: full sweeps of the chi = 256 XXZ chain, FP64 and mixed ----
     if args.config2_sweeps > 0:
@@ -561,12 +591,23 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, ROOT)
         import oracle.core as oc  # CPU baseline only
+        nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        model = "unknown"
         try:
-            from threadpoolctl import threadpool_info
-            threads = max([p.get("num_threads", 1) for p in threadpool_info() if p.get("internal_api") == "openblas"]
-                          or [os.cpu_count()])
+            with open("/proc/cpuinfo") as f:
+                for ln in f:
+                    if ln.startswith("model name"):
+                        model = ln.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        try:
+            from threadpoolctl import threadpool_info, threadpool_limits
+            limiter = threadpool_limits(limits=nproc, user_api="blas")
+            threads = max([p.get("num_threads", 1) for p in threadpool_info() if p.get("user_api") == "blas"]
+                          or [nproc])
         except Exception:
-            threads = os.cpu_count()
+            limiter, threads = None, nproc
         t0 = time.perf_counter()
         stage1 = oc.iteration_stage_1(bl, op.cl, op.cr, d, L, op.idx, op.mats)
         t_stage1 = time.perf_counter() - t0
@@ -576,16 +617,55 @@ def main():
             t0 = time.perf_counter()
             yc = oc.matvec(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, R, x, stage1)
             times.append(time.perf_counter() - t0)
+        del stage1
         cpu_ms = float(np.median(times)) * 1e3
-        log("cpu oracle matvec %.1f ms (%d threads)" % (cpu_ms, threads))
+        log("cpu oracle matvec %.1f ms (%d threads, %s)" % (cpu_ms, threads, model))
         err = float(np.linalg.norm((y64 - yc).ravel()) / np.linalg.norm(yc.ravel()))
-        result["cpu_baseline"] = {
+        # one environment update of each side and one site move at the same shape
+        t0 = time.perf_counter()
+        oc.contract_sos_left(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, x)
+        t_envl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oc.contract_sos_right(bl, br, op.cl, op.cr, d, R, op.idx, op.mats, x)
+        t_envr = time.perf_counter() - t0
+        nb = np.asfortranarray(rng.standard_normal((br, br, d)) + 1j * rng.standard_normal((br, br, d)))
+        t0 = time.perf_counter()
+        oc.norm_denorm_going_right(bl, br, br, d, d, x, nb)
+        t_move = time.perf_counter() - t0
+        log("cpu oracle: stage-1 %.2f s, left env %.2f s, right env %.2f s, site move %.2f s"
+            % (t_stage1, t_envl, t_envr, t_move))
+        cb = {
             "value": round(1000.0 / cpu_ms, 4), "unit": "matvecs/s", "cores": int(threads), "kind": "port",
+            "nproc": int(nproc), "cpu_model": model,
             "sample": "%d oracle strategy-3 matvecs (dense stage-1 tensor built once in %.1f s, then 2 zgemms each, "
-                      "OpenBLAS) at the same chi=%d bulk site; median %.1f ms" % (len(times), t_stage1, chi, cpu_ms),
+                      "OpenBLAS on %d threads) at the same chi=%d bulk site, median %.1f ms; one contract_sos_left "
+                      "(%.2f s), one contract_sos_right (%.2f s) and one zgesdd site move 1024 x 2048 (%.2f s)"
+                      % (len(times), t_stage1, threads, chi, cpu_ms, t_envl, t_envr, t_move),
+            "matvec_ms": round(cpu_ms, 1), "stage1_s": round(t_stage1, 3), "env_left_s": round(t_envl, 3),
+            "env_right_s": round(t_envr, 3), "site_move_s": round(t_move, 3),
         }
+        fs = result.get("full_sweep")
+        if fs:
+            # the CPU time of the same sweep from these per-operation costs and the
+            # GPU FP64 sweep's own operation counts (every site step priced as a
+            # bulk one: an upper bound; the reference's ncv = 3 needs more matvecs)
+            n_mv = fs["fp64"]["matvecs_fp64"]
+            n_solve = fs["fp64"]["site_solves"]
+            n_move = 2 * (args.sites - 1)
+            est = n_mv * cpu_ms * 1e-3 + n_solve * t_stage1 + n_move * (0.5 * (t_envl + t_envr) + t_move)
+            cb["sweep_s_extrapolated"] = round(est, 1)
+            cb["sweep_extrapolation"] = ("%d matvecs x %.3f s + %d site solves x stage-1 %.2f s + %d moves x "
+                                         "(environment %.2f s + site move %.2f s), the operation counts of the GPU "
+                                         "FP64 sweep, every site priced as a bulk site"
+                                         % (n_mv, cpu_ms * 1e-3, n_solve, t_stage1, n_move,
+                                            0.5 * (t_envl + t_envr), t_move))
+            result["gpu_vs_cpu_sweep_speedup_extrapolated"] = round(est / fs["fp64"]["sweep_s"], 1)
+            log("cpu sweep extrapolated %.0f s" % est)
+        result["cpu_baseline"] = cb
         result["gpu_vs_cpu_matvec_speedup"] = round(cpu_ms / ms64, 2)
         result["max_rel_err_vs_oracle"] = err
+        if limiter is not None:
+            limiter.restore_original_limits()
     # ---- autotuner choices for the measured shapes -------------------------
     table = core.autotune_table()
     keys = ["chi_l %d chi_r %d d %d" % (c, c, dd) for c, dd in ((chi, d), (256, 2), (2048, 3))]
