@@ -242,6 +242,9 @@ def main():
                     help="skip sweeping the 64 chains to convergence (energies against the closed form)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy calibration kernel")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--gemm-choice", default="",
+                    help="force one GEMM launch choice 'bk,nsplit,m3' on every plan (nc_gemm_force_choice), for "
+                         "profiling one kernel variant; e.g. 6,1,1 puts the complex64 GEMMs on the bf16 split kernel")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -255,6 +258,13 @@ def main():
 
     lib = core.lib()
     bind(lib)
+    if args.gemm_choice:
+        bk, nsplit, m3 = (int(v) for v in args.gemm_choice.split(","))
+        lib.nc_gemm_force_choice.restype = C.c_int
+        lib.nc_gemm_force_choice.argtypes = [C.c_int] * 3
+        if lib.nc_gemm_force_choice(bk, nsplit, m3) != 0:
+            raise SystemExit("bad --gemm-choice %s" % args.gemm_choice)
+        log("GEMM launch choice forced to", args.gemm_choice)
     dev = core.device_info()
     log("device", dev)
     chi = args.chi

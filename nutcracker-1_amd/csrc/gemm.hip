@@ -430,9 +430,18 @@ __device__ __forceinline__ void b6_load(const SegT<float>& sg, int k0, int r0, i
     }
 }
 
+// LDS chunk of (row r, K chunk g): rows are 64 bytes (four 16-byte chunks), so
+// a 16-byte access of consecutive rows meets every bank once only if the chunk
+// is permuted by row.  The permutation f(r) takes four distinct values on the
+// even and on the odd rows of every 8 consecutive rows (conflict-free when 8
+// lanes are served per cycle, 32 banks) and on each residue mod 4 of every 16
+// rows (16 lanes per cycle, 64 banks); the earlier f(r) = (r >> 2) & 3 met the
+// second condition only, and PMC counted half the LDS cycles as bank conflicts.
+__device__ __forceinline__ int b6_chunk(int r, int g) { return g ^ ((r >> 2) & 3) ^ (((r >> 1) & 1) << 1); }
+
 // splits re, im and re + im of the 8 values into the 9 planes of row r, chunk g
 __device__ __forceinline__ void b6_store(bf16x8 (*S)[64][4], int r, int g, const float2 (&v)[8]) {
-    const int c = g ^ ((r >> 2) & 3);
+    const int c = b6_chunk(r, g);
     float x[8];
     bf16x8 h, m, l;
 #pragma unroll
@@ -517,14 +526,14 @@ __global__ __launch_bounds__(256) void cgemm_b6_kernel(const BatchT<float>* __re
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int r = wm + i * 16 + lrow;
-                const int c = lk ^ ((r >> 2) & 3);
+                const int c = b6_chunk(r, lk);
 #pragma unroll
                 for (int p = 0; p < 3; ++p) a[i][p] = As[3 * q + p][r][c];
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int r = wn + j * 16 + lrow;
-                const int c = lk ^ ((r >> 2) & 3);
+                const int c = b6_chunk(r, lk);
 #pragma unroll
                 for (int p = 0; p < 3; ++p) b[j][p] = Bs[3 * q + p][r][c];
             }
