@@ -264,6 +264,13 @@ int nc_contract_sos_left_noalloc(uint32_t bl, uint32_t br, uint32_t cl, uint32_t
    previous value, or -1 for a value outside that range. */
 int nc_set_optimizer_ncv(int ncv);
 
+/* How mixed-precision single-site solves apply H_eff in complex64: 0 on the
+   complex64 GEMM (operands split inside it), 1 on operands split once into
+   bf16 planes when bl * br >= 128 * 128 (the default), 2 always on planes.
+   Returns the previous mode, or -1 for a mode outside 0..2.  No counterpart in
+   the reference (its matvec is complex128 only). */
+int nc_set_mixed_planes(int mode);
+
 /* Counters of the site optimizer since load (5 values): site solves, first
    Krylov runs that ended with -14, ncv x 3/2 retries run (core.f95:2588-2602),
    dense strategy-1 fallbacks (core.f95:2569-2587), solves returned with -14. */
@@ -305,6 +312,22 @@ int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32
                       const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L, const nc_zcomplex* R,
                       const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
                       double* phase_ms);
+
+/* The same matvec in complex64 on the bf16 matrix cores with its operands
+   split once into bf16 planes (the mixed form's single-site matvec): L and R
+   are split before timing; each timed matvec splits x, runs step A, mixes into
+   Q's planes, runs step C and sums the K slices into complex128 y.  phase_ms,
+   when not NULL, receives 5 averages: split of x, step A, mix, step C, sum.
+   Benchmark entry point, no counterpart in the reference. */
+int nc_bench_heff_planes(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                         const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L, const nc_zcomplex* R,
+                         const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup, uint32_t reps, double* total_ms,
+                         double* phase_ms);
+
+/* K slices the plane matvec's two GEMM launches use (out[0] step A, out[1]
+   step C) for a site of bonds bl, br, physical dimension d and nfrom / nto
+   distinct MPO block indices.  Benchmark entry point. */
+int nc_planes_splits(uint32_t bl, uint32_t br, uint32_t d, uint32_t nfrom, uint32_t nto, int32_t* out);
 
 /* Calibration kernel for HBM byte counters: copies `bytes` (a multiple of 16)
    between two device buffers `reps` times; *avg_ms is the mean launch time. */

@@ -27,6 +27,7 @@ SOURCES = [
     "runtime.cpp",
     "kernels.hip",
     "gemm.hip",
+    "planes.hip",
     "contract.cpp",
     "eigen_host.cpp",
     "iram.cpp",

@@ -40,6 +40,13 @@ int nc_set_optimizer_ncv(int ncv) {
     return prev;
 }
 
+int nc_set_mixed_planes(int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    const int prev = mixed_planes_mode();
+    mixed_planes_mode() = mode;
+    return prev;
+}
+
 int nc_optimizer_stats(int64_t* out) {
     const OptimizerStats st = optimizer_stats();
     out[0] = st.solves;

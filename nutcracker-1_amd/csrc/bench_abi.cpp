@@ -184,6 +184,87 @@ extern "C" int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t
                               total_ms, phase_ms);
 }
 
+// The complex64 matvec on bf16 planes (the mixed form's single-site matvec):
+// L and R are split once outside the timed region; every timed matvec splits
+// x, runs step A, the mixing into Q's planes, step C and the FP64 sum of the
+// K slices.  phase_ms (5 entries, optional): split of x, step A, mix, step C,
+// slice sum.
+extern "C" int nc_bench_heff_planes(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
+                                    const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
+                                    const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
+                                    uint32_t reps, double* total_ms, double* phase_ms) {
+    try {
+        auto& c = Context::get();
+        hipStream_t s = c.stream;
+        MpoSite op;
+        op.build((int)d, (int)cl, (int)cr, (int)nmat, idx, reinterpret_cast<const zd*>(mats));
+        const size_t nL = (size_t)bl * bl * cl, nR = (size_t)br * br * cr, nx = (size_t)br * bl * d;
+        ZBuf dL(nL), dR(nR), dx(nx), dy(nx);
+        h2d(dL.ptr, L, nL * sizeof(double2));
+        h2d(dR.ptr, R, nR * sizeof(double2));
+        h2d(dx.ptr, x, nx * sizeof(double2));
+        PlaneHeff ph((int)bl, (int)br, op, dL.ptr, dR.ptr, s);
+        for (uint32_t i = 0; i < warmup; ++i) ph.apply(dx.ptr, dy.ptr, s);
+        NC_HIP(hipStreamSynchronize(s));
+        hipEvent_t ev[6];
+        for (auto& e : ev) NC_HIP(hipEventCreate(&e));
+        NC_HIP(hipEventRecord(ev[0], s));
+        for (uint32_t i = 0; i < reps; ++i) ph.apply(dx.ptr, dy.ptr, s);
+        NC_HIP(hipEventRecord(ev[1], s));
+        NC_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        NC_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        *total_ms = ms;
+        if (phase_ms) {
+            for (int k = 0; k < 5; ++k) phase_ms[k] = 0;
+            const uint32_t preps = reps < 3 ? (reps ? reps : 1) : 3;
+            const int64_t rs = (int64_t)br * bl;
+            for (uint32_t i = 0; i < preps; ++i) {
+                NC_HIP(hipEventRecord(ev[0], s));
+                split_planes(dx.ptr, 1, br, rs, (int)d, ph.Xp, 0, s);
+                NC_HIP(hipEventRecord(ev[1], s));
+                ph.gemm_a.launch(s);
+                NC_HIP(hipEventRecord(ev[2], s));
+                mix_split_planes(op, ph.P.ptr, ph.pslice, ph.nsa, (int)br, (int)bl, ph.Qp, s);
+                NC_HIP(hipEventRecord(ev[3], s));
+                ph.gemm_c.launch(s);
+                NC_HIP(hipEventRecord(ev[4], s));
+                reduce_slices(ph.Y.ptr, ph.yslice, ph.nsc, ph.yslice, dy.ptr, s);
+                NC_HIP(hipEventRecord(ev[5], s));
+                NC_HIP(hipEventSynchronize(ev[5]));
+                for (int k = 0; k < 5; ++k) {
+                    float t;
+                    NC_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                    phase_ms[k] += t / preps;
+                }
+            }
+        }
+        for (auto& e : ev) NC_HIP(hipEventDestroy(e));
+        NC_HIP(hipStreamSynchronize(s));
+        d2h(y, dy.ptr, nx * sizeof(double2));
+        return 0;
+    } catch (std::exception const& e) {
+        set_last_error(std::string("nc_bench_heff_planes: ") + e.what());
+        fprintf(stderr, "[nutcracker-hip] nc_bench_heff_planes: %s\n", e.what());
+        return -1;
+    }
+}
+
+// K slices of the two plane GEMMs of a matvec: out[0] step A, out[1] step C
+extern "C" int nc_planes_splits(uint32_t bl, uint32_t br, uint32_t d, uint32_t nfrom, uint32_t nto, int32_t* out) {
+    const int64_t tiles_a = (int64_t)((br + kPlaneTile - 1) / kPlaneTile) * ((bl + kPlaneTile - 1) / kPlaneTile) *
+                            nfrom * d;
+    const int64_t tiles_c = (int64_t)((br + kPlaneTile - 1) / kPlaneTile) * ((bl * d + kPlaneTile - 1) / kPlaneTile);
+    try {
+        out[0] = plane_split_count(tiles_a, (int)((bl + 31) / 32));
+        out[1] = plane_split_count(tiles_c, (int)(nto * ((br + 31) / 32)));
+    } catch (std::exception const& e) {
+        set_last_error(std::string("nc_planes_splits: ") + e.what());
+        return -1;
+    }
+    return 0;
+}
+
 // Site-move statistics: polar splits, Newton-Schulz iterations, SVD fallbacks.
 extern "C" int nc_polar_stats(int64_t* out) {
     nc::PolarStats p = nc::polar_stats();

@@ -16,6 +16,7 @@
 // Workgroups are dispatched round-robin over the 8 XCDs; the linear workgroup
 // id is remapped so that each XCD works on a contiguous block of output tiles
 // (neighbouring tiles share rows of A or columns of B in that XCD's L2).
+#include "nc_bf16.hpp"
 #include "nc_internal.hpp"
 
 #include <algorithm>
@@ -353,49 +354,14 @@ __global__ __launch_bounds__(256, (cgemm_min_waves<T, M3, BK>())) void cgemm_ker
 // Layout: 64 x 64 output tile per 256-thread workgroup, K staged 32 deep.  The
 // staging thread (row r = lane, K group g = wave) loads 8 consecutive K values
 // of row r, splits them and writes one 16-byte chunk per (matrix, part) into
-// LDS planes [9][64 rows][4 chunks of 8 bf16], chunk g stored at g ^ ((r >> 2)
-// & 3) so that the 16-byte reads of 16 consecutive rows hit distinct banks.
+// LDS planes [9][64 rows][4 chunks of 8 bf16], chunk g stored at a position
+// permuted by row (b6_chunk) so that the 16-byte reads of consecutive rows hit
+// distinct banks.
 // MFMA operands (measured lane layout): lane l holds row l % 16 of the block
 // and K values 8 (l / 16) .. 8 (l / 16) + 7; the accumulator holds rows
 // 4 (l / 16) + v of column l % 16, as the FP32 16x16x4 one.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int B6K = 32;
 constexpr int kBf16x6 = 6;  // launch choice code of this kernel (GemmChoice::bk)
-
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-
-// two FP32 values rounded to nearest bf16, packed (a in the low half)
-__device__ __forceinline__ unsigned bf16_pair(float a, float b) {
-    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2{a, b}), bf16x2));
-}
-// the packed bf16 values as FP32 (a bf16 is the upper half of an FP32)
-__device__ __forceinline__ float bf16_lo(unsigned u) { return __builtin_bit_cast(float, u << 16); }
-__device__ __forceinline__ float bf16_hi(unsigned u) { return __builtin_bit_cast(float, u & 0xffff0000u); }
-
-// v = h + m + l + O(2^-24 |v|), each part round-to-nearest bf16 of what the
-// previous ones leave (the residuals are exact in FP32)
-__device__ __forceinline__ void split3(const float (&v)[8], bf16x8& h, bf16x8& m, bf16x8& l) {
-    u32x4 H, Mid, L;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        float a = v[2 * e], b = v[2 * e + 1];
-        const unsigned ph = bf16_pair(a, b);
-        a -= bf16_lo(ph);
-        b -= bf16_hi(ph);
-        const unsigned pm = bf16_pair(a, b);
-        a -= bf16_lo(pm);
-        b -= bf16_hi(pm);
-        H[e] = ph;
-        Mid[e] = pm;
-        L[e] = bf16_pair(a, b);
-    }
-    h = __builtin_bit_cast(bf16x8, H);
-    m = __builtin_bit_cast(bf16x8, Mid);
-    l = __builtin_bit_cast(bf16x8, L);
-}
 
 // the 8 complex values of row r (M index for A, N index for B) at K positions
 // k0 + 8 g .. + 7 of the current segment; zero outside the matrix

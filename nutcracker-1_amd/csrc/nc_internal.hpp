@@ -292,6 +292,78 @@ void to_c64(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void to_c128(const float2* src, double2* dst, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------------------
+// Operands split once into bf16 planes for the complex64 matvec on the bf16
+// matrix cores.  A plane set holds `count` complex matrices (rows r < R, inner
+// index k < K) as nine planes each (re, im, re + im; hi, mid, lo parts); a
+// plane is an array of 16-byte chunks, chunk (kb, r) = 8 bf16 of k = 8 kb ..
+// 8 kb + 7 in row r at index kb * Rp + r (Rp = R rounded up to kPlaneTile,
+// Kb = K rounded up to 32, over 8).  Chunks of k >= K are zero.
+// ----------------------------------------------------------------------------
+constexpr int kPlaneTile = 128;
+struct PlaneSet {
+    DevBuf<uint4> buf;
+    int R = 0, K = 0, Rp = 0, Kb = 0, count = 0;
+    int64_t ps = 0;  // plane stride in chunks
+    void shape(int R, int K, int count);
+    uint4* at(int i) const { return buf.ptr + (int64_t)i * 9 * ps; }
+};
+// planes of matrices z0 .. z0 + nz - 1 of dst: element (r, k) of matrix z0 + z
+// at src[z sz + r sr + k sk] (complex128, rounded to FP32 and split)
+void split_planes(const double2* src, int64_t sr, int64_t sk, int64_t sz, int nz, PlaneSet& dst, int z0,
+                  hipStream_t s);
+// Q's planes (one matrix per MPO 'to' index t, rows n = l + bl b, K index
+// k < br) from step A's output P_{f,a} (nslice K slices pslice apart, summed)
+void mix_split_planes(MpoSite const& op, const float2* P, int64_t pslice, int nslice, int br, int bl, PlaneSet& Q,
+                      hipStream_t s);
+// out[e] = sum_c Y[c slice + e] in FP64, e < n
+void reduce_slices(const float2* Y, int64_t slice, int ns, int64_t n, double2* out, hipStream_t s);
+
+// The GEMM on planes: C_b (M x N, complex64, column-major ldc) = sum over the
+// batch's nseg segments of A_s B_s, with A's planes in row role (rows = M
+// index) and B's in row role too (rows = N index).  Segment s's planes start
+// s * a_seg (b_seg) chunks after A (B), and every segment has nks K stages of
+// 32.  K is split over nsplit workgroups per tile; slice c is written at
+// C + c * slice.
+struct PlaneBatch {
+    float2* C;
+    int64_t ldc, slice;
+    const uint4* A;
+    const uint4* B;
+    int64_t a_ps, b_ps;      // plane strides (chunks)
+    int64_t a_seg, b_seg;    // segment strides (chunks)
+    int32_t a_rows, b_rows;  // Rp of the two plane sets
+    int32_t M, N;
+    int32_t nseg, nks;
+};
+struct PlaneGemm {
+    DevBuf<PlaneBatch> batches;
+    int nsplit = 1, nbatch = 0;
+    int64_t items = 0;
+    void prepare(std::vector<PlaneBatch> const& bs, int nsplit);
+    void launch(hipStream_t s) const;
+};
+// K slices per tile that fill the CUs best for `tiles` tiles of nks K stages
+int plane_split_count(int64_t tiles, int nks);
+
+// H_eff x in complex64 on planes (no projectors): L and R split when built,
+// x, Q split per application; the result is summed into FP64.
+struct PlaneHeff {
+    int bl, br;
+    MpoSite const& op;
+    PlaneSet Lp, Rp, Xp, Qp;
+    DevBuf<float2> P, Y;
+    int nsa = 1, nsc = 1;
+    int64_t pslice = 0, yslice = 0;
+    PlaneGemm gemm_a, gemm_c;
+    PlaneHeff(int bl, int br, MpoSite const& op, const double2* L, const double2* R, hipStream_t s);
+    void apply(const double2* x, double2* y, hipStream_t s);
+};
+// Mixed-form single-site matvecs: 0 on the complex64 GEMM, 1 on planes when
+// bl * br >= kPlaneTile^2 (default), 2 always on planes.
+int& mixed_planes_mode();
+bool use_planes(int bl, int br);
+
+// ----------------------------------------------------------------------------
 // Contractions (FP64, grouped).  All pointers are device pointers.
 // ----------------------------------------------------------------------------
 struct Workspace {

@@ -203,18 +203,24 @@ struct HeffOperator : Operator {
     }
 };
 
-// The same operator with the contraction in complex64 on FP32 matrix cores;
-// the vectors handed to the eigen-solver (and the projectors) stay FP64.
+// The same operator with the contraction in complex64 (bf16 split products or
+// FP32 matrix cores); the vectors handed to the eigen-solver (and the
+// projectors) stay FP64.  With `planes` the matvec runs on operands split once
+// into bf16 planes (L and R per site solve, x per application).
 struct Heff32Operator : Operator {
     OptimizeArgs const& a;
     Workspace& ws;
+    PlaneHeff* planes;
     ZBuf full_in, full_out, tmp;
     DevBuf<float2> x32, y32;
-    Heff32Operator(OptimizeArgs const& a_, Workspace& ws_) : a(a_), ws(ws_) {
+    Heff32Operator(OptimizeArgs const& a_, Workspace& ws_, PlaneHeff* planes_ = nullptr)
+        : a(a_), ws(ws_), planes(planes_) {
         const int64_t nfull = (int64_t)a.d * a.bl * a.br;
         n = a.proj->active() ? a.proj->orth : nfull;
-        x32.resize(nfull);
-        y32.resize(nfull);
+        if (!planes) {
+            x32.resize(nfull);
+            y32.resize(nfull);
+        }
         if (a.proj->active()) {
             full_in.resize(nfull);
             full_out.resize(nfull);
@@ -227,6 +233,12 @@ struct Heff32Operator : Operator {
         if (a.proj->active()) {
             a.proj->unproject(x, full_in.ptr, s);
             xin = full_in.ptr;
+        }
+        if (planes) {
+            double2* yo = a.proj->active() ? full_out.ptr : y;
+            planes->apply(xin, yo, s);
+            if (a.proj->active()) a.proj->project(full_out.ptr, y, tmp.ptr, s);
+            return;
         }
         to_c64(xin, x32.ptr, nfull, s);
         heff_matvec(a.bl, a.br, *a.op, a.L32, a.R32, x32.ptr, y32.ptr, ws, s);
@@ -329,8 +341,10 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
         const bool mixed = a.mixed && a.L32 && a.R32;
         Heff32Operator* op32 = nullptr;
         std::unique_ptr<Heff32Operator> op32_owner;
+        std::unique_ptr<PlaneHeff> planes;
         if (mixed) {
-            op32_owner.reset(new Heff32Operator(a, ws));
+            if (use_planes(a.bl, a.br)) planes.reset(new PlaneHeff(a.bl, a.br, *a.op, a.L, a.R, s));
+            op32_owner.reset(new Heff32Operator(a, ws, planes.get()));
             op32 = op32_owner.get();
         }
         Operator& kop = mixed ? static_cast<Operator&>(*op32) : static_cast<Operator&>(op);

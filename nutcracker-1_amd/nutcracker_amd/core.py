@@ -102,6 +102,19 @@ def autotune_table() -> List[str]:
     return [ln for ln in buf.value.decode().splitlines() if ln]
 
 
+def set_mixed_planes(mode: int) -> int:
+    """How mixed-precision single-site solves apply H_eff: 0 on the complex64
+    GEMM, 1 on operands split once into bf16 planes when bl * br >= 128 * 128
+    (the default), 2 always on planes.  Returns the previous mode."""
+    f = lib().nc_set_mixed_planes
+    f.restype = C.c_int
+    f.argtypes = [C.c_int]
+    prev = f(int(mode))
+    if prev < 0:
+        raise ValueError("mode must be 0, 1 or 2")
+    return prev
+
+
 def set_optimizer_ncv(ncv: int) -> int:
     """Krylov basis size of optimize_ (the reference's 3 by default); returns the
     previous value."""
