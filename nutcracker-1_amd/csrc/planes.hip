@@ -112,71 +112,100 @@ __global__ __launch_bounds__(256) void split_planes_kernel(const double2* __rest
     store_planes(dst + z * dz + (int64_t)kb * Rp + r, ps, re, im);
 }
 
+// three bf16 parts of the two values (x0, x1): one packed pair per part
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned& h, unsigned& m, unsigned& l) {
+    h = bf16_pair(x0, x1);
+    x0 -= bf16_lo(h);
+    x1 -= bf16_hi(h);
+    m = bf16_pair(x0, x1);
+    x0 -= bf16_lo(m);
+    x1 -= bf16_hi(m);
+    l = bf16_pair(x0, x1);
+}
+
 // Step C's B operand planes from step A's output.  For destination t (an MPO
 // 'to' index), column n = l + bl b and inner index k < br,
 //   Q_t(k, n) = sum_{blocks j: to = t} sum_a W_j(a, b) sum_c P_c[(src_j d + a) rs + k + br l],
-// with the nslice K-split slices of P summed on the fly.  A wave covers 16
-// values of l x 4 chunks of k; NB outputs b per pass over the blocks.
-template <int NB>
+// with the nslice K-split slices of P summed on the fly.  A wave takes one
+// chunk column kb and 16 consecutive l; lane (l, i) holds k = 8 kb + 2 i and
+// k + 1, so each source block is one 16-byte load per lane (64 contiguous
+// bytes per l) and each plane one 4-byte store per lane (256 contiguous bytes
+// per store).  EVEN (br even): the pair is 16-byte aligned and inside the
+// matrix whenever its first value is.  D > 0 fixes the physical dimension;
+// D = 0 takes any d, four outputs b per pass over the blocks.
+template <int D, bool EVEN>
 __global__ __launch_bounds__(256) void mix_split_kernel(const float2* __restrict__ P, int64_t pslice, int nslice,
                                                         const int* __restrict__ dst_ptr,
                                                         const int* __restrict__ blk_src,
-                                                        const float2* __restrict__ W, int d, int br, int bl,
-                                                        int64_t rs, u32x4* __restrict__ Q, int64_t ps, int64_t qz,
+                                                        const float2* __restrict__ W, int d_rt, int br, int bl,
+                                                        int64_t rs, unsigned* __restrict__ Q, int64_t ps, int64_t qz,
                                                         int Np, int Kb) {
+    constexpr int NB = D > 0 ? D : 4;
+    const int d = D > 0 ? D : d_rt;
     const int t = blockIdx.y;
+    const int nlg = (bl + 15) >> 4;
     const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const int nlg = (bl + 15) >> 4;
+    const int kb = (int)(w / nlg);
     const int l = (int)(w % nlg) * 16 + (lane >> 2);
-    const int kb = (int)(w / nlg) * 4 + (lane & 3);
-    if (l >= bl || kb >= Kb) return;
-    const int k0 = 8 * kb;
-    const int nk = br - k0;  // valid values in this chunk (<= 0: a zero pad chunk)
-    const int64_t e0 = (int64_t)br * l + k0;
+    const int pi = lane & 3;
+    if (kb >= Kb || l >= bl) return;
+    const int k = 8 * kb + 2 * pi;
+    const bool in0 = k < br, in1 = k + 1 < br;
+    const int64_t e = (int64_t)br * l + k;
+    const int jb = dst_ptr[t], je = dst_ptr[t + 1];
     for (int b0 = 0; b0 < d; b0 += NB) {
-        float qr[NB][8], qi[NB][8];
+        float r0[NB], i0[NB], r1[NB], i1[NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) { qr[b][e] = 0.f; qi[b][e] = 0.f; }
-        for (int j = dst_ptr[t]; j < dst_ptr[t + 1]; ++j) {
-            const int src = blk_src[j];
+        for (int b = 0; b < NB; ++b) r0[b] = i0[b] = r1[b] = i1[b] = 0.f;
+        for (int j = jb; j < je; ++j) {
+            const float2* Pj = P + (int64_t)blk_src[j] * d * rs + e;
             const float2* Wj = W + (int64_t)j * d * d;
+#pragma unroll(D > 0 ? D : 1)
             for (int a = 0; a < d; ++a) {
-                const float2* pp = P + ((int64_t)src * d + a) * rs + e0;
-                float pr[8], pi[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    pr[e] = 0.f;
-                    pi[e] = 0.f;
-                    if (e < nk) {
-                        float2 v = pp[e];
+                const float2* pp = Pj + (int64_t)a * rs;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (EVEN) {
+                    if (in0) {
+                        v = *reinterpret_cast<const float4*>(pp);
                         for (int c = 1; c < nslice; ++c) {
-                            const float2 u = pp[c * pslice + e];
-                            v.x += u.x;
-                            v.y += u.y;
+                            const float4 u = *reinterpret_cast<const float4*>(pp + c * pslice);
+                            v.x += u.x; v.y += u.y; v.z += u.z; v.w += u.w;
                         }
-                        pr[e] = v.x;
-                        pi[e] = v.y;
+                    }
+                } else {
+                    for (int c = 0; c < nslice; ++c) {
+                        if (in0) { const float2 u = pp[c * pslice]; v.x += u.x; v.y += u.y; }
+                        if (in1) { const float2 u = pp[c * pslice + 1]; v.z += u.x; v.w += u.y; }
                     }
                 }
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     if (b0 + b < d) {
                         const float2 wv = Wj[a + (b0 + b) * d];
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            qr[b][e] += wv.x * pr[e] - wv.y * pi[e];
-                            qi[b][e] += wv.x * pi[e] + wv.y * pr[e];
-                        }
+                        r0[b] += wv.x * v.x - wv.y * v.y;
+                        i0[b] += wv.x * v.y + wv.y * v.x;
+                        r1[b] += wv.x * v.z - wv.y * v.w;
+                        i1[b] += wv.x * v.w + wv.y * v.z;
                     }
                 }
             }
         }
+        // nine planes, dword pi of chunk (kb, n); dword stride of a plane 4 ps
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (b0 + b < d) store_planes(Q + t * qz + (int64_t)kb * Np + l + (int64_t)bl * (b0 + b), ps, qr[b], qi[b]);
+        for (int b = 0; b < NB; ++b) {
+            if (b0 + b < d) {
+                unsigned* q = Q + 4 * (t * qz + (int64_t)kb * Np + l + (int64_t)bl * (b0 + b)) + pi;
+                const int64_t s4 = 4 * ps;
+                unsigned h, m, lo;
+                split_pair(r0[b], r1[b], h, m, lo);
+                q[0] = h; q[s4] = m; q[2 * s4] = lo;
+                split_pair(i0[b], i1[b], h, m, lo);
+                q[3 * s4] = h; q[4 * s4] = m; q[5 * s4] = lo;
+                split_pair(r0[b] + i0[b], r1[b] + i1[b], h, m, lo);
+                q[6 * s4] = h; q[7 * s4] = m; q[8 * s4] = lo;
+            }
+        }
     }
 }
 
@@ -206,11 +235,17 @@ struct PStage {
 // LDS reads of other buffers; it has no register result, and its completion is
 // awaited only by the vmcnt of step_barrier.  One wait state between the M0
 // write and its use.
+// (M0 is clobbered: clang warns that it is a reserved register, which matters
+// only to code that keeps a value in M0 across the statement; nothing in this
+// file's GEMM does.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void dma16(const u32x4* g, unsigned voff, const u32x4* l) {
     const unsigned m0 = (unsigned)(uintptr_t)(LdsPtr)(l);
     asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(g), "s"(m0)
                  : "m0");
 }
+#pragma clang diagnostic pop
 
 // One pipeline step's copies: component Q of the K stage whose chunk column 0
 // is at pa (A, offset to the tile's first row) and pb (B).  The 24 runs of 64
@@ -402,22 +437,29 @@ void mix_split_planes(MpoSite const& op, const float2* P, int64_t pslice, int ns
                       hipStream_t s) {
     const int ndst = (int)op.tos.size();
     if (Q.count != ndst || Q.R != bl * op.d || Q.K != br) throw Error("mix_split_planes: plane shape", -2);
+    if (nslice < 1) throw Error("mix_split_planes: no slices", -2);
     const int64_t rs = (int64_t)br * bl;
-    const int64_t waves = (int64_t)((bl + 15) / 16) * ((Q.Kb + 3) / 4);
-    dim3 grid((unsigned)((waves * 64 + 255) / 256), (unsigned)ndst);
-    u32x4* q = reinterpret_cast<u32x4*>(Q.buf.ptr);
+    const int64_t waves = (int64_t)((bl + 15) / 16) * Q.Kb;
+    dim3 grid((unsigned)((waves + 3) / 4), (unsigned)ndst);
+    unsigned* q = reinterpret_cast<unsigned*>(Q.buf.ptr);
     const int* ptr = op.mix_ptr[0].ptr;
     const int* src = op.mix_src[0].ptr;
     const float2* W = op.mix_W32[0].ptr;
-    if (op.d == 1)
-        hipLaunchKernelGGL(mix_split_kernel<1>, grid, dim3(256), 0, s, P, pslice, nslice, ptr, src, W, op.d, br, bl,
-                           rs, q, Q.ps, 9 * Q.ps, Q.Rp, Q.Kb);
-    else if (op.d == 3)
-        hipLaunchKernelGGL(mix_split_kernel<3>, grid, dim3(256), 0, s, P, pslice, nslice, ptr, src, W, op.d, br, bl,
-                           rs, q, Q.ps, 9 * Q.ps, Q.Rp, Q.Kb);
-    else
-        hipLaunchKernelGGL(mix_split_kernel<2>, grid, dim3(256), 0, s, P, pslice, nslice, ptr, src, W, op.d, br, bl,
-                           rs, q, Q.ps, 9 * Q.ps, Q.Rp, Q.Kb);
+    const int64_t qz = 9 * Q.ps;
+    // 16-byte pair loads need an even br (then every pair offset is even) and
+    // an even slice stride
+    const bool even = br % 2 == 0 && pslice % 2 == 0;
+#define NC_MIX_SPLIT(DD, EV)                                                                                      \
+    hipLaunchKernelGGL((mix_split_kernel<DD, EV>), grid, dim3(256), 0, s, P, pslice, nslice, ptr, src, W, op.d, br, \
+                       bl, rs, q, Q.ps, qz, Q.Rp, Q.Kb)
+    switch (op.d) {
+        case 1: if (even) NC_MIX_SPLIT(1, true); else NC_MIX_SPLIT(1, false); break;
+        case 2: if (even) NC_MIX_SPLIT(2, true); else NC_MIX_SPLIT(2, false); break;
+        case 3: if (even) NC_MIX_SPLIT(3, true); else NC_MIX_SPLIT(3, false); break;
+        case 4: if (even) NC_MIX_SPLIT(4, true); else NC_MIX_SPLIT(4, false); break;
+        default: if (even) NC_MIX_SPLIT(0, true); else NC_MIX_SPLIT(0, false); break;
+    }
+#undef NC_MIX_SPLIT
     NC_HIP(hipGetLastError());
 }
 
