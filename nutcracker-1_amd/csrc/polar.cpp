@@ -40,6 +40,13 @@ __global__ void scale_cols_kernel(double2* X, const double* sv, int64_t m, int64
 namespace {
 PolarStats g_polar_stats;
 
+// Lower bound of the normalized singular values that the scaled iteration's
+// schedule starts from: a guess (truncated MPS spectra and random sites lie
+// above it), checked against the error at every step; the guaranteed bound
+// 1e-16 (the perturbation below keeps every singular value above it) is
+// tracked beside it and takes over when the guess is contradicted.
+constexpr double kNsGuess = 1e-6;
+
 // alpha_cap solves (a/2)(3 - a^2) = 0.1 on [1, sqrt(3)]
 double ns_alpha_cap() {
     static const double cap = [] {
@@ -228,7 +235,8 @@ __global__ void bherm_kernel(const BItem* __restrict__ it) {
 
 // ---- device-driven batched iteration: per-job schedule state on the device
 struct NsState {
-    double l;     // lower bound of the singular values
+    double l;     // lower bound the schedule follows (a guess until contradicted)
+    double ls;    // guaranteed lower bound (from 1e-16 through the same polynomials)
     double a, b;  // this step's H = a E + b I
     int phase;    // 0 running, 1 one more step, 2 converged, 3 failed (non-finite)
     int iters;
@@ -309,12 +317,23 @@ __global__ __launch_bounds__(64) void ns_decide_kernel(NsState* __restrict__ st,
         s.live = 0;
         s.just = 1;
     } else {
-        double l = s.l;
-        if (err < 1.0) l = fmax(l, sqrt(1.0 - err));
+        double l = s.l, ls = s.ls;
+        if (err < 1.0) {
+            l = fmax(l, sqrt(1.0 - err));
+            ls = fmax(ls, sqrt(1.0 - err));
+        }
+        // singular values all in [l, 1] give |E|_F <= sqrt(m) (1 - l^2); a larger
+        // error contradicts the guess, and the schedule continues from the
+        // guaranteed bound
+        if (l > ls && err > sqrt((double)s.m) * (1.0 - l * l) * (1.0 + 1e-9)) l = ls;
         double alpha = 1.0;
         if (l < 1.0 - 1e-3) alpha = fmin(cap, sqrt(3.0 / (1.0 + l + l * l)));
         const double p1 = 0.5 * alpha * (3.0 - alpha * alpha);
         s.l = fmin(fmin(1.0, 0.5 * alpha * l * (3.0 - alpha * alpha * l * l)), p1);
+        // the polynomial increases up to 1 / alpha, so a bound below it maps to
+        // a bound (and the largest singular value, 1, to p1)
+        s.ls = fmin(fmin(1.0, 0.5 * alpha * ls * (3.0 - alpha * alpha * ls * ls)), p1);
+        s.ls = fmin(s.ls, s.l);
         // H = (alpha/2)(3 - alpha^2) I - (alpha^3/2) E
         s.a = -0.5 * alpha * alpha * alpha;
         s.b = 0.5 * alpha * (3.0 - alpha * alpha);
@@ -424,9 +443,13 @@ void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, 
 // after a bond grows, the new directions are low-index basis states, not
 // random ones.  A hashed perturbation a hundred times smaller keeps the floor
 // where the unit vectors have no component in the null space.  The schedule
-// starts from l = 1e-16 and converges in about 45 steps for any input; the
-// two-site state changes by ~1e-14 relative (P = A U_p^H below, kept
-// Hermitian).
+// follows the guessed bound kNsGuess and, beside it, the guaranteed bound from
+// l = 1e-16; every step checks the guess against the error (all singular
+// values in [l, 1] give ||X X^H - I||_F <= sqrt(m) (1 - l^2)) and falls back to
+// the guaranteed bound when it is contradicted.  A well-conditioned site (a
+// random one, as when a chain is reset) converges in about 17 steps, a site
+// with singular values near the floor in about 45; the two-site state changes
+// by ~1e-14 relative (P = A U_p^H below, kept Hermitian).
 // The Gram matrix of each step gives ||X X^H - I||_F for free; being
 // Hermitian, only its upper tiles are computed (a step costs 1.5 GEMMs of
 // m x m x n instead of 2).  Only GEMMs of
@@ -502,7 +525,7 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
             const size_t q = act[k];
             const int m = jobs[q].m;
             items[k] = NsItem{st[q].G.ptr, {st[q].X0.ptr, st[q].X1.ptr}, jobs[q].Up, ns[q]};
-            hs[k] = NsState{1e-16, 0, 0, 0, 0, 1, 0, m, 0};
+            hs[k] = NsState{kNsGuess, 1e-16, 0, 0, 0, 0, 1, 0, m, 0};
             maxmm = std::max<int64_t>(maxmm, (int64_t)m * m);
             maxn = std::max<int64_t>(maxn, ns[q]);
         }
