@@ -101,7 +101,7 @@ def measured_traffic():
 
 
 def bind(lib):
-    for name in ("nc_bench_heff", "nc_bench_heff_c64"):
+    for name in ("nc_bench_heff", "nc_bench_heff_c64", "nc_bench_heff_planes"):
         f = getattr(lib, name)
         f.restype = C.c_int
         f.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)] + [C.c_void_p] * 5 + \
@@ -110,13 +110,13 @@ def bind(lib):
     lib.nc_bench_copy.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]
 
 
-def time_matvec(core, lib, name, op, L, R, x, warmup, steps):
+def time_matvec(core, lib, name, op, L, R, x, warmup, steps, nphases=3):
     bl, br, d = L.shape[0], R.shape[0], op.d
     y = np.zeros_like(x)
     idx = np.ascontiguousarray(np.asarray(op.idx, np.uint32).reshape(-1, order="F"))
     mats = np.ascontiguousarray(np.asarray(op.mats).reshape(-1, order="F"))
     total = C.c_double(0)
-    phases = (C.c_double * 3)()
+    phases = (C.c_double * nphases)()
     rc = getattr(lib, name)(bl, br, op.cl, op.cr, d, op.nmat, idx.ctypes.data_as(C.POINTER(C.c_uint32)),
                             mats.ctypes.data, L.ctypes.data, R.ctypes.data, x.ctypes.data, y.ctypes.data, warmup,
                             steps, C.byref(total), phases)
@@ -281,8 +281,12 @@ def main():
     ms64, ph64, y64 = time_matvec(core, lib, "nc_bench_heff", op, L, R, x, args.warmup, args.steps)
     log("fp64 matvec %.4f ms" % ms64, "phases", ph64)
     ms32, ph32, y32 = time_matvec(core, lib, "nc_bench_heff_c64", op, L, R, x, args.warmup, args.steps)
-    log("complex64 matvec %.4f ms" % ms32, "phases", ph32)
+    log("complex64 matvec (complex64 GEMM) %.4f ms" % ms32, "phases", ph32)
     err32 = float(np.linalg.norm((y32 - y64).ravel()) / np.linalg.norm(y64.ravel()))
+    # the mixed form's single-site matvec: operands split once into bf16 planes
+    msp, php, yp = time_matvec(core, lib, "nc_bench_heff_planes", op, L, R, x, args.warmup, args.steps, 5)
+    log("complex64 matvec (bf16 planes) %.4f ms" % msp, "phases", php)
+    errp = float(np.linalg.norm((yp - y64).ravel()) / np.linalg.norm(y64.ravel()))
 
     copy_ms = C.c_double(0)
     copy_bytes = args.copy_mib << 20
@@ -338,20 +342,29 @@ def main():
         "config": {"workload": "XXZ chain J=1 Delta=%g N=%d d=2 MPO bond 5 at chi=%d: bulk-site H_eff matvec "
                                "(value) and one whole sweep in each form" % (args.delta, args.sites, chi),
                    "chi": chi, "d": d, "mpo_bond": op.cl, "mpo_blocks": op.nmat,
-                   "form": "value = plain FP64 complex drop-in; 'mixed' = complex64 matvec on FP32 MFMA"},
+                   "form": "value = plain FP64 complex drop-in; 'mixed' = complex64 matvec with its operands "
+                           "split once into bf16 planes (bf16 MFMA, FP32 accuracy)"},
         "matvec_tflops": round(f_mv / (ms64 * 1e-3) / 1e12, 3),
         "phase_ms": {"gemm_step_a": round(ph64[0], 4), "mix": round(ph64[1], 4), "gemm_step_c": round(ph64[2], 4)},
         "roofline": {**roofline_of(form64, ph64, FP64_MFMA_PEAK_TFLOPS, "cgemm_kernel (complex128, FP64 MFMA)"),
                      "traffic": traffic},
         "mixed": {
-            "matvecs_per_s": round(1000.0 / ms32, 3), "ms_per_matvec": round(ms32, 4),
-            "matvec_speedup_vs_fp64": round(ms64 / ms32, 3),
-            "matvec_rel_err_vs_fp64": err32,
-            "phase_ms": {"gemm_step_a": round(ph32[0], 4), "mix": round(ph32[1], 4),
-                         "gemm_step_c": round(ph32[2], 4)},
-            "roofline": roofline_of(form32, ph32, FP32_MFMA_PEAK_TFLOPS,
-                                    "cgemm_b6_kernel (complex64 split into bf16 parts, bf16 MFMA)"
-                                    if form32 == "bf16x6" else "cgemm_kernel (complex64, FP32 MFMA)"),
+            "matvecs_per_s": round(1000.0 / msp, 3), "ms_per_matvec": round(msp, 4),
+            "matvec_speedup_vs_fp64": round(ms64 / msp, 3),
+            "matvec_rel_err_vs_fp64": errp,
+            "phase_ms": {"split_x": round(php[0], 4), "gemm_step_a": round(php[1], 4),
+                         "mix_split": round(php[2], 4), "gemm_step_c": round(php[3], 4),
+                         "slice_sum": round(php[4], 4)},
+            "roofline": roofline_of("bf16x6", [php[1], 0.0, php[3]], FP32_MFMA_PEAK_TFLOPS,
+                                    "cgemm_planes_kernel (complex64 operands split once into bf16 planes, "
+                                    "bf16 MFMA)"),
+            "complex64_gemm_path": {
+                "ms_per_matvec": round(ms32, 4), "matvec_rel_err_vs_fp64": err32,
+                "phase_ms": {"gemm_step_a": round(ph32[0], 4), "mix": round(ph32[1], 4),
+                             "gemm_step_c": round(ph32[2], 4)},
+                "roofline": roofline_of(form32, ph32, FP32_MFMA_PEAK_TFLOPS,
+                                        "cgemm_b6_kernel (complex64 split into bf16 parts, bf16 MFMA)"
+                                        if form32 == "bf16x6" else "cgemm_kernel (complex64, FP32 MFMA)")},
         },
         "copy_calibration": {"bytes": copy_bytes, "avg_ms": round(copy_ms.value, 4),
                              "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},
@@ -375,9 +388,11 @@ def main():
             fl = flops_matvec(cchi, cchi, cop.d, nf, nt, cop.nmat)
             m64, _, _ = time_matvec(core, lib, "nc_bench_heff", cop, cL, cR, cx, 2, reps)
             m32, _, _ = time_matvec(core, lib, "nc_bench_heff_c64", cop, cL, cR, cx, 2, reps)
+            mp, _, _ = time_matvec(core, lib, "nc_bench_heff_planes", cop, cL, cR, cx, 2, reps, 5)
             others[name] = {"chi": cchi, "d": cop.d, "flops_per_matvec": fl,
                             "fp64_ms": round(m64, 4), "fp64_tflops": round(fl / (m64 * 1e-3) / 1e12, 2),
-                            "c64_ms": round(m32, 4), "c64_tflops": round(fl / (m32 * 1e-3) / 1e12, 2)}
+                            "c64_ms": round(m32, 4), "c64_tflops": round(fl / (m32 * 1e-3) / 1e12, 2),
+                            "c64_planes_ms": round(mp, 4), "c64_planes_tflops": round(fl / (mp * 1e-3) / 1e12, 2)}
             log(name, others[name])
             del cL, cR, cx
         result["other_configs_matvec"] = others

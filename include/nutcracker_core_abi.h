@@ -266,7 +266,7 @@ int nc_set_optimizer_ncv(int ncv);
 
 /* How mixed-precision single-site solves apply H_eff in complex64: 0 on the
    complex64 GEMM (operands split inside it), 1 on operands split once into
-   bf16 planes when bl * br >= 128 * 128 (the default), 2 always on planes.
+   bf16 planes when bl * br >= 512 * 512 (the default), 2 always on planes.
    Returns the previous mode, or -1 for a mode outside 0..2.  No counterpart in
    the reference (its matvec is complex128 only). */
 int nc_set_mixed_planes(int mode);

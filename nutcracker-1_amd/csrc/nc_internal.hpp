@@ -359,7 +359,7 @@ struct PlaneHeff {
     void apply(const double2* x, double2* y, hipStream_t s);
 };
 // Mixed-form single-site matvecs: 0 on the complex64 GEMM, 1 on planes when
-// bl * br >= kPlaneTile^2 (default), 2 always on planes.
+// bl * br >= 512 * 512 (default), 2 always on planes.
 int& mixed_planes_mode();
 bool use_planes(int bl, int br);
 

@@ -104,7 +104,7 @@ def autotune_table() -> List[str]:
 
 def set_mixed_planes(mode: int) -> int:
     """How mixed-precision single-site solves apply H_eff: 0 on the complex64
-    GEMM, 1 on operands split once into bf16 planes when bl * br >= 128 * 128
+    GEMM, 1 on operands split once into bf16 planes when bl * br >= 512 * 512
     (the default), 2 always on planes.  Returns the previous mode."""
     f = lib().nc_set_mixed_planes
     f.restype = C.c_int
