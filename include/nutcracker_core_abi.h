@@ -432,7 +432,10 @@ int64_t nc_autotune_table(char* buf, int64_t len);
    (no counterpart in the reference): bk 16 or 32 (K step per LDS stage) or -16
    (16 with two LDS stages), or 6 (complex64 untransposed operands on the bf16
    matrix cores, each FP32 value split into three bf16 parts; other plans take
-   bk 16), nsplit 0 (heuristic) or 1/2/4/8, m3 -1 (the precision's default), 0
+   bk 16), 8 or 9 (complex128 untransposed operands whose M and N are multiples
+   of 64 and K of 8, copied global -> LDS through a ring of 3 or 4 eight-deep
+   stages; other plans take bk 16), nsplit 0 (heuristic) or 1/2/4/8, m3 -1 (the
+   precision's default), 0
    (4 real products) or 1 (3); bk = 0 ends the forcing.
    Returns -1 for values outside these. */
 int nc_gemm_force_choice(int bk, int nsplit, int m3);

@@ -294,7 +294,7 @@ int nc_set_autotune(int on) {
 }
 
 int nc_gemm_force_choice(int bk, int nsplit, int m3) {
-    if (bk != 0 && bk != 16 && bk != 32 && bk != -16 && bk != 6) return -1;
+    if (bk != 0 && bk != 16 && bk != 32 && bk != -16 && bk != 6 && bk != 8 && bk != 9) return -1;
     if (nsplit < 0 || nsplit > 8 || (nsplit & (nsplit - 1))) return -1;
     nc::gemm_force_choice(bk, nsplit, m3 < 0 ? -1 : (m3 ? 1 : 0));
     return 0;

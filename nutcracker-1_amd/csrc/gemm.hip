@@ -18,6 +18,7 @@
 // (neighbouring tiles share rows of A or columns of B in that XCD's L2).
 #include "nc_bf16.hpp"
 #include "nc_internal.hpp"
+#include "nc_lds_dma.hpp"
 
 #include <algorithm>
 #include <cstddef>
@@ -339,6 +340,210 @@ __global__ __launch_bounds__(256, (cgemm_min_waves<T, M3, BK>())) void cgemm_ker
         }
 }
 
+// ---- complex128 GEMM, both operands untransposed, fed by global -> LDS copies
+//
+// For plans whose batches all have M and N multiples of 64 and whose segments
+// all have K a multiple of 8 (the bulk sites of a chain once its bonds are
+// multiples of 64).  Each K stage of 8 is copied straight into LDS by
+// global_load_lds_dwordx4 (one complex128 per lane and copy), NST - 1 stages
+// ahead of its use, into a ring of NST LDS stages.  No prefetch registers are
+// held, so the 3-multiply form fits three waves per SIMD (three workgroups per
+// CU at NST = 3) instead of two, and a stage's copies have NST - 1 stages of
+// matrix-core work to land in.  Output tile, wave layout, MFMAs and epilogue
+// are those of cgemm_kernel.
+//
+// LDS stage (16 KB): a[k][m] as A lies in memory (64 consecutive m of one k
+// are one 1 KB run, one copy instruction), and b[n][k'] with the k slot of
+// column n permuted, k' = k ^ ((n / 2) mod 8): B's columns are K-contiguous,
+// so one copy instruction takes 8 columns x 8 k (128 contiguous bytes per
+// column), and without the permutation the 16 lanes that read one MFMA B
+// fragment (16 consecutive n at one k, 128 bytes apart) would fall on two
+// 16-byte bank groups; with it they cover all sixteen.
+// Copies per stage: 8 runs of A (one per k) and 8 of B (8 columns each); wave
+// w issues A runs 2w, 2w + 1 and B runs 2w, 2w + 1, four per wave, so a wave
+// waiting for stage s leaves 4 (NST - 2) newer copies in flight.
+constexpr int kDmaF64 = 8;   // launch choice codes of this kernel (GemmChoice::bk): NST = 3
+constexpr int kDmaF64x4 = 9;  // NST = 4
+constexpr int DBK = 8;
+struct DmaStage {
+    double2 a[DBK][64];
+    double2 b[64 * DBK];
+};
+
+// the k slot of (column n, k) in a stage's b
+__device__ __forceinline__ int dma_bslot(int n, int k) { return n * DBK + (k ^ ((n >> 1) & 7)); }
+
+// one segment's per-lane copy offsets (bytes from the stage's A / B base)
+struct DmaOffsets {
+    unsigned a0, a1, b0, b1;
+};
+__device__ __forceinline__ DmaOffsets dma_offsets(int64_t lda, int64_t ldb, int wave, int lane) {
+    DmaOffsets o;
+    o.a0 = 16u * (unsigned)(lane + (2 * wave) * lda);
+    o.a1 = 16u * (unsigned)(lane + (2 * wave + 1) * lda);
+    const int n0 = 8 * (2 * wave) + (lane >> 3), n1 = n0 + 8, ks = lane & 7;
+    o.b0 = 16u * (unsigned)((ks ^ ((n0 >> 1) & 7)) + n0 * ldb);
+    o.b1 = 16u * (unsigned)((ks ^ ((n1 >> 1) & 7)) + n1 * ldb);
+    return o;
+}
+
+// this wave's copies of every stage but the NST - 2 newest have landed, its LDS
+// reads are done, then the workgroup barrier; the memory clobber keeps the
+// compiler from moving LDS reads across it
+template <int NST>
+__device__ __forceinline__ void dma_stage_barrier() {
+    static_assert(NST == 3 || NST == 4, "stage ring of 3 or 4");
+    if constexpr (NST == 3) asm volatile("s_waitcnt vmcnt(4)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(8)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <bool M3>
+__device__ __forceinline__ void dma_compute(const DmaStage& S, Cplx<double>::v4 (&acc)[M3 ? 3 : 2][2][2], int wm,
+                                            int wn, int lrow, int lk) {
+#pragma unroll
+    for (int q = 0; q < DBK / 4; ++q) {
+        const int k = 4 * q + lk;
+        double2 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = S.a[k][wm + 16 * i + lrow];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = S.b[dma_bslot(wn + 16 * j + lrow, k)];
+        if constexpr (M3) {
+            double as[2], bs[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) as[i] = a[i].x + a[i].y;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bs[j] = b[j].x + b[j].y;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[0][i][j] = Cplx<double>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                    acc[1][i][j] = Cplx<double>::mfma(a[i].y, b[j].y, acc[1][i][j]);
+                    acc[2][i][j] = Cplx<double>::mfma(as[i], bs[j], acc[2][i][j]);
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const double nai = -a[i].y;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[0][i][j] = Cplx<double>::mfma(a[i].x, b[j].x, acc[0][i][j]);
+                    acc[0][i][j] = Cplx<double>::mfma(nai, b[j].y, acc[0][i][j]);
+                    acc[1][i][j] = Cplx<double>::mfma(a[i].x, b[j].y, acc[1][i][j]);
+                    acc[1][i][j] = Cplx<double>::mfma(a[i].y, b[j].x, acc[1][i][j]);
+                }
+            }
+        }
+    }
+}
+
+template <bool ACC, bool M3, int NST>
+__global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kernel(
+    const BatchT<double>* __restrict__ batches, const SegT<double>* __restrict__ segs) {
+    typedef Cplx<double>::v4 V4;
+    __shared__ DmaStage S[NST];
+
+    const BatchT<double> bt = batches[blockIdx.y];
+    if (bt.live && *bt.live == 0) return;
+    const int M = bt.M, N = bt.N;
+    const int tiles_m = M / TBM, tiles_n = N / TBN;
+    const int ntiles = tiles_m * tiles_n;
+    const int w = blockIdx.x;
+    if (w >= ntiles) return;
+    int tile = w;
+    if (!bt.upper && (ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    if (bt.upper && tm > tn) return;
+
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave & 1) * 32;
+    const int wn = (wave >> 1) * 32;
+    const int lrow = lane & 15;
+    const int lk = lane >> 4;
+
+    constexpr int NACC = M3 ? 3 : 2;
+    V4 acc[NACC][2][2];
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = V4{0, 0, 0, 0};
+
+    int sidx = bt.seg_begin;
+    if (sidx < bt.seg_end) {
+        // issue cursor: the segment and K offset of the next stage to copy; past
+        // the last stage it stays there, so every step issues the same number of
+        // copies (the extra ones land in stages that are never read)
+        SegT<double> sg = segs[sidx];
+        int k0 = 0;
+        DmaOffsets off = dma_offsets(sg.lda, sg.ldb, wave, lane);
+        int nks = 0;
+        for (int i = bt.seg_begin; i < bt.seg_end; ++i) nks += segs[i].K / DBK;
+        auto issue = [&](DmaStage& st) {
+            const double2* pa = sg.A + m0 + (int64_t)k0 * sg.lda;
+            const double2* pb = sg.B + k0 + (int64_t)n0 * sg.ldb;
+            lds_dma16(pa, off.a0, &st.a[2 * wave][0]);
+            lds_dma16(pa, off.a1, &st.a[2 * wave + 1][0]);
+            lds_dma16(pb, off.b0, &st.b[64 * (2 * wave)]);
+            lds_dma16(pb, off.b1, &st.b[64 * (2 * wave + 1)]);
+            if (k0 + DBK < sg.K) {
+                k0 += DBK;
+            } else if (sidx + 1 < bt.seg_end) {
+                ++sidx;
+                sg = segs[sidx];
+                k0 = 0;
+                off = dma_offsets(sg.lda, sg.ldb, wave, lane);
+            }
+        };
+#pragma unroll
+        for (int s = 0; s < NST - 1; ++s) issue(S[s]);
+        int cur = 0;
+        for (int ks = 0; ks < nks; ++ks) {
+            dma_stage_barrier<NST>();
+            // the stage consumed in the previous step is free for the copies of
+            // the step NST - 1 ahead
+            issue(S[cur == 0 ? NST - 1 : cur - 1]);
+            dma_compute<M3>(S[cur], acc, wm, wn, lrow, lk);
+            cur = cur == NST - 1 ? 0 : cur + 1;
+        }
+        // no copy may still write LDS when the workgroup ends
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+
+    double2* __restrict__ C = bt.C;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + j * 16 + lrow;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + Cplx<double>::row(lk, v);
+                double2 o;
+                if (M3) {
+                    const double t1 = acc[0][i][j][v], t2 = acc[1][i][j][v], t3 = acc[2][i][j][v];
+                    o.x = t1 - t2;
+                    o.y = t3 - t1 - t2;
+                } else {
+                    o.x = acc[0][i][j][v];
+                    o.y = acc[1][i][j][v];
+                }
+                double2* p = C + (int64_t)row + (int64_t)col * bt.ldc;
+                if (ACC) {
+                    const double2 c = *p;
+                    o.x += c.x;
+                    o.y += c.y;
+                }
+                *p = o;
+            }
+        }
+}
+
 // ---- complex64 GEMM on the bf16 matrix cores (both operands untransposed)
 //
 // Every FP32 operand value v is split into three bf16 parts, v = hi + mid + lo
@@ -652,9 +857,30 @@ void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, cons
     else launch_bk<T, OPA, OPB, 16, 1>(acc, m3, grid, b, s, st);
 }
 
+template <int NST>
+void launch_dma(bool acc, bool m3, dim3 grid, const BatchT<double>* b, const SegT<double>* s, hipStream_t st) {
+#define NC_L(ACCV, M3V) hipLaunchKernelGGL((cgemm_dma_kernel<ACCV, M3V, NST>), grid, dim3(256), 0, st, b, s)
+    if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
+    else { if (m3) NC_L(false, true); else NC_L(false, false); }
+#undef NC_L
+}
+
 template <typename T>
 void launch_any(int opA, int opB, int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
                 hipStream_t st) {
+    if (bk == kDmaF64 || bk == kDmaF64x4) {
+        // build_plan keeps these codes only for eligible complex128 NN plans
+        if constexpr (sizeof(T) == sizeof(double)) {
+            if (opA == OpN && opB == OpN) {
+                const auto* bd = reinterpret_cast<const BatchT<double>*>(b);
+                const auto* sd = reinterpret_cast<const SegT<double>*>(s);
+                if (bk == kDmaF64) launch_dma<3>(acc, m3, grid, bd, sd, st);
+                else launch_dma<4>(acc, m3, grid, bd, sd, st);
+                return;
+            }
+        }
+        bk = 16;
+    }
     if (bk == kBf16x6) {
         if constexpr (sizeof(T) == sizeof(float)) {
             if (opA == OpN && opB == OpN) {
@@ -736,6 +962,31 @@ struct GemmChoice {
 // a choice forced on every plan (tests of each kernel variant); bk = 0: none
 GemmChoice g_forced{0, 0, -1};
 
+// whether a built plan can run on cgemm_dma_kernel: complex128, both operands
+// untransposed, every batch's M and N multiples of the 64 x 64 tile, every
+// segment's K a positive multiple of the 8-deep stage, and the per-lane copy
+// offsets (at most 7 columns of A and 63 columns of B past a stage's base)
+// within 32 bits
+template <typename T>
+bool dma_eligible(Built<T> const& bp) {
+    if (sizeof(T) != sizeof(double) || bp.opA != OpN || bp.opB != OpN) return false;
+    for (auto const& b : bp.b) {
+        if (b.M <= 0 || b.N <= 0 || b.M % TBM || b.N % TBN) return false;
+        for (int i = b.seg_begin; i < b.seg_end; ++i) {
+            auto const& sg = bp.sg[i];
+            if (sg.K <= 0 || sg.K % DBK || sg.lda < b.M || sg.ldb < sg.K) return false;
+            if (16 * (7 * sg.lda + 64) >= (int64_t(1) << 31) || 16 * (63 * sg.ldb + 8) >= (int64_t(1) << 31))
+                return false;
+        }
+    }
+    return true;
+}
+inline bool is_dma(int bk) { return bk == kDmaF64 || bk == kDmaF64x4; }
+template <typename T>
+void settle_kernel(Built<T>& bp) {
+    if (is_dma(bp.bk) && !dma_eligible(bp)) bp.bk = 16;
+}
+
 // Fills per-batch sizes, chooses split-K and expands it.  `work(bytes)` gives the
 // split-K workspace.
 template <typename T, typename WorkFn>
@@ -780,7 +1031,8 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
     }
     out.nsplit = nsplit;
     out.m3 = ch.m3 >= 0 ? ch.m3 != 0 : (sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32);
-    out.bk = (ch.bk == 32 || ch.bk == -16 || ch.bk == kBf16x6) ? ch.bk : 16;
+    out.bk = (ch.bk == 32 || ch.bk == -16 || ch.bk == kBf16x6 || ch.bk == kDmaF64 || ch.bk == kDmaF64x4) ? ch.bk
+                                                                                                       : 16;
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
@@ -791,6 +1043,7 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
                 return (int64_t)x.M * x.N > (int64_t)y.M * y.N;
             });
         }
+        settle_kernel(out);
         return;
     }
     // split-K: partial products into a workspace, then a reduction into C
@@ -818,6 +1071,7 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
         out.ldcs.push_back(batches[b].ldc);
         out.lives.push_back(batches[b].live);
     }
+    settle_kernel(out);
 }
 
 // device copies of a built plan's lists
@@ -921,6 +1175,15 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
                 if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
                 cands.push_back(GemmChoice{bk, ns, m3});
             }
+    // complex128 untransposed operands: the LDS-copy pipeline (dropped below
+    // where the plan's shapes do not allow it)
+    if (sizeof(T) == sizeof(double) && opA == OpN && opB == OpN)
+        for (int bk : {kDmaF64, kDmaF64x4})
+            for (int m3 : {0, 1})
+                for (int ns : {1, 2, 4, 8}) {
+                    if (ns > 1 && (!uniform || maxk / ns < 64)) continue;
+                    cands.push_back(GemmChoice{bk, ns, m3});
+                }
     // complex64 untransposed operands: the bf16 split kernel (always 3M)
     if (sizeof(T) == sizeof(float) && opA == OpN && opB == OpN && g_tuning.bf16x6)
         for (int ns : {1, 2, 4, 8}) {
@@ -933,6 +1196,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
         Built<T> bp;
         DevBuf<char> desc;
         BuiltDev<T> dv{};
+        bool skip = false;
     };
     std::vector<Cand> built(cands.size());
     size_t work_need = 0;
@@ -949,6 +1213,9 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
         Cand& cd = built[c];
         build_plan<T>(opA, opB, false, M, N, b2, segs, cd.bp, [&](size_t) { return (void*)work.ptr; }, cands[c]);
         if (cd.bp.empty()) return GemmChoice();
+        // a kernel the plan's shapes do not allow: not a candidate
+        cd.skip = is_dma(cands[c].bk) && cd.bp.bk != cands[c].bk;
+        if (cd.skip) continue;
         const void* parts[5];
         size_t sizes[5], offs[5], off = 0;
         parts_of(cd.bp, parts, sizes);
@@ -972,6 +1239,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
     const int reps = 3;
     for (int round = 0; round < 2; ++round)
         for (size_t c = 0; c < cands.size(); ++c) {
+            if (built[c].skip) continue;
             launch_built(built[c].bp, built[c].dv, s);
             NC_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < reps; ++r) launch_built(built[c].bp, built[c].dv, s);
@@ -984,12 +1252,16 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     size_t best = 0;
-    for (size_t c = 1; c < cands.size(); ++c)
+    int ncand = 0;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        if (built[c].skip) continue;
+        ++ncand;
         if (us[c] < us[best]) best = c;
+    }
     TuneEntry e;
     e.default_us = us[0];
     e.best_us = us[best];
-    e.candidates = (int)cands.size();
+    e.candidates = ncand;
     e.choice = us[best] < 0.97 * us[0] ? cands[best] : GemmChoice();
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune[id] = e;
@@ -1079,8 +1351,14 @@ std::string gemm_tune_table() {
         snprintf(line, sizeof line,
                  "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d%s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
                  k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9],
-                 e.choice.bk == -16 ? 16 : (e.choice.bk == kBf16x6 ? 32 : e.choice.bk),
-                 e.choice.bk == -16 ? " 2-stage" : (e.choice.bk == kBf16x6 ? " bf16x6" : ""), e.choice.nsplit,
+                 e.choice.bk == -16 ? 16 : (e.choice.bk == kBf16x6 ? 32 : (is_dma(e.choice.bk) ? DBK : e.choice.bk)),
+                 e.choice.bk == -16 ? " 2-stage"
+                                    : (e.choice.bk == kBf16x6 ? " bf16x6"
+                                                              : (e.choice.bk == kDmaF64 ? " lds-copy 3-stage"
+                                                                                        : (e.choice.bk == kDmaF64x4
+                                                                                               ? " lds-copy 4-stage"
+                                                                                               : ""))),
+                 e.choice.nsplit,
                  e.choice.m3 < 0 ? "3M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
                  e.candidates);
         out += line;

@@ -145,7 +145,9 @@ struct TuneKey {
 bool& gemm_autotune();
 std::string gemm_tune_table();
 // force one launch choice on every GEMM plan (bk 16, 32 or -16 = 16 with two LDS
-// stages; nsplit 0 = heuristic; m3 -1 = default, 0 = 4M, 1 = 3M); bk = 0 clears
+// stages, 6 = complex64 on the bf16 split kernel, 8 / 9 = complex128 LDS-copy
+// pipeline of 3 / 4 stages, where the plan allows them; nsplit 0 = heuristic;
+// m3 -1 = default, 0 = 4M, 1 = 3M); bk = 0 clears
 void gemm_force_choice(int bk, int nsplit, int m3);
 
 // Host-side builder: keeps descriptor lists and uploads them per launch into a

@@ -35,6 +35,7 @@
 // over several workgroups per tile (slices summed by the consumer).
 #include "nc_bf16.hpp"
 #include "nc_internal.hpp"
+#include "nc_lds_dma.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -57,7 +58,6 @@ namespace {
 constexpr int kPT = kPlaneTile;
 
 typedef const __attribute__((address_space(1))) void* GlobalPtr;
-typedef __attribute__((address_space(3))) void* LdsPtr;
 
 // nine planes of the 8 values re[], im[] at chunk d (plane stride ps)
 __device__ __forceinline__ void store_planes(u32x4* __restrict__ d, int64_t ps, const float (&re)[8],
@@ -229,23 +229,9 @@ struct PStage {
     u32x4 b[3][4][kPT];
 };
 
-// 16 bytes per lane from global memory into LDS: lane i's chunk at
-// g + voff_i lands at LDS address M0 + 16 i.  Written out (rather than through
-// the builtin) so that the compiler does not add waits for these copies before
-// LDS reads of other buffers; it has no register result, and its completion is
-// awaited only by the vmcnt of step_barrier.  One wait state between the M0
-// write and its use.
-// (M0 is clobbered: clang warns that it is a reserved register, which matters
-// only to code that keeps a value in M0 across the statement; nothing in this
-// file's GEMM does.)
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void dma16(const u32x4* g, unsigned voff, const u32x4* l) {
-    const unsigned m0 = (unsigned)(uintptr_t)(LdsPtr)(l);
-    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(g), "s"(m0)
-                 : "m0");
-}
-#pragma clang diagnostic pop
+// 16 bytes per lane from global memory into LDS (lane i's chunk at g + voff_i
+// lands at LDS address l + 16 i); awaited only by the vmcnt of step_barrier.
+__device__ __forceinline__ void dma16(const u32x4* g, unsigned voff, const u32x4* l) { lds_dma16(g, voff, l); }
 
 // One pipeline step's copies: component Q of the K stage whose chunk column 0
 // is at pa (A, offset to the tile's first row) and pb (B).  The 24 runs of 64

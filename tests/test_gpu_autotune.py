@@ -117,3 +117,46 @@ def test_every_launch_variant_against_oracle(lib, variant):
     assert rel(newL, refL) < 1e-12
     assert rel(newR, refR) < 1e-12
     assert f(24, 0, -1) == -1 and f(16, 3, 0) == -1 and f(7, 1, 1) == -1
+
+
+# (d, chi_left, chi_right): bonds that are multiples of 64, so that the
+# complex128 matvec's plans may run on the LDS-copy pipeline kernel
+LDS_COPY_SHAPES = [(2, 128, 192), (3, 64, 128), (2, 256, 256)]
+LDS_COPY_VARIANTS = [(8, 1, 1), (8, 1, 0), (9, 1, 1), (9, 1, 0), (8, 2, 1), (9, 4, 1), (8, 8, 0)]
+
+
+@pytest.mark.parametrize("variant", LDS_COPY_VARIANTS)
+@pytest.mark.parametrize("shape", LDS_COPY_SHAPES)
+def test_lds_copy_kernel_against_oracle(lib, shape, variant):
+    """The complex128 kernel fed by global -> LDS copies (3- or 4-stage ring,
+    3M or 4M, K split 1-8), forced on every plan: on shapes it takes (M, N
+    multiples of 64, K of 8) the matvec keeps the drop-in's 1e-12 against the
+    oracle's strategy-3 matvec and agrees with the register-staged kernel to
+    round-off; the environment contractors (transposed operands) fall back to
+    the register-staged kernel and stay exact."""
+    core, h = lib
+    f = h.nc_gemm_force_choice
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int]
+    d, bl, br = shape
+    cl = cr = 5
+    print("shape", shape, "variant", variant, flush=True)
+    rng = np.random.default_rng(bl * 7919 + br * 31 + d)
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    x = crand(rng, br, bl, d)
+    ref = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x, oc.iteration_stage_1(bl, cl, cr, d, L, idx, mats))
+    refL = oc.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, x)
+    assert f(16, 1, variant[2]) == 0
+    try:
+        y16 = _heff(h, "nc_bench_heff", bl, br, cl, cr, d, idx, mats, L, R, x)
+        assert f(*variant) == 0
+        y = _heff(h, "nc_bench_heff", bl, br, cl, cr, d, idx, mats, L, R, x)
+        newL = core.contract_sos_left(bl, br, cl, cr, d, L, idx, mats, x)
+    finally:
+        assert f(0, 0, -1) == 0
+    print("rel vs oracle %.3e, vs register-staged kernel %.3e" % (rel(y, ref), rel(y, y16)), flush=True)
+    assert rel(y, ref) < 1e-12
+    assert rel(y, y16) < 1e-13
+    assert rel(newL, refL) < 1e-12
