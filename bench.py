@@ -142,6 +142,19 @@ def product_form(table, prec, chi, d):
     return forms.pop() if forms else "3M"
 
 
+def fp64_kernel(table, chi, d):
+    """The complex128 GEMM kernel the step-A and step-C plans of a single chain
+    at this shape ran, from the autotuner's table."""
+    key = "fp64 op00 step %%s chi_l %d chi_r %d d %d" % (chi, chi, d)
+    names = set()
+    for step in "AC":
+        for ln in table:
+            if ln.startswith(key % step) and "chains 1:" in ln:
+                names.add("cgemm_dma_kernel (complex128, FP64 MFMA, operands copied global -> LDS)"
+                          if " lds-copy " in ln else "cgemm_kernel (complex128, FP64 MFMA)")
+    return " + ".join(sorted(names)) if names else "cgemm_kernel (complex128, FP64 MFMA)"
+
+
 def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
     name = "mixed" if mixed else "fp64"
     t0 = time.perf_counter()
@@ -346,7 +359,7 @@ def main():
                            "split once into bf16 planes (bf16 MFMA, FP32 accuracy)"},
         "matvec_tflops": round(f_mv / (ms64 * 1e-3) / 1e12, 3),
         "phase_ms": {"gemm_step_a": round(ph64[0], 4), "mix": round(ph64[1], 4), "gemm_step_c": round(ph64[2], 4)},
-        "roofline": {**roofline_of(form64, ph64, FP64_MFMA_PEAK_TFLOPS, "cgemm_kernel (complex128, FP64 MFMA)"),
+        "roofline": {**roofline_of(form64, ph64, FP64_MFMA_PEAK_TFLOPS, fp64_kernel(tune_lines, chi, d)),
                      "traffic": traffic},
         "mixed": {
             "matvecs_per_s": round(1000.0 / msp, 3), "ms_per_matvec": round(msp, 4),

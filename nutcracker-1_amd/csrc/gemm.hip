@@ -438,22 +438,49 @@ __device__ __forceinline__ void dma_compute(const DmaStage& S, Cplx<double>::v4 
     }
 }
 
+// Tile order over the 8 XCDs (workgroups are dispatched round-robin over them,
+// so linear workgroup id l runs on XCD l mod 8), chosen on the host for plans
+// whose batches share one output size and whose tile count per batch is a
+// multiple of 8:
+//   xmap 1 (several batches): each XCD takes one eighth of the (batch, tile)
+//     range, batch-major, so a batch's tiles run on one or two XCDs and its
+//     operand panels are read into their L2 only;
+//   xmap 2 (one batch): the tile grid is cut into xa x xb = 8 blocks, one per
+//     XCD, so that an XCD reads tiles_m / xa row panels of A and tiles_n / xb
+//     column panels of B instead of all of them;
+//   xmap 0: each XCD takes a contiguous row-major range of every batch's tiles.
+// Within an XCD's share the tiles run in row-major order.
 template <bool ACC, bool M3, int NST>
 __global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kernel(
-    const BatchT<double>* __restrict__ batches, const SegT<double>* __restrict__ segs) {
+    const BatchT<double>* __restrict__ batches, const SegT<double>* __restrict__ segs, int xmap, int xa, int xb) {
     typedef Cplx<double>::v4 V4;
     __shared__ DmaStage S[NST];
 
-    const BatchT<double> bt = batches[blockIdx.y];
+    int bidx = blockIdx.y, w = blockIdx.x;
+    if (xmap == 1) {
+        const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+        const int item = (lin & 7) * ((gridDim.y * gridDim.x) >> 3) + (lin >> 3);
+        bidx = item / gridDim.x;
+        w = item - bidx * gridDim.x;
+    }
+    const BatchT<double> bt = batches[bidx];
     if (bt.live && *bt.live == 0) return;
     const int M = bt.M, N = bt.N;
     const int tiles_m = M / TBM, tiles_n = N / TBN;
     const int ntiles = tiles_m * tiles_n;
-    const int w = blockIdx.x;
     if (w >= ntiles) return;
-    int tile = w;
-    if (!bt.upper && (ntiles & 7) == 0 && (int)gridDim.x == ntiles) tile = (w & 7) * (ntiles >> 3) + (w >> 3);
-    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    int tm, tn;
+    if (xmap == 2) {
+        const int bm = tiles_m / xa, bn = tiles_n / xb, x = w & 7, slot = w >> 3;
+        tm = (x / xb) * bm + slot / bn;
+        tn = (x % xb) * bn + slot % bn;
+    } else {
+        int tile = w;
+        if (xmap == 0 && !bt.upper && (ntiles & 7) == 0 && (int)gridDim.x == ntiles)
+            tile = (w & 7) * (ntiles >> 3) + (w >> 3);
+        tm = tile / tiles_n;
+        tn = tile % tiles_n;
+    }
     if (bt.upper && tm > tn) return;
 
     const int m0 = tm * TBM, n0 = tn * TBN;
@@ -858,24 +885,31 @@ void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, cons
 }
 
 template <int NST>
-void launch_dma(bool acc, bool m3, dim3 grid, const BatchT<double>* b, const SegT<double>* s, hipStream_t st) {
-#define NC_L(ACCV, M3V) hipLaunchKernelGGL((cgemm_dma_kernel<ACCV, M3V, NST>), grid, dim3(256), 0, st, b, s)
+void launch_dma(bool acc, bool m3, dim3 grid, const BatchT<double>* b, const SegT<double>* s, int xmap, int xa,
+                int xb, hipStream_t st) {
+#define NC_L(ACCV, M3V) \
+    hipLaunchKernelGGL((cgemm_dma_kernel<ACCV, M3V, NST>), grid, dim3(256), 0, st, b, s, xmap, xa, xb)
     if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
     else { if (m3) NC_L(false, true); else NC_L(false, false); }
 #undef NC_L
 }
 
+// tile order of an LDS-copy launch (see cgemm_dma_kernel)
+struct XcdMap {
+    int mode = 0, a = 1, b = 1;
+};
+
 template <typename T>
 void launch_any(int opA, int opB, int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
-                hipStream_t st) {
+                hipStream_t st, XcdMap xm = XcdMap()) {
     if (bk == kDmaF64 || bk == kDmaF64x4) {
         // build_plan keeps these codes only for eligible complex128 NN plans
         if constexpr (sizeof(T) == sizeof(double)) {
             if (opA == OpN && opB == OpN) {
                 const auto* bd = reinterpret_cast<const BatchT<double>*>(b);
                 const auto* sd = reinterpret_cast<const SegT<double>*>(s);
-                if (bk == kDmaF64) launch_dma<3>(acc, m3, grid, bd, sd, st);
-                else launch_dma<4>(acc, m3, grid, bd, sd, st);
+                if (bk == kDmaF64) launch_dma<3>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                else launch_dma<4>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
                 return;
             }
         }
@@ -948,14 +982,18 @@ struct Built {
     int64_t slice = 0, maxtiles = 0;
     int M = 0, N = 0;
     CT* W = nullptr;                // split-K workspace
+    XcdMap xmap;                    // tile order of an LDS-copy launch
     bool empty() const { return b.empty() || maxtiles <= 0; }
 };
 
-// Launch choices of one plan: K step staged per LDS round (16, 32, or -16 for
-// 16 with two LDS stages), K split (0 = the heuristic below), complex product
-// form (-1 = the precision's default).
+// Launch choices of one plan: the kernel and its K step per LDS round (16, 32,
+// -16 for 16 with two LDS stages, kBf16x6, kDmaF64 / kDmaF64x4; 0 = the
+// default: the LDS-copy pipeline where the plan allows it, else 16), K split
+// (0 = the heuristic below), complex product form (-1 = the precision's
+// default).
+constexpr int kDefaultKernel = 0;
 struct GemmChoice {
-    int bk = 16;
+    int bk = kDefaultKernel;
     int nsplit = 0;
     int m3 = -1;
 };
@@ -968,12 +1006,12 @@ GemmChoice g_forced{0, 0, -1};
 // offsets (at most 7 columns of A and 63 columns of B past a stage's base)
 // within 32 bits
 template <typename T>
-bool dma_eligible(Built<T> const& bp) {
-    if (sizeof(T) != sizeof(double) || bp.opA != OpN || bp.opB != OpN) return false;
-    for (auto const& b : bp.b) {
+bool dma_shapes_ok(int opA, int opB, std::vector<BatchT<T>> const& batches, std::vector<SegT<T>> const& segs) {
+    if (sizeof(T) != sizeof(double) || opA != OpN || opB != OpN) return false;
+    for (auto const& b : batches) {
         if (b.M <= 0 || b.N <= 0 || b.M % TBM || b.N % TBN) return false;
         for (int i = b.seg_begin; i < b.seg_end; ++i) {
-            auto const& sg = bp.sg[i];
+            auto const& sg = segs[i];
             if (sg.K <= 0 || sg.K % DBK || sg.lda < b.M || sg.ldb < sg.K) return false;
             if (16 * (7 * sg.lda + 64) >= (int64_t(1) << 31) || 16 * (63 * sg.ldb + 8) >= (int64_t(1) << 31))
                 return false;
@@ -982,9 +1020,31 @@ bool dma_eligible(Built<T> const& bp) {
     return true;
 }
 inline bool is_dma(int bk) { return bk == kDmaF64 || bk == kDmaF64x4; }
+// the final kernel of a built plan (an LDS-copy choice the split lists do not
+// allow falls back to the register-staged kernel) and its XCD tile order
 template <typename T>
 void settle_kernel(Built<T>& bp) {
-    if (is_dma(bp.bk) && !dma_eligible(bp)) bp.bk = 16;
+    if (is_dma(bp.bk) && !dma_shapes_ok(bp.opA, bp.opB, bp.b, bp.sg)) bp.bk = 16;
+    bp.xmap = XcdMap();
+    if (!is_dma(bp.bk) || !g_tuning.dma_xcd_blocks) return;
+    bool uniform = true;
+    for (auto const& b : bp.b) uniform = uniform && b.M == bp.b[0].M && b.N == bp.b[0].N && !b.upper;
+    const int tm = bp.b[0].M / TBM, tn = bp.b[0].N / TBN, nb = (int)bp.b.size();
+    if (!uniform || (tm * tn) % 8 || bp.maxtiles != (int64_t)tm * tn) return;
+    if (nb > 1) {
+        bp.xmap.mode = 1;
+        return;
+    }
+    // the 8 blocks with the fewest panels per XCD (tiles_m / a + tiles_n / b)
+    int best = 1 << 30;
+    for (int a : {1, 2, 4, 8}) {
+        const int b = 8 / a;
+        if (tm % a || tn % b) continue;
+        if (tm / a + tn / b < best) {
+            best = tm / a + tn / b;
+            bp.xmap = XcdMap{2, a, b};
+        }
+    }
 }
 
 // Fills per-batch sizes, chooses split-K and expands it.  `work(bytes)` gives the
@@ -1021,18 +1081,20 @@ void build_plan(int opA, int opB, bool accumulate, int M, int N, std::vector<Bat
         for (int i = b.seg_begin; i < b.seg_end; ++i) k += segs[i].K;
         maxk = std::max(maxk, k);
     }
+    out.bk = (ch.bk == 32 || ch.bk == -16 || ch.bk == kBf16x6 || is_dma(ch.bk)) ? ch.bk : 16;
+    if (ch.bk == kDefaultKernel) out.bk = dma_shapes_ok<T>(opA, opB, batches, segs) ? kDmaF64 : 16;
     // split K when the launch would leave most of the chip idle (only plans
-    // whose batches share one output size)
+    // whose batches share one output size): fewer tiles than the workgroups the
+    // chip runs at once (three per CU for the LDS-copy kernel)
+    const int64_t few = is_dma(out.bk) ? g_tuning.split_k_tiles_dma : g_tuning.split_k_tiles;
     int nsplit = 1;
     if (ch.nsplit > 0) {
         if (uniform) nsplit = ch.nsplit;
-    } else if (uniform && g_tuning.split_k && tiles < g_tuning.split_k_tiles) {
-        while (nsplit < 8 && tiles * nsplit * 2 <= g_tuning.split_k_tiles && maxk / (nsplit * 2) >= 256) nsplit *= 2;
+    } else if (uniform && g_tuning.split_k && tiles < few) {
+        while (nsplit < 8 && tiles * nsplit * 2 <= few && maxk / (nsplit * 2) >= 256) nsplit *= 2;
     }
     out.nsplit = nsplit;
     out.m3 = ch.m3 >= 0 ? ch.m3 != 0 : (sizeof(T) == sizeof(double) ? g_tuning.use_3m_f64 : g_tuning.use_3m_f32);
-    out.bk = (ch.bk == 32 || ch.bk == -16 || ch.bk == kBf16x6 || ch.bk == kDmaF64 || ch.bk == kDmaF64x4) ? ch.bk
-                                                                                                       : 16;
     if (nsplit == 1) {
         out.b = batches;
         out.sg = segs;
@@ -1104,7 +1166,7 @@ template <typename T>
 void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
     typedef typename Cplx<T>::type CT;
     dim3 grid((unsigned)bp.maxtiles, (unsigned)bp.b.size());
-    launch_any<T>(bp.opA, bp.opB, bp.bk, bp.nsplit == 1 ? bp.acc : false, bp.m3, grid, dv.b, dv.sg, s);
+    launch_any<T>(bp.opA, bp.opB, bp.bk, bp.nsplit == 1 ? bp.acc : false, bp.m3, grid, dv.b, dv.sg, s, bp.xmap);
     NC_HIP(hipGetLastError());
     if (bp.nsplit == 1) return;
     unsigned g = (unsigned)std::min<int64_t>(2048, (bp.slice + 255) / 256);
@@ -1124,6 +1186,7 @@ void launch_built(Built<T> const& bp, BuiltDev<T> const& dv, hipStream_t s) {
 // ---------------------------------------------------------------------------
 struct TuneEntry {
     GemmChoice choice;
+    int kernel = 16;  // the kernel the choice resolves to for this plan
     double default_us = 0, best_us = 0;
     int candidates = 0;
 };
@@ -1215,6 +1278,10 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
         if (cd.bp.empty()) return GemmChoice();
         // a kernel the plan's shapes do not allow: not a candidate
         cd.skip = is_dma(cands[c].bk) && cd.bp.bk != cands[c].bk;
+        // an explicit choice that builds the default's launch: timed as the default
+        if (c > 0 && cd.bp.bk == built[0].bp.bk && cd.bp.nsplit == built[0].bp.nsplit &&
+            cd.bp.m3 == built[0].bp.m3)
+            cd.skip = true;
         if (cd.skip) continue;
         const void* parts[5];
         size_t sizes[5], offs[5], off = 0;
@@ -1262,7 +1329,9 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
     e.default_us = us[0];
     e.best_us = us[best];
     e.candidates = ncand;
-    e.choice = us[best] < 0.97 * us[0] ? cands[best] : GemmChoice();
+    const size_t pick = us[best] < 0.97 * us[0] ? best : 0;
+    e.choice = pick ? cands[pick] : GemmChoice();
+    e.kernel = built[pick].bp.bk;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune[id] = e;
     return e.choice;
@@ -1341,6 +1410,17 @@ bool& gemm_autotune() {
 
 void gemm_force_choice(int bk, int nsplit, int m3) { g_forced = GemmChoice{bk, nsplit, m3}; }
 
+// the kernel of a launch choice as the tuning table prints it
+std::string kernel_name(int bk) {
+    switch (bk) {
+        case -16: return "bk 16 2-stage";
+        case kBf16x6: return "bk 32 bf16x6";
+        case kDmaF64: return "bk 8 lds-copy 3-stage";
+        case kDmaF64x4: return "bk 8 lds-copy 4-stage";
+        default: return "bk " + std::to_string(bk);
+    }
+}
+
 std::string gemm_tune_table() {
     std::lock_guard<std::mutex> g(g_tune_mu);
     std::string out;
@@ -1349,15 +1429,9 @@ std::string gemm_tune_table() {
         auto const& k = kv.first;
         auto const& e = kv.second;
         snprintf(line, sizeof line,
-                 "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: bk %d%s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
+                 "fp%d op%d%d step %c chi_l %d chi_r %d d %d w %d/%d chains %d: %s split %d %s (%.1f us, default %.1f us, %d candidates)\n",
                  k[0], k[1], k[2], (char)k[3], k[4], k[5], k[6], k[7], k[8], k[9],
-                 e.choice.bk == -16 ? 16 : (e.choice.bk == kBf16x6 ? 32 : (is_dma(e.choice.bk) ? DBK : e.choice.bk)),
-                 e.choice.bk == -16 ? " 2-stage"
-                                    : (e.choice.bk == kBf16x6 ? " bf16x6"
-                                                              : (e.choice.bk == kDmaF64 ? " lds-copy 3-stage"
-                                                                                        : (e.choice.bk == kDmaF64x4
-                                                                                               ? " lds-copy 4-stage"
-                                                                                               : ""))),
+                 kernel_name(e.kernel).c_str(),
                  e.choice.nsplit,
                  e.choice.m3 < 0 ? "3M default" : (e.choice.m3 ? "3M" : "4M"), e.best_us, e.default_us,
                  e.candidates);

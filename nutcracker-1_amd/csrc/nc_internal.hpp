@@ -129,6 +129,13 @@ struct GemmTuning {
     }();
     bool split_k = true;            // split K when a launch has few output tiles
     int64_t split_k_tiles = 1024;   // ... fewer than this many (tiles x batches)
+    int64_t split_k_tiles_dma = 768;  // ... for the LDS-copy kernel (3 workgroups per CU)
+    // XCD-blocked tile order of the LDS-copy kernel (NC_DMA_XCD_BLOCKS=0 keeps
+    // the per-batch row-major ranges)
+    bool dma_xcd_blocks = [] {
+        const char* e = std::getenv("NC_DMA_XCD_BLOCKS");
+        return !(e && e[0] == '0');
+    }();
 };
 GemmTuning& gemm_tuning();
 
