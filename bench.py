@@ -152,7 +152,11 @@ def fp64_kernel(table, chi, d):
             if ln.startswith(key % step) and "chains 1:" in ln:
                 names.add("cgemm_dma_kernel (complex128, FP64 MFMA, operands copied global -> LDS)"
                           if " lds-copy " in ln else "cgemm_kernel (complex128, FP64 MFMA)")
-    return " + ".join(sorted(names)) if names else "cgemm_kernel (complex128, FP64 MFMA)"
+    if not names:
+        # untuned: the default launch, the LDS-copy kernel where the bonds are multiples of 64
+        names.add("cgemm_dma_kernel (complex128, FP64 MFMA, operands copied global -> LDS)" if chi % 64 == 0
+                  else "cgemm_kernel (complex128, FP64 MFMA)")
+    return " + ".join(sorted(names))
 
 
 def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
