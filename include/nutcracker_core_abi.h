@@ -334,8 +334,13 @@ int nc_planes_splits(uint32_t bl, uint32_t br, uint32_t d, uint32_t nfrom, uint3
 int nc_bench_copy(uint64_t bytes, uint32_t reps, double* avg_ms);
 
 /* Site-move statistics since load: out[0] polar splits, out[1] Newton-Schulz
-   iterations, out[2] splits that fell back to the SVD (rank-deficient input). */
+   iterations, out[2] splits completed by the SVD (zero input, or no convergence
+   within the step limit). */
 int nc_polar_stats(int64_t* out);
+/* Newton-Schulz steps a polar split may take before rocSOLVER's SVD completes it
+   (no counterpart in the reference; default 64, 0 sends every split to the SVD).
+   Returns the previous limit, or -1 outside [0, 1000]. */
+int nc_set_polar_max_steps(int steps);
 
 typedef struct nc_chain nc_chain;
 

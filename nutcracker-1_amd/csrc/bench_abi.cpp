@@ -273,3 +273,10 @@ extern "C" int nc_polar_stats(int64_t* out) {
     out[2] = p.svd_fallbacks;
     return 0;
 }
+
+extern "C" int nc_set_polar_max_steps(int steps) {
+    if (steps < 0 || steps > 1000) return -1;
+    const int prev = nc::polar_max_steps();
+    nc::polar_max_steps() = steps;
+    return prev;
+}

@@ -651,6 +651,8 @@ struct PolarStats {
     int64_t calls = 0, iterations = 0, svd_fallbacks = 0;
 };
 PolarStats polar_stats();
+// Newton-Schulz steps before a polar split is completed by the SVD (default 64)
+int& polar_max_steps();
 
 // norm_denorm_going_left / right on device tensors (core.f95:2708-2878).
 int norm_denorm_left(int bl, int bm, int br, int dl, int dr, const double2* to_denorm,

@@ -65,6 +65,11 @@ double ns_alpha_cap() {
 
 PolarStats polar_stats() { return g_polar_stats; }
 
+int& polar_max_steps() {
+    static int n = 64;
+    return n;
+}
+
 int polar_split(int m, int64_t ncol, const double2* A, double2* Up, double2* Pconj, hipStream_t s) {
     // the batched iteration with one job (its schedule runs on the device)
     std::vector<PolarJob> jobs(1);
@@ -455,7 +460,8 @@ void scale_batch(std::vector<double2*> const& x, std::vector<int64_t> const& n, 
 // m x m x n instead of 2).  Only GEMMs of
 // this library are used.
 // Non-finite input is refused (info 1000, reported as a normalisation error);
-// a job that does not converge in 64 steps (or has zero norm) is completed by
+// a job that does not converge in polar_max_steps() steps (64 unless set; 0
+// sends every job there) or has zero norm is completed by
 // the SVD (rocsolver_zgesvd), which gives the null directions an orthonormal
 // basis as zgesdd does.  P = A U_p^H is Hermitian positive semi-definite; its
 // Hermitian part is kept (conjugated, as the neighbour absorbs it).
@@ -575,7 +581,8 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
         DevBuf<double> partial((size_t)A * gs);
         const double cap = ns_alpha_cap();
         constexpr int kCheck = 6;  // iterations between host reads of the phases
-        for (int it = 0; it < 64; ++it) {
+        const int max_steps = polar_max_steps();
+        for (int it = 0; it < max_steps; ++it) {
             const int p = it & 1;
             pg[p]->launch(s);
             hipLaunchKernelGGL(ns_eye_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
@@ -587,7 +594,7 @@ void polar_split_batch(std::vector<PolarJob>& jobs, hipStream_t s) {
             hipLaunchKernelGGL(ns_h_kernel, dim3(gm, (unsigned)A), dim3(256), 0, s, ditems.ptr, dst.ptr);
             NC_HIP(hipGetLastError());
             pu[p]->launch(s);
-            if (it % kCheck == kCheck - 1 || it == 63) {
+            if (it % kCheck == kCheck - 1 || it == max_steps - 1) {
                 d2h(hs.data(), dst.ptr, A * sizeof(NsState));
                 bool running = false;
                 for (auto const& h : hs) running |= h.live != 0;

@@ -236,3 +236,74 @@ def test_site_move_unitary_input(core, dims):
     before = np.einsum("mla,rmb->labr", tn, td)
     after = np.einsum("mla,rmb->labr", on, od)
     assert rel(after, before) < 1e-12
+
+
+@pytest.fixture
+def svd_only(core):
+    """Every polar split completed by rocSOLVER's SVD (Newton-Schulz step limit 0)."""
+    import ctypes as C
+    h = core.lib()
+    f = h.nc_set_polar_max_steps
+    f.restype = C.c_int
+    f.argtypes = [C.c_int]
+    st = h.nc_polar_stats
+    st.restype = C.c_int
+    st.argtypes = [C.POINTER(C.c_int64)]
+
+    def stats():
+        out = (C.c_int64 * 3)()
+        assert st(out) == 0
+        return list(out)
+
+    prev = f(0)
+    assert prev >= 1 and f(-1) == -1 and f(1001) == -1
+    try:
+        yield stats
+    finally:
+        f(prev)
+
+
+@pytest.mark.parametrize("dims", [(2, 3, 4, 2, 2), (16, 32, 40, 2, 2), (64, 128, 130, 2, 3)])
+def test_site_move_svd_completion(core, svd_only, dims):
+    """The rocSOLVER completion of the polar split (taken when the Newton-Schulz
+    iteration does not converge, here forced for every split): on full-rank
+    input the tensors equal the oracle's zgesdd ones (core.f95:1-55,
+    2708-2878), on both sides, and every split is counted as completed by the
+    SVD."""
+    bl, bm, br, dl, dr = dims
+    rng = np.random.default_rng(700 + sum(dims))
+    before_stats = svd_only()
+    tn = crand(rng, bm, bl, dl)
+    td = crand(rng, br, bm, dr)
+    info, on, od = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    oinfo, oon, ood = oc.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    assert info == 0 and oinfo == 0
+    assert rel(on, oon) < 1e-10 and rel(od, ood) < 1e-10
+    # the left move on mirrored shapes
+    tdl = crand(rng, bl, br, dl)
+    tnl = crand(rng, bm, bl, dr)
+    info, od2, on2 = core.norm_denorm_going_left(br, bl, bm, dl, dr, tdl, tnl)
+    oinfo, ood2, oon2 = oc.norm_denorm_going_left(br, bl, bm, dl, dr, tdl, tnl)
+    assert info == 0 and oinfo == 0
+    assert rel(on2, oon2) < 1e-10 and rel(od2, ood2) < 1e-10
+    after_stats = svd_only()
+    assert after_stats[2] - before_stats[2] == 2
+
+
+@pytest.mark.parametrize("rank", [1, 3])
+def test_site_move_svd_completion_rank_deficient(core, svd_only, rank):
+    """The SVD completion on a rank-deficient site: the null directions get an
+    orthonormal completion and the two-site state is kept."""
+    rng = np.random.default_rng(80 + rank)
+    bl, bm, br, dl, dr = 4, 6, 5, 2, 2
+    u = crand(rng, bm, rank)
+    v = crand(rng, rank, bl * dl)
+    tn = np.asfortranarray((u @ v).reshape(bm, bl, dl, order="F"))
+    td = crand(rng, br, bm, dr)
+    info, on, od = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    assert info == 0
+    A = on.reshape(bm, bl * dl, order="F")
+    assert np.abs(A @ A.conj().T - np.eye(bm)).max() < 1e-12
+    before = np.einsum("mla,rmb->labr", tn, td)
+    after = np.einsum("mla,rmb->labr", on, od)
+    assert rel(after, before) < 1e-12
