@@ -25,6 +25,10 @@ Pinning (what checks each routine; see tests/test_oracle_*.py):
 * optimize (strategies 1/2/3): optimizer KATs (tests/optimizer.cpp:22-138),
   projector KATs (tests/python/test.py:978-1126) and scipy.linalg.eigh on the
   dense H_eff (library routine).
+* compute_overlap_with_projectors, filter_components_outside_orthog: the closed
+  form ||V^T v|| of the overlap, tests/projectors.cpp:195-243 restated (the
+  overlap at every site equals |<state 1|state 2>|), and non-identity null
+  swaps (tests/test_oracle_projectors.py).
 * norm_denorm_going_left/right: polar-factor uniqueness (U Vh unitary rows,
   P U Vh == A with P Hermitian PSD) and state preservation.
 * end-to-end: TFIM energies of tests/chain.cpp and tests/stress.cpp, checked
