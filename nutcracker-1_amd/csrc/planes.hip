@@ -25,8 +25,8 @@
 // l % 16, k = 8 (l / 16) .. + 7) is then a single 16-byte chunk, 16 lanes read
 // 256 contiguous bytes, and a tile's stage is a set of contiguous 1 KB runs.
 //
-// GEMM: 128 x 128 output tile per 256-thread workgroup (4 waves in 2 x 2, each
-// wave 4 x 4 MFMA blocks of 16 x 16 for each of the three real products), K
+// GEMM: 128 x 128 output tile per 512-thread workgroup (8 waves in 2 x 4, each
+// wave 4 x 2 MFMA blocks of 16 x 16 for each of the three real products), K
 // in stages of 32, one component (3 parts of A and of B, 48 KB) per pipeline
 // step.  The three components cycle through three LDS buffers, so buffer q
 // always holds component q; each step's planes are copied global -> LDS by
@@ -236,30 +236,37 @@ __device__ __forceinline__ void dma16(const u32x4* g, unsigned voff, const u32x4
 // One pipeline step's copies: component Q of the K stage whose chunk column 0
 // is at pa (A, offset to the tile's first row) and pb (B).  The 24 runs of 64
 // chunks of each operand (t = 8 part + 2 chunk column + half) are shared out
-// six per wave; off.a[Q][u] is the byte offset of this lane's chunk of run
-// t = 6 wave + u from pa (likewise for B), fixed for the whole launch.
+// 24 / NW per wave; off.a[Q][u] is the byte offset of this lane's chunk of run
+// t = (24 / NW) wave + u from pa (likewise for B), fixed for the whole launch.
+template <int NW>
 struct IssueOffsets {
-    unsigned a[3][6], b[3][6];
+    static constexpr int kRuns = 24 / NW;
+    unsigned a[3][kRuns], b[3][kRuns];
 };
-template <int Q>
-__device__ __forceinline__ void planes_issue(PStage& S, const u32x4* pa, const u32x4* pb, IssueOffsets const& off,
+template <int Q, int NW>
+__device__ __forceinline__ void planes_issue(PStage& S, const u32x4* pa, const u32x4* pb, IssueOffsets<NW> const& off,
                                              int wave) {
+    constexpr int R = IssueOffsets<NW>::kRuns;
 #pragma unroll
-    for (int u = 0; u < 6; ++u) {
-        const int t = wave * 6 + u;
+    for (int u = 0; u < R; ++u) {
+        const int t = wave * R + u;
         const int p = t >> 3, kb = (t >> 1) & 3, h = (t & 1) * 64;
         dma16(pa, off.a[Q][u], &S.a[p][kb][h]);
         dma16(pb, off.b[Q][u], &S.b[p][kb][h]);
     }
 }
 
-// The six partial products of one component for the wave's 4 x 4 blocks,
-// smallest first (lo.hi, mid.mid, hi.lo, mid.hi, hi.mid, hi.hi).
-__device__ __forceinline__ void planes_compute(PStage const& S, f32x4 (&acc)[4][4], int wm, int wn, int lane) {
+// The six partial products of one component for the wave's 4 x JB blocks,
+// smallest first (lo.hi, mid.mid, hi.lo, mid.hi, hi.mid, hi.hi).  The
+// scheduling barrier keeps each step's MFMAs inside it, so the accumulators of
+// the three components keep their registers across the loop.
+template <int JB>
+__device__ __forceinline__ void planes_compute(PStage const& S, f32x4 (&acc)[4][JB], int wm, int wn, int lane) {
+    __builtin_amdgcn_sched_barrier(0);
     const int r = lane & 15, kb = lane >> 4;
-    bf16x8 b[4][3];
+    bf16x8 b[JB][3];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < JB; ++j)
 #pragma unroll
         for (int p = 0; p < 3; ++p) b[j][p] = __builtin_bit_cast(bf16x8, S.b[p][kb][wn + 16 * j + r]);
     constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
@@ -272,7 +279,7 @@ __device__ __forceinline__ void planes_compute(PStage const& S, f32x4 (&acc)[4][
 #pragma unroll
         for (int t = 0; t < 6; ++t)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < JB; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[pa[t]], b[j][pb[t]], acc[i][j], 0, 0, 0);
     }
 }
@@ -284,13 +291,20 @@ __device__ __forceinline__ void planes_compute(PStage const& S, f32x4 (&acc)[4][
 // accesses across the barrier.
 template <int N>
 __device__ __forceinline__ void step_barrier() {
-    static_assert(N == 0 || N == 12, "vmcnt of a pipeline step");
+    static_assert(N == 0 || N == 6 || N == 12, "vmcnt of a pipeline step");
     if (N == 0) asm volatile("s_waitcnt vmcnt(0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else if (N == 6) asm volatile("s_waitcnt vmcnt(6)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     else asm volatile("s_waitcnt vmcnt(12)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
-constexpr int kIssuesPerStep = 12;  // DMA instructions per wave per pipeline step
 
-__global__ __launch_bounds__(256) void cgemm_planes_kernel(const PlaneBatch* __restrict__ batches, int nsplit) {
+// NW waves per workgroup on the 128 x 128 tile: NW = 4, each wave 64 x 64 (one
+// wave per SIMD); NW = 8, each wave 64 x 32 (two waves per SIMD: while one
+// waits at a barrier or for its LDS reads the other keeps the matrix pipe
+// busy; 96 accumulator registers per wave instead of 192).
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void cgemm_planes_kernel(const PlaneBatch* __restrict__ batches, int nsplit) {
+    constexpr int JB = 16 / NW;                         // 16 x 16 blocks per wave along N
+    constexpr int kIssuesPerStep = 2 * IssueOffsets<NW>::kRuns;  // copies per wave per step
     __shared__ PStage S0, S1, S2;
     const PlaneBatch bt = batches[blockIdx.y];
     const int tiles_m = (bt.M + kPT - 1) / kPT, tiles_n = (bt.N + kPT - 1) / kPT;
@@ -303,25 +317,25 @@ __global__ __launch_bounds__(256) void cgemm_planes_kernel(const PlaneBatch* __r
     const int m0 = (tile / tiles_n) * kPT, n0 = (tile % tiles_n) * kPT;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-    IssueOffsets off;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * (16 * JB);
+    IssueOffsets<NW> off;
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            const int t = wave * 6 + u;
+        for (int u = 0; u < IssueOffsets<NW>::kRuns; ++u) {
+            const int t = wave * IssueOffsets<NW>::kRuns + u;
             const int p = t >> 3, kb = (t >> 1) & 3, h = (t & 1) * 64;
             off.a[q][u] = 16u * (unsigned)((3 * q + p) * bt.a_ps + (int64_t)kb * bt.a_rows + h + lane);
             off.b[q][u] = 16u * (unsigned)((3 * q + p) * bt.b_ps + (int64_t)kb * bt.b_rows + h + lane);
         }
 
-    f32x4 acc[3][4][4];
+    f32x4 acc[3][4][JB];
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[q][i][j] = f32x4{0, 0, 0, 0};
+            for (int j = 0; j < JB; ++j) acc[q][i][j] = f32x4{0, 0, 0, 0};
 
     const int nks = bt.nseg * bt.nks;
     const int ks0 = (int)((int64_t)nks * split / nsplit), ks1 = (int)((int64_t)nks * (split + 1) / nsplit);
@@ -343,24 +357,24 @@ __global__ __launch_bounds__(256) void cgemm_planes_kernel(const PlaneBatch* __r
         const u32x4* na = pa;
         const u32x4* nb = pb;
         if (ks0 + 1 < ks1) advance(na, nb);
-        planes_issue<0>(S0, pa, pb, off, wave);
-        planes_issue<1>(S1, pa, pb, off, wave);
+        planes_issue<0, NW>(S0, pa, pb, off, wave);
+        planes_issue<1, NW>(S1, pa, pb, off, wave);
         // One body for every K stage: the last one copies its own stage again
         // into the buffers of the following steps (never read), so the loop has
         // no tail and all copies are awaited before the epilogue.
         for (int ks = ks0; ks < ks1; ++ks) {
             // component re: buffer 0 (its copies were issued two steps ago)
             step_barrier<kIssuesPerStep>();
-            planes_issue<2>(S2, pa, pb, off, wave);
-            planes_compute(S0, acc[0], wm, wn, lane);
+            planes_issue<2, NW>(S2, pa, pb, off, wave);
+            planes_compute<JB>(S0, acc[0], wm, wn, lane);
             // component im: buffer 1
             step_barrier<kIssuesPerStep>();
-            planes_issue<0>(S0, na, nb, off, wave);
-            planes_compute(S1, acc[1], wm, wn, lane);
+            planes_issue<0, NW>(S0, na, nb, off, wave);
+            planes_compute<JB>(S1, acc[1], wm, wn, lane);
             // component re + im: buffer 2
             step_barrier<kIssuesPerStep>();
-            planes_issue<1>(S1, na, nb, off, wave);
-            planes_compute(S2, acc[2], wm, wn, lane);
+            planes_issue<1, NW>(S1, na, nb, off, wave);
+            planes_compute<JB>(S2, acc[2], wm, wn, lane);
             pa = na;
             pb = nb;
             if (ks + 2 < ks1) advance(na, nb);
@@ -373,7 +387,7 @@ __global__ __launch_bounds__(256) void cgemm_planes_kernel(const PlaneBatch* __r
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JB; ++j) {
             const int col = n0 + wn + 16 * j + lr;
             if (col >= bt.N) continue;
 #pragma unroll
@@ -491,9 +505,13 @@ void PlaneGemm::prepare(std::vector<PlaneBatch> const& bs, int nsplit_) {
     h2d(batches.ptr, bs.data(), bs.size() * sizeof(PlaneBatch));
 }
 
+// eight waves per workgroup: measured 0.80 against 0.84 ms per chi = 1024 plane
+// matvec with four (the two waves of a SIMD overlap each other's LDS waits)
+constexpr int kPlaneWaves = 8;
+
 void PlaneGemm::launch(hipStream_t s) const {
-    hipLaunchKernelGGL(cgemm_planes_kernel, dim3((unsigned)items, (unsigned)nbatch), dim3(256), 0, s, batches.ptr,
-                       nsplit);
+    hipLaunchKernelGGL(cgemm_planes_kernel<kPlaneWaves>, dim3((unsigned)items, (unsigned)nbatch),
+                       dim3(64 * kPlaneWaves), 0, s, batches.ptr, nsplit);
     NC_HIP(hipGetLastError());
 }
 
