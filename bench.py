@@ -159,6 +159,43 @@ def fp64_kernel(table, chi, d):
     return " + ".join(sorted(names))
 
 
+def dropin_timings(core, op, L, R, x, bl, br, d):
+    """The drop-in path as the reference's C++ layer takes it: host arrays through
+    the C-ABI (contract_sos_left_, optimize_ at ncv = 16), every input uploaded
+    and every output downloaded per call (residency off), and with host-pointer
+    residency on, where inputs the library has seen are served from their device
+    copies (the environment handed back unchanged)."""
+    out = np.zeros((br, br, op.cr), complex, order="F")
+    res = {}
+    prev_ncv = core.set_optimizer_ncv(16)
+    try:
+        for name, budget in (("residency_off", 0), ("residency_on", 8 << 30)):
+            core.set_host_residency(budget)
+            core.contract_sos_left(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, x, out=out)  # first call
+            ts = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                core.contract_sos_left(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, x, out=out)
+                ts.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            info, it, y, ev, nrm = core.optimize(bl, br, op.cl, op.cr, d, L, op.idx, op.mats, R, 0, 0, bl * br * d,
+                                                 None, None, None, "SR", 1e-12, 10000, x)
+            t_opt = time.perf_counter() - t0
+            st = core.host_residency_stats()
+            res[name] = {"env_update_ms": round(1e3 * float(np.median(ts)), 3), "optimize_s": round(t_opt, 3),
+                         "optimize_info": int(info), "eigenvalue": float(np.real(ev)), "residency_stats": st}
+            log("drop-in %s: contract_sos_left_ %.2f ms, optimize_ %.3f s" % (name, res[name]["env_update_ms"],
+                                                                             t_opt))
+    finally:
+        core.set_host_residency(0)
+        core.set_optimizer_ncv(prev_ncv)
+    mb_in = (L.nbytes + x.nbytes) / 1e6
+    res["workload"] = ("contract_sos_left_ (inputs %.0f MB, output %.0f MB) and optimize_ (inputs %.0f MB) at the "
+                       "chi=%d bulk site through the C-ABI on host arrays, ncv=16, median of 5 environment updates"
+                       % (mb_in, out.nbytes / 1e6, (L.nbytes + R.nbytes + x.nbytes) / 1e6, bl))
+    return res
+
+
 def sweep_window(chain_mod, ops, chi, seed, ncv, start, window, mixed):
     name = "mixed" if mixed else "fp64"
     t0 = time.perf_counter()
@@ -253,6 +290,8 @@ def main():
                     help="full sweeps of the chi=256 XXZ chain (config 2), FP64 and mixed; 0 skips them")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form (64 chains) timing")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the host-array (drop-in C-ABI) timings with residency off and on")
     ap.add_argument("--batch-chains", type=int, default=64)
     ap.add_argument("--batch-sample", type=int, default=16)
     ap.add_argument("--no-batch-converge", action="store_true",
@@ -387,6 +426,10 @@ def main():
                              "gb_per_s": round(copy_gbs, 1), "hbm_peak_gb_per_s": HBM_PEAK_GBS},
         "device": dev,
     }
+
+    # ---- the drop-in C-ABI on host arrays (PCIe included) -------------------
+    if not args.no_dropin:
+        result["dropin"] = dropin_timings(core, op, L, R, x, bl, br, d)
 
     # ---- the matvec at the other configurations' bulk sites ---------------
     if not args.no_configs:

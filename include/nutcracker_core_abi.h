@@ -264,6 +264,23 @@ int nc_contract_sos_left_noalloc(uint32_t bl, uint32_t br, uint32_t cl, uint32_t
    previous value, or -1 for a value outside that range. */
 int nc_set_optimizer_ncv(int ncv);
 
+/* Host-pointer residency of the C-ABI entry points above (no counterpart in the
+   reference; opt-in, for callers such as the reference's C++ layer, whose host
+   arrays keep their address and are handed back unchanged): with a byte budget
+   > 0 the library keeps a device copy of the arrays it reads and writes, keyed by
+   host address, and does not upload an input again when its address, size and
+   fingerprint (every word of arrays up to 32 KiB; 4096 words spread over larger
+   ones) match a copy.  Outputs are still written to the host.  An in-place edit
+   that touches no sampled word is not seen: forget such an array first.  Copies
+   are evicted least recently used first beyond the budget; 0 turns residency off
+   and frees every copy.  Not for concurrent calls from several threads.
+   nc_set_host_residency returns the previous budget; nc_host_residency_stats
+   fills out[0] hits, out[1] misses, out[2] bytes uploaded, out[3] bytes served
+   from copies, out[4] outputs kept, out[5] copies held, out[6] bytes held. */
+int64_t nc_set_host_residency(int64_t max_bytes);
+int nc_host_residency_forget(const void* host);
+int nc_host_residency_stats(int64_t* out);
+
 /* How mixed-precision single-site solves apply H_eff in complex64: 0 on the
    complex64 GEMM (operands split inside it), 1 on operands split once into
    bf16 planes when bl * br >= 512 * 512 (the default), 2 always on planes.

@@ -39,6 +39,7 @@ SOURCES = [
     "abi.cpp",
     "abi_support.cpp",
     "abi_noalloc.cpp",
+    "residency.cpp",
     "chain_abi.cpp",
     "bench_abi.cpp",
     "formats.cpp",

@@ -47,6 +47,25 @@ int nc_set_mixed_planes(int mode) {
     return prev;
 }
 
+int64_t nc_set_host_residency(int64_t max_bytes) { return residency_budget(max_bytes); }
+
+int nc_host_residency_forget(const void* host) {
+    residency_forget(host);
+    return 0;
+}
+
+int nc_host_residency_stats(int64_t* out) {
+    const ResidencyStats st = residency_stats();
+    out[0] = st.hits;
+    out[1] = st.misses;
+    out[2] = st.bytes_uploaded;
+    out[3] = st.bytes_reused;
+    out[4] = st.outputs_kept;
+    out[5] = st.entries;
+    out[6] = st.bytes_held;
+    return 0;
+}
+
 int nc_optimizer_stats(int64_t* out) {
     const OptimizerStats st = optimizer_stats();
     out[0] = st.solves;
@@ -89,7 +108,7 @@ void contract_sos_left_(const uint32_t* bl, const uint32_t* br, const uint32_t* 
         HostIn psi(state_site_tensor, (size_t)*br * *bl * *d);
         ZBuf out(nout);
         env_left((int)*bl, (int)*br, op, L.ptr(), psi.ptr(), out.ptr, ws(), s);
-        d2h(new_left_environment, out.ptr, nout * sizeof(double2));
+        host_out(new_left_environment, out.ptr, nout);
         return 0;
     });
     if (rc) fill_zero(new_left_environment, nout * sizeof(nc_zcomplex));
@@ -108,7 +127,7 @@ void contract_sos_right_(const uint32_t* bl, const uint32_t* br, const uint32_t*
         HostIn psi(state_site_tensor, (size_t)*br * *bl * *d);
         ZBuf out(nout);
         env_right((int)*bl, (int)*br, op, R.ptr(), psi.ptr(), out.ptr, ws(), s);
-        d2h(new_right_environment, out.ptr, nout * sizeof(double2));
+        host_out(new_right_environment, out.ptr, nout);
         return 0;
     });
     if (rc) fill_zero(new_right_environment, nout * sizeof(nc_zcomplex));
@@ -316,7 +335,7 @@ uint32_t optimize_(const uint32_t* bl, const uint32_t* br, const uint32_t* cl, c
         a.ncv = optimizer_ncv();
         OptimizeOut o = optimize_site(a, ws(), s);
         if (o.info == 5 || o.info == 10 || o.info == 11) return o.info;
-        d2h(result, res.ptr, n * sizeof(double2));
+        host_out(result, res.ptr, n);
         eigenvalue->re = o.eigenvalue.real();
         eigenvalue->im = o.eigenvalue.imag();
         *normal = o.normal;
@@ -340,8 +359,8 @@ int32_t norm_denorm_going_left_(const uint32_t* bll, const uint32_t* bl, const u
         HostIn td(site_tensor_to_denormalize, nd), tn(site_tensor_to_normalize, nn);
         ZBuf od(nd), on(nn);
         int info = norm_denorm_left((int)*bll, (int)*bl, (int)*br, (int)*dl, (int)*d, td.ptr(), tn.ptr(), od.ptr, on.ptr, s);
-        d2h(denormalized_site_tensor, od.ptr, nd * sizeof(double2));
-        d2h(normalized_site_tensor, on.ptr, nn * sizeof(double2));
+        host_out(denormalized_site_tensor, od.ptr, nd);
+        host_out(normalized_site_tensor, on.ptr, nn);
         return info;
     });
     return rc;
@@ -358,8 +377,8 @@ int32_t norm_denorm_going_right_(const uint32_t* bl, const uint32_t* br, const u
         HostIn tn(site_tensor_to_normalize, nn), td(site_tensor_to_denormalize, nd);
         ZBuf on(nn), od(nd);
         int info = norm_denorm_right((int)*bl, (int)*br, (int)*brr, (int)*d, (int)*dr, tn.ptr(), td.ptr(), on.ptr, od.ptr, s);
-        d2h(normalized_site_tensor, on.ptr, nn * sizeof(double2));
-        d2h(denormalized_site_tensor, od.ptr, nd * sizeof(double2));
+        host_out(normalized_site_tensor, on.ptr, nn);
+        host_out(denormalized_site_tensor, od.ptr, nd);
         return info;
     });
     return rc;

@@ -192,13 +192,46 @@ class _ErrorWatch:
         return False
 
 
-def contract_sos_left(bl, br, cl, cr, d, L, idx, mats, psi) -> np.ndarray:
-    """core.hpp contract_sos_left; returns the new left environment (br, br, cr)."""
+def set_host_residency(max_bytes: int) -> int:
+    """Host-pointer residency of the C-ABI (nc_set_host_residency): device copies
+    of the host arrays the entry points read and write, up to max_bytes (0: off).
+    Returns the previous budget."""
+    f = lib().nc_set_host_residency
+    f.restype = C.c_int64
+    f.argtypes = [C.c_int64]
+    return int(f(int(max_bytes)))
+
+
+def host_residency_forget(a: np.ndarray) -> None:
+    """Drop the device copy of a host array (before editing it in place)."""
+    f = lib().nc_host_residency_forget
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p]
+    f(a.ctypes.data)
+
+
+def host_residency_stats() -> dict:
+    f = lib().nc_host_residency_stats
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(C.c_int64)]
+    out = (C.c_int64 * 7)()
+    f(out)
+    keys = ("hits", "misses", "bytes_uploaded", "bytes_reused", "outputs_kept", "entries", "bytes_held")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def contract_sos_left(bl, br, cl, cr, d, L, idx, mats, psi, out=None) -> np.ndarray:
+    """core.hpp contract_sos_left; returns the new left environment (br, br, cr),
+    written into `out` when given (a Fortran-ordered complex128 array of that
+    shape, whose address then stays the same from call to call)."""
     i, nmat = _idx(idx)
     L = _zin(L, bl * bl * cl)
     m = _zin(mats, d * d * nmat)
     p = _zin(psi, br * bl * d)
-    out = np.zeros((br, br, cr), CPLX, order="F")
+    if out is None:
+        out = np.zeros((br, br, cr), CPLX, order="F")
+    elif out.dtype != CPLX or out.size != br * br * cr or not out.flags.f_contiguous:
+        raise ValueError("out must be a Fortran-ordered complex128 array of %d elements" % (br * br * cr))
     with _ErrorWatch():
         lib().contract_sos_left_(_u(bl), _u(br), _u(cl), _u(cr), _u(d), _z(L), _u(nmat),
                                  i.ctypes.data_as(_u32p), _z(m), _z(p), _z(out))
