@@ -47,6 +47,82 @@ __device__ __forceinline__ double kwarp_sum(double v) {
     return v;
 }
 
+// Vectors normalised on the device, with no host read: dst = src / ||src|| (0
+// when the norm is 0), the norm also written to *beta when beta is set.  Partial
+// sums per block and their total are formed in a fixed order.
+struct NormItem {
+    const double2* src;
+    double2* dst;
+    int64_t n;
+    double* beta;
+};
+__global__ __launch_bounds__(256) void knorm_partial_kernel(const NormItem* __restrict__ it,
+                                                            double* __restrict__ partial) {
+    const NormItem d = it[blockIdx.y];
+    double acc = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = d.src[e];
+        acc += v.x * v.x + v.y * v.y;
+    }
+    __shared__ double sh[4];
+    acc = kwarp_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+__global__ __launch_bounds__(64) void knorm_final_kernel(const NormItem* __restrict__ it,
+                                                         const double* __restrict__ partial, int nb,
+                                                         double* __restrict__ scale) {
+    const int p = blockIdx.x;
+    double s2 = 0;
+    for (int b = threadIdx.x; b < nb; b += 64) s2 += partial[(int64_t)p * nb + b];
+    s2 = kwarp_sum(s2);
+    if (threadIdx.x != 0) return;
+    const double beta = sqrt(s2);
+    if (it[p].beta) *it[p].beta = beta;
+    scale[p] = beta > 0.0 ? 1.0 / beta : 0.0;
+}
+__global__ __launch_bounds__(256) void knorm_apply_kernel(const NormItem* __restrict__ it,
+                                                          const double* __restrict__ scale) {
+    const NormItem d = it[blockIdx.y];
+    const double f = scale[blockIdx.y];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < d.n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = d.src[e];
+        d.dst[e] = make_double2(v.x * f, v.y * f);
+    }
+}
+
+// work space of normalize_on_device for up to `items` vectors
+struct NormSpace {
+    DevBuf<double> partial, scale;
+    int cap = 0;
+    static unsigned blocks(size_t items, int64_t maxn) {
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(1, 2048 / (int64_t)items),
+                                                                 (maxn + 255) / 256));
+    }
+    void reserve(int items) {
+        if (items <= cap) return;
+        partial.resize((size_t)items * blocks(1, 1 << 30));
+        scale.resize((size_t)items);
+        cap = items;
+    }
+};
+void normalize_on_device(std::vector<NormItem> const& items, NormSpace& ws, hipStream_t s) {
+    if (items.empty()) return;
+    if ((int)items.size() > ws.cap) throw Error("normalize_on_device: work space too small", -8);
+    int64_t maxn = 1;
+    for (auto const& it : items) maxn = std::max(maxn, it.n);
+    const unsigned g = NormSpace::blocks(items.size(), maxn);
+    int slot;
+    auto* d = (const NormItem*)stage_descriptors(items.data(), items.size() * sizeof(NormItem), s, &slot);
+    const unsigned P = (unsigned)items.size();
+    hipLaunchKernelGGL(knorm_partial_kernel, dim3(g, P), dim3(256), 0, s, d, ws.partial.ptr);
+    hipLaunchKernelGGL(knorm_final_kernel, dim3(P), dim3(64), 0, s, d, ws.partial.ptr, (int)g, ws.scale.ptr);
+    hipLaunchKernelGGL(knorm_apply_kernel, dim3(g, P), dim3(256), 0, s, d, ws.scale.ptr);
+    NC_HIP(hipGetLastError());
+    release_descriptors(slot, s);
+}
+
 // partial dots of columns 0..j+1 of V against w = column j+1 (the last one is
 // ||w||^2): partial[((p * ldp) + c) * nblk + b].  gate (optional): problems
 // whose flag is 0 skip the step.
@@ -487,6 +563,9 @@ struct KState {
     bool done = false;
     int iter = 0;       // restarts performed (ARPACK iparam(3))
     int fresh = 0;      // random vectors drawn so far
+    // a restart normalised v_k on the device; ||f|| = H(k, k-1) is read back with
+    // the Hessenberg columns of the next extension
+    bool restart_pending = false;
 };
 
 // ARPACK's getv0 after an invariant subspace (znaitr step 1, znaupd): the
@@ -557,6 +636,8 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     ZBuf partial((size_t)P * ldp * nblk), coef1((size_t)P * ldp), coef2((size_t)P * ldp), coef3((size_t)P * ldp);
     DevBuf<int> need3(P);
     DevBuf<double> b2s(P);
+    NormSpace nspace;
+    nspace.reserve(2 * P);
     // Hessenberg columns and subdiagonals of all problems in one buffer, read
     // back with one copy per extension
     const size_t nH = (size_t)P * (ncv + 1) * ncv, nB = (size_t)P * ncv;
@@ -705,7 +786,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         struct Rst { int p, keep; std::vector<zd> Q; zd hk, qm; std::vector<zd> Hk; };
         // Per problem, on host threads (no device work): merge the new
         // Hessenberg columns, Ritz values, convergence test, restart shifts.
-        enum { D_NONE, D_STOP, D_FAIL, D_FIN, D_RST };
+        enum { D_NONE, D_STOP, D_FAIL, D_FIN, D_RST, D_RSTZERO };
         struct Dec {
             int kind = D_NONE;
             int stop = -1;
@@ -721,6 +802,19 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             const int iter = ++x.iter;
             auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
             const double2* Hp = Hh + (size_t)p * (ncv + 1) * ncv;
+            if (x.restart_pending) {
+                // the residual norm of the last restart, normalised on the device
+                x.restart_pending = false;
+                const double rb = bh[(size_t)p * ncv + x.k - 1];
+                if (rb == 0.0) {
+                    // the restart landed on an invariant subspace: the extension
+                    // just run started from a zero vector and is dropped
+                    --x.iter;
+                    D.kind = D_RSTZERO;
+                    return;
+                }
+                h(x.k, x.k - 1) = rb;
+            }
             int stop = -1;  // first step whose new vector vanished (invariant subspace)
             for (int c = x.k; c < ncv; ++c) {
                 for (int r = 0; r <= c; ++r) h(r, c) = zd(Hp[r + (size_t)c * (ncv + 1)].x, Hp[r + (size_t)c * (ncv + 1)].y);
@@ -783,6 +877,26 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 t.out.restarts = x.iter;
                 t.out.info = -14;
                 x.done = true;
+            } else if (D.kind == D_RSTZERO) {
+                auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
+                const int keep = x.k;
+                for (int c = keep; c < ncv; ++c)
+                    for (int r = 0; r < ncv; ++r) h(r, c) = 0.0;
+                t.out.matvecs -= ncv - keep;
+                // continue from a random vector orthogonal to the kept ones
+                // (H(keep, keep - 1) = 0), as ARPACK's getv0 does
+                if (continue_with_random(x, p, keep, s)) {
+                    h(keep, keep - 1) = 0.0;
+                    continue;
+                }
+                // the kept vectors span the whole space: their Ritz pairs are exact
+                std::vector<zd> Hk((size_t)keep * keep), th2, Y2;
+                for (int c = 0; c < keep; ++c)
+                    for (int r = 0; r < keep; ++r) Hk[(size_t)r + (size_t)c * keep] = h(r, c);
+                hessenberg_eig(keep, Hk, th2, Y2);
+                const int w2 = wanted_index(th2, t.which);
+                std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
+                finish_with(x, keep, yv, th2[w2], x.iter, s);
             } else if (D.kind == D_STOP) {
                 auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
                 const int stop = D.stop;
@@ -868,76 +982,34 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             axpby_batch(y, a, xv, b, nn, s);
             for (auto const& r : rsts) std::swap(st[r.p].V, st[r.p].Vk);
         }
-        // norms of the results, the new residuals and (keep = 1) of v_0
-        std::vector<const double2*> nv;
-        std::vector<int64_t> nl;
-        for (auto const& f : fins) { nv.push_back(st[f.p].task->result); nl.push_back(st[f.p].n); }
-        for (auto const& r : rsts) { nv.push_back(st[r.p].V.ptr + (size_t)st[r.p].n * ncv); nl.push_back(st[r.p].n); }
-        for (auto const& r : rsts)
-            if (r.keep == 1) { nv.push_back(st[r.p].V.ptr); nl.push_back(st[r.p].n); }
-        std::vector<double> ss = sumsq_batch(nv, nl, s);
-        std::vector<double2*> sx;
-        std::vector<int64_t> sn;
-        std::vector<double> sf;
-        size_t k2 = 0;
+        // Normalised on the device, with no host read: the results; for every
+        // restart v_keep = f / ||f|| (||f|| = H(keep, keep - 1) lands in the
+        // Hessenberg read-back buffer and is taken up with the next extension's
+        // columns); with keep = 1 also v_0 (kept exactly unit norm against
+        // round-off drift).  The host goes straight on to the next extension.
+        std::vector<NormItem> ni;
         for (auto const& f : fins) {
             KState& x = st[f.p];
-            sx.push_back(x.task->result);
-            sn.push_back(x.n);
-            sf.push_back(1.0 / std::sqrt(ss[k2++]));
+            ni.push_back(NormItem{x.task->result, x.task->result, x.n, nullptr});
             x.task->out.eigenvalue = f.theta;
             x.task->out.restarts = f.iter;
             x.task->out.info = 0;
             x.done = true;
         }
-        std::vector<double> betas_r;
-        for (size_t q = 0; q < rsts.size(); ++q) betas_r.push_back(std::sqrt(ss[k2++]));
-        for (auto const& r : rsts)
-            if (r.keep == 1) {  // keep v_0 exactly unit norm against round-off drift
-                sx.push_back(st[r.p].V.ptr);
-                sn.push_back(st[r.p].n);
-                sf.push_back(1.0 / std::sqrt(ss[k2++]));
-            }
-        scale_batch(sx, sn, sf, s);
-        std::vector<double2*> vy;
-        std::vector<const double2*> vx;
-        std::vector<zd> va, vb;
-        std::vector<int64_t> vn;
-        for (size_t q = 0; q < rsts.size(); ++q) {
-            Rst const& r = rsts[q];
+        for (auto const& r : rsts) {
             KState& x = st[r.p];
-            IramTask& t = *x.task;
             const int keep = r.keep;
             auto h = [&](int i, int jj) -> zd& { return x.H[(size_t)i + (size_t)jj * ncv]; };
             std::fill(x.H.begin(), x.H.end(), zd(0, 0));
             for (int c = 0; c < keep; ++c)
                 for (int rr = 0; rr < keep; ++rr) h(rr, c) = r.Hk[(size_t)rr + (size_t)c * keep];
-            x.beta = betas_r[q];
             x.k = keep;
-            if (x.beta == 0.0 && continue_with_random(x, r.p, keep, s)) {
-                // the restart landed on an invariant subspace: continue from a
-                // random vector orthogonal to it (H(keep, keep - 1) = 0)
-                h(keep, keep - 1) = 0.0;
-                continue;
-            }
-            if (x.beta == 0.0) {
-                // the kept vectors span the whole space: their Ritz pairs are exact
-                std::vector<zd> Hk = r.Hk, th2, Y2;
-                hessenberg_eig(keep, Hk, th2, Y2);
-                const int w2 = wanted_index(th2, t.which);
-                std::vector<zd> yv(Y2.begin() + (size_t)w2 * keep, Y2.begin() + (size_t)(w2 + 1) * keep);
-                finish_with(x, keep, yv, th2[w2], x.iter, s);
-                continue;
-            }
-            // the next extension starts from v_keep = f / beta, H(keep, keep - 1) = beta
-            vy.push_back(x.V.ptr + (size_t)x.n * keep);
-            vx.push_back(x.V.ptr + (size_t)x.n * ncv);
-            va.push_back(zd(1.0 / x.beta, 0));
-            vb.push_back(zd(0, 0));
-            vn.push_back(x.n);
-            h(keep, keep - 1) = x.beta;
+            x.restart_pending = true;
+            ni.push_back(NormItem{x.V.ptr + (size_t)x.n * ncv, x.V.ptr + (size_t)x.n * keep, x.n,
+                                  betas_ptr + (size_t)r.p * ncv + keep - 1});
+            if (keep == 1) ni.push_back(NormItem{x.V.ptr, x.V.ptr, x.n, nullptr});
         }
-        axpby_batch(vy, va, vx, vb, vn, s);
+        normalize_on_device(ni, nspace, s);
     }
 }
 
