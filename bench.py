@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the whole chi=1024 sweeps")
+    ap.add_argument("--sweep-forms", default="fp64,mixed",
+                    help="forms whose whole chi=1024 sweep is timed (the comparison needs both)")
     ap.add_argument("--window-sweep", action="store_true",
                     help="also time a window of bulk site steps (optimise + move) in both forms")
     ap.add_argument("--levels", type=int, default=3,
@@ -498,10 +500,19 @@ def main():
         ops = xxz_operator(args.sites, d, 1.0, args.delta)
         res = {}
         chs = {}
+        forms = [f.strip() for f in args.sweep_forms.split(",") if f.strip()]
         for mixed in (False, True):
+            if ("mixed" if mixed else "fp64") not in forms:
+                continue
             ch, r = full_sweep(chain_mod, ops, chi, args.seed + 1, args.ncv, mixed)
             res["mixed" if mixed else "fp64"] = r
             chs["mixed" if mixed else "fp64"] = ch
+    if not args.no_sweep and len(chs) < 2:
+        # one form only (profiling runs): no comparison
+        for ch in chs.values():
+            ch.close()
+        result["full_sweep_single"] = res
+    elif not args.no_sweep:
         ov = chs["fp64"].compute_state_overlap(chs["mixed"])
         for ch in chs.values():
             ch.close()

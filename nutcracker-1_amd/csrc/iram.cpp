@@ -175,6 +175,60 @@ __global__ __launch_bounds__(KT) void kdot_kernel(const KDesc* __restrict__ desc
     }
 }
 
+// The same dots for m = j + 2 <= MM columns in one pass over w (the general
+// kernel above reads w once per group of KC columns).  The per-block partial
+// sums are formed in the same order as there, so the results are the same.
+template <int MM>
+__global__ __launch_bounds__(KT) void kdot1_kernel(const KDesc* __restrict__ desc, int j,
+                                                   double2* __restrict__ partial, int ldp,
+                                                   const int* __restrict__ gate) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    if (gate && gate[p] == 0) return;
+    const int m = j + 2;
+    const int64_t n = d.n;
+    const double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    const int nb = gridDim.x;
+    double re[MM], im[MM];
+#pragma unroll
+    for (int q = 0; q < MM; ++q) { re[q] = 0; im[q] = 0; }
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+        const double2 wv = w[i];
+        double2 a[MM];
+#pragma unroll
+        for (int q = 0; q < MM; ++q)
+            if (q < m) a[q] = d.V[i + (int64_t)q * n];
+#pragma unroll
+        for (int q = 0; q < MM; ++q) {
+            if (q < m) {
+                re[q] += a[q].x * wv.x + a[q].y * wv.y;
+                im[q] += a[q].x * wv.y - a[q].y * wv.x;
+            }
+        }
+    }
+    __shared__ double sh[KT / 64][2 * MM];
+#pragma unroll
+    for (int q = 0; q < MM; ++q) {
+        re[q] = kwarp_sum(re[q]);
+        im[q] = kwarp_sum(im[q]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < MM; ++q) {
+            sh[threadIdx.x >> 6][2 * q] = re[q];
+            sh[threadIdx.x >> 6][2 * q + 1] = im[q];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < MM && (int)threadIdx.x < m) {
+        const int q = threadIdx.x;
+        double tr = 0, ti = 0;
+        for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2 * q]; ti += sh[wv][2 * q + 1]; }
+        partial[((int64_t)p * ldp + q) * nb + blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
 // out[p * ldp + c] = sum_b partial[((p * ldp) + c) * nblk + b], c = 0..j+1
 __global__ __launch_bounds__(KT) void kreduce_kernel(const KDesc* __restrict__ desc, int j,
                                                      const double2* __restrict__ partial, int ldp, int nblk,
@@ -684,9 +738,18 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     };
     // the Gram-Schmidt kernels of step j (classical Gram-Schmidt twice; the
     // first update and the second pass's dots share one read of the basis)
+    // the dots of step j against columns 0..j+1, in one pass over w where the
+    // column count allows it
+    auto dots = [&](int j, const int* gate) {
+        const int m = j + 2;
+        if (m <= 8) hipLaunchKernelGGL(kdot1_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else if (m <= 12) hipLaunchKernelGGL(kdot1_kernel<12>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else if (m <= 18) hipLaunchKernelGGL(kdot1_kernel<18>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+    };
     auto orthogonalize = [&](int j) {
         const int* ng = nullptr;
-        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, ng);
+        dots(j, ng);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
         const int m = j + 1;
         if (m <= 4) hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
@@ -694,14 +757,14 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         else if (m <= 16) hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
         else {
             hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
-            hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, ng);
+            dots(j, ng);
         }
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr, ng);
         hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, ng);
         // znaitr's third pass, only for the problems whose second pass lost too much
         hipLaunchKernelGGL(kcheck_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, nblk,
                            need3.ptr, b2s.ptr);
-        hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, need3.ptr);
+        dots(j, need3.ptr);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef3.ptr,
                            need3.ptr);
         hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
