@@ -209,6 +209,88 @@ __global__ __launch_bounds__(256) void mix_split_kernel(const float2* __restrict
     }
 }
 
+// The same mix for every destination at once: a thread loads each source
+// block's pair (k, k + 1) of row l once (every K-split slice summed), and forms
+// all Q_t(k, l + bl b) from registers, instead of one grid row per destination
+// t reading the sources of its own blocks (the source slabs of a block list
+// with several blocks per source are then read once, not once per block).
+// Up to MAXF distinct sources and MAXT destinations; br even (16-byte pairs).
+template <int D, int MAXF, int MAXT>
+__global__ __launch_bounds__(256) void mix_split_all_kernel(const float2* __restrict__ P, int64_t pslice, int nslice,
+                                                            int nsrc, int ndst, const int* __restrict__ dst_ptr,
+                                                            const int* __restrict__ blk_src,
+                                                            const float2* __restrict__ W, int br, int bl, int64_t rs,
+                                                            unsigned* __restrict__ Q, int64_t ps, int64_t qz, int Np,
+                                                            int Kb) {
+    const int nlg = (bl + 15) >> 4;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int kb = (int)(w / nlg);
+    const int l = (int)(w % nlg) * 16 + (lane >> 2);
+    const int pi = lane & 3;
+    if (kb >= Kb || l >= bl) return;
+    const int k = 8 * kb + 2 * pi;
+    const bool in0 = k < br;
+    const int64_t e = (int64_t)br * l + k;
+    // every source slab's pair, slices summed
+    float4 v[MAXF][D];
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f)
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (f < nsrc && in0) {
+                const float2* pp = P + ((int64_t)f * D + a) * rs + e;
+                x = *reinterpret_cast<const float4*>(pp);
+                for (int c = 1; c < nslice; ++c) {
+                    const float4 u = *reinterpret_cast<const float4*>(pp + c * pslice);
+                    x.x += u.x; x.y += u.y; x.z += u.z; x.w += u.w;
+                }
+            }
+            v[f][a] = x;
+        }
+    const int64_t s4 = 4 * ps;
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+        if (t >= ndst) break;
+        float r0[D], i0[D], r1[D], i1[D];
+#pragma unroll
+        for (int b = 0; b < D; ++b) r0[b] = i0[b] = r1[b] = i1[b] = 0.f;
+        const int jb = dst_ptr[t], je = dst_ptr[t + 1];
+        for (int j = jb; j < je; ++j) {
+            const int f = blk_src[j];
+            const float2* Wj = W + (int64_t)j * D * D;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                // the source's pair, picked from registers
+                float4 x = v[0][a];
+#pragma unroll
+                for (int q = 1; q < MAXF; ++q)
+                    if (f == q) x = v[q][a];
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    const float2 wv = Wj[a + b * D];
+                    r0[b] += wv.x * x.x - wv.y * x.y;
+                    i0[b] += wv.x * x.y + wv.y * x.x;
+                    r1[b] += wv.x * x.z - wv.y * x.w;
+                    i1[b] += wv.x * x.w + wv.y * x.z;
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            unsigned* q = Q + 4 * (t * qz + (int64_t)kb * Np + l + (int64_t)bl * b) + pi;
+            unsigned h, m, lo;
+            split_pair(r0[b], r1[b], h, m, lo);
+            q[0] = h; q[s4] = m; q[2 * s4] = lo;
+            split_pair(i0[b], i1[b], h, m, lo);
+            q[3 * s4] = h; q[4 * s4] = m; q[5 * s4] = lo;
+            split_pair(r0[b] + i0[b], r1[b] + i1[b], h, m, lo);
+            q[6 * s4] = h; q[7 * s4] = m; q[8 * s4] = lo;
+        }
+    }
+}
+
 // out[e] = sum_c Y[c slice + e] (in FP64), e < n
 __global__ __launch_bounds__(256) void reduce_slices_kernel(const float2* __restrict__ Y, int64_t slice, int ns,
                                                             int64_t n, double2* __restrict__ out) {
@@ -449,6 +531,21 @@ void mix_split_planes(MpoSite const& op, const float2* P, int64_t pslice, int ns
     // 16-byte pair loads need an even br (then every pair offset is even) and
     // an even slice stride
     const bool even = br % 2 == 0 && pslice % 2 == 0;
+    const int nsrc = (int)op.froms.size();
+    // every destination from one read of each source, where that still gives
+    // the chip enough waves (measured at chi = 1024: 0.072 against 0.089 ms;
+    // at chi = 256 a launch of 512 waves is slower than the per-destination one)
+    if (even && (op.d == 2 || op.d == 3) && nsrc <= 8 && ndst <= 8 && waves >= 4096) {
+        dim3 g1((unsigned)((waves + 3) / 4));
+        if (op.d == 2)
+            hipLaunchKernelGGL((mix_split_all_kernel<2, 8, 8>), g1, dim3(256), 0, s, P, pslice, nslice, nsrc, ndst, ptr,
+                               src, W, br, bl, rs, q, Q.ps, qz, Q.Rp, Q.Kb);
+        else
+            hipLaunchKernelGGL((mix_split_all_kernel<3, 8, 8>), g1, dim3(256), 0, s, P, pslice, nslice, nsrc, ndst, ptr,
+                               src, W, br, bl, rs, q, Q.ps, qz, Q.Rp, Q.Kb);
+        NC_HIP(hipGetLastError());
+        return;
+    }
 #define NC_MIX_SPLIT(DD, EV)                                                                                      \
     hipLaunchKernelGGL((mix_split_kernel<DD, EV>), grid, dim3(256), 0, s, P, pslice, nslice, ptr, src, W, op.d, br, \
                        bl, rs, q, Q.ps, qz, Q.Rp, Q.Kb)
