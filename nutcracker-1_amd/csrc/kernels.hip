@@ -69,6 +69,55 @@ __global__ __launch_bounds__(256) void mix_kernel(const CT* __restrict__ P, CT* 
     }
 }
 
+// The same mix for every destination at once: one thread, one slab element e,
+// each source block's value P(e, a, src) loaded once (up to MAXF sources) and
+// every Q(e, b, dst) formed from registers, instead of one grid row per
+// destination reading the sources of its own blocks.
+template <typename CT, int D, int MAXF, int MAXT>
+__global__ __launch_bounds__(256) void mix_all_kernel(const CT* __restrict__ P, CT* __restrict__ Q,
+                                                      const int* __restrict__ dst_ptr,
+                                                      const int* __restrict__ blk_src, const CT* __restrict__ W,
+                                                      int nsrc, int ndst, int64_t rs) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rs) return;
+    CT v[MAXF][D];
+#pragma unroll
+    for (int f = 0; f < MAXF; ++f)
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            CT x;
+            x.x = 0; x.y = 0;
+            if (f < nsrc) x = P[((int64_t)f * D + a) * rs + e];
+            v[f][a] = x;
+        }
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+        if (t >= ndst) break;
+        CT q[D];
+#pragma unroll
+        for (int b = 0; b < D; ++b) { q[b].x = 0; q[b].y = 0; }
+        for (int j = dst_ptr[t]; j < dst_ptr[t + 1]; ++j) {
+            const int f = blk_src[j];
+            const CT* Wn = W + (int64_t)j * D * D;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                CT p = v[0][a];
+#pragma unroll
+                for (int r = 1; r < MAXF; ++r)
+                    if (f == r) p = v[r][a];
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    const CT w = Wn[a + b * D];
+                    q[b].x += w.x * p.x - w.y * p.y;
+                    q[b].y += w.x * p.y + w.y * p.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < D; ++b) Q[((int64_t)t * D + b) * rs + e] = q[b];
+    }
+}
+
 template <typename CT>
 static void mix_blocks_t(MpoSite const& op, bool src_is_from, const CT* P, CT* Q, int64_t rs, const CT* W,
                          hipStream_t s) {
@@ -80,6 +129,18 @@ static void mix_blocks_t(MpoSite const& op, bool src_is_from, const CT* P, CT* Q
     dim3 grid((unsigned)((rs + 255) / 256), (unsigned)ndst);
     const int* ptr = op.mix_ptr[dir].ptr;
     const int* srcs = op.mix_src[dir].ptr;
+    // every destination from one read of each source slab, where the slab alone
+    // still gives the chip enough waves (4096)
+    const int nsrc = (int)(src_is_from ? op.froms.size() : op.tos.size());
+    if ((op.d == 2 || op.d == 3) && nsrc <= 8 && ndst <= 8 && rs >= 4096 * 64) {
+        dim3 g1((unsigned)((rs + 255) / 256));
+        if (op.d == 2)
+            hipLaunchKernelGGL((mix_all_kernel<CT, 2, 8, 8>), g1, dim3(256), 0, s, P, Q, ptr, srcs, W, nsrc, ndst, rs);
+        else
+            hipLaunchKernelGGL((mix_all_kernel<CT, 3, 8, 8>), g1, dim3(256), 0, s, P, Q, ptr, srcs, W, nsrc, ndst, rs);
+        NC_HIP(hipGetLastError());
+        return;
+    }
     switch (op.d) {
         case 2: hipLaunchKernelGGL((mix_kernel<CT, 2>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
         case 3: hipLaunchKernelGGL((mix_kernel<CT, 3>), grid, dim3(256), 0, s, P, Q, ptr, srcs, W, md); break;
