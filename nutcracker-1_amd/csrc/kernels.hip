@@ -428,6 +428,35 @@ __global__ __launch_bounds__(RED_THREADS) void zdotu_kernel(const double2* __res
     }
 }
 
+// x <- x - tau (x . v) vc for one Householder reflector, with the dot product
+// kept on the device: every block adds up the nb partial sums zdotu_kernel
+// left (nb <= RED_BLOCKS) before it updates its share of x.
+__global__ __launch_bounds__(RED_THREADS) void refl_update_kernel(double2* __restrict__ x, const double2* __restrict__ vc,
+                                                                  int64_t n, const double2* __restrict__ partial, int nb,
+                                                                  double2 tau) {
+    __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        re += partial[b].x;
+        im += partial[b].y;
+    }
+    re = warp_sum(re);
+    im = warp_sum(im);
+    if ((threadIdx.x & 63) == 0) { sre[threadIdx.x >> 6] = re; sim[threadIdx.x >> 6] = im; }
+    __syncthreads();
+    double tr = 0, ti = 0;
+    for (int k = 0; k < RED_THREADS / 64; ++k) { tr += sre[k]; ti += sim[k]; }
+    // c = -tau * dot
+    const double cr = -(tau.x * tr - tau.y * ti), ci = -(tau.x * ti + tau.y * tr);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = vc[i];
+        double2 a = x[i];
+        a.x += cr * v.x - ci * v.y;
+        a.y += cr * v.y + ci * v.x;
+        x[i] = a;
+    }
+}
+
 __global__ __launch_bounds__(RED_THREADS) void trace_pair_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
                                                                  int64_t b, int64_t c, double2* __restrict__ partial) {
     __shared__ double sre[RED_THREADS / 64], sim[RED_THREADS / 64];
@@ -582,6 +611,18 @@ zd zdotu(const double2* x, const double2* y, int64_t n, hipStream_t s) {
     NC_HIP(hipMemcpyAsync(r.host, r.out.ptr, sizeof(double2), hipMemcpyDeviceToHost, s));
     NC_HIP(hipStreamSynchronize(s));
     return zd(r.host[0].x, r.host[0].y);
+}
+
+int refl_partials() { return RED_BLOCKS; }
+
+void zrefl_apply(double2* x, const double2* v, const double2* vc, zd tau, int64_t n, double2* partial,
+                 hipStream_t s) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(zdotu_kernel, dim3(nb), dim3(RED_THREADS), 0, s, x, v, n, partial);
+    const unsigned g = (unsigned)std::min<int64_t>(1024, (n + RED_THREADS - 1) / RED_THREADS);
+    hipLaunchKernelGGL(refl_update_kernel, dim3(std::max(1u, g)), dim3(RED_THREADS), 0, s, x, vc, n, partial, nb,
+                       make_double2(tau.real(), tau.imag()));
+    NC_HIP(hipGetLastError());
 }
 
 zd trace_pair(const double2* A, const double2* B, int64_t b, int64_t c, hipStream_t s) {

@@ -426,6 +426,11 @@ void zgemv_sub(const double2* V, int64_t ldv, int m, const zd* h, double2* w, in
                hipStream_t s);
 double dznrm2(const double2* x, int64_t n, hipStream_t s);
 zd zdotu(const double2* x, const double2* y, int64_t n, hipStream_t s);  // sum x*y (no conj)
+// x -= tau (sum_i x_i v_i) vc, stream-ordered with no host read; partial holds
+// refl_partials() entries and must not be shared with concurrent work
+void zrefl_apply(double2* x, const double2* v, const double2* vc, zd tau, int64_t n, double2* partial,
+                 hipStream_t s);
+int refl_partials();
 void zscal(double2* x, zd a, int64_t n, hipStream_t s);
 void zaxpy(double2* y, zd a, const double2* x, int64_t n, hipStream_t s);  // y += a x
 // y = a x + b y  (general combination)
@@ -550,6 +555,9 @@ struct BatchApply {
 // persistent batch in a HIP graph and replay it (default on; the environment
 // variable NC_KRYLOV_GRAPHS=0 makes direct launches the default).
 bool& krylov_graphs();
+// first Gram-Schmidt pass against the last two basis vectors only (default on;
+// NC_GS_LOCAL=0 in the environment makes it a full pass)
+bool& gs_local_first();
 void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s);
 
 // Several reductions with one launch pair and one host read: an item is

@@ -48,6 +48,7 @@ __global__ void build_reflectors_kernel(const double2* refl, double2* V, double2
 
 struct ProjectorDevice {
     ZBuf V, Vc;           // nrefl explicit reflectors
+    ZBuf partial;         // per-block dot products of one reflector application
     DevBuf<uint32_t> swaps;
     std::vector<zd> Rtop;  // nrefl x nrefl upper block of the stored reflectors
 };
@@ -97,6 +98,7 @@ void Projector::finish(const std::vector<zd>& rtop, hipStream_t s) {
     if (!nrefl) return;
     dev->V.resize((size_t)n * nrefl);
     dev->Vc.resize((size_t)n * nrefl);
+    dev->partial.resize(refl_partials());
     unsigned g = (unsigned)std::min<int64_t>(4096, ((int64_t)n * nrefl + 255) / 256);
     hipLaunchKernelGGL(build_reflectors_kernel, dim3(g), dim3(256), 0, s, d_refl.ptr, dev->V.ptr, dev->Vc.ptr,
                        (int64_t)n, nrefl);
@@ -106,7 +108,9 @@ void Projector::finish(const std::vector<zd>& rtop, hipStream_t s) {
     dev->Rtop = rtop;
 }
 
-// x <- x H_0 H_1 ... (forward, zunmqr 'R','N') or x <- x H_{k-1}^H ... H_0^H ('R','C')
+// x <- x H_0 H_1 ... (forward, zunmqr 'R','N') or x <- x H_{k-1}^H ... H_0^H ('R','C');
+// two launches per reflector and no host read, so a projected matvec stays
+// stream-ordered
 static void apply_reflectors_row(Projector const& p, double2* x, bool conj_trans, hipStream_t s) {
     ProjectorDevice& dv = proj_dev(p);
     const int64_t n = p.n;
@@ -115,8 +119,7 @@ static void apply_reflectors_row(Projector const& p, double2* x, bool conj_trans
         const zd tau = conj_trans ? std::conj(p.coeffs[i]) : p.coeffs[i];
         if (tau == zd(0, 0)) continue;
         // x H = x - tau (x v) v^H
-        const zd sdot = zdotu(x, dv.V.ptr + (size_t)i * n, n, s);
-        zaxpy(x, -tau * sdot, dv.Vc.ptr + (size_t)i * n, n, s);
+        zrefl_apply(x, dv.V.ptr + (size_t)i * n, dv.Vc.ptr + (size_t)i * n, tau, n, dv.partial.ptr, s);
     }
 }
 
