@@ -7,7 +7,11 @@
 //     norm -- here it always runs, which is never less orthogonal), a third
 //     pass when the second loses as much, and znaitr's rule for the residual:
 //     zero (an invariant subspace) only when the last pass taken also lost
-//     that much, otherwise kept as the next direction however small;
+//     that much, otherwise kept as the next direction however small.  The
+//     first pass projects on the last two basis vectors only: for the
+//     Hermitian effective Hamiltonian the other components are round-off,
+//     which the full second pass removes (a non-Hermitian operator leaves
+//     more, and the second and third passes then act as znaitr's two);
 //   * Ritz pairs of the small Hessenberg H on the host; convergence when the
 //     Ritz estimate |beta e_m^T s| <= tol max(eps^(2/3), |theta|) (znconv);
 //   * restart by ncv - 1 shifted QR steps with the unwanted Ritz values
@@ -226,6 +230,60 @@ __global__ __launch_bounds__(KT) void kdot1_kernel(const KDesc* __restrict__ des
         double tr = 0, ti = 0;
         for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2 * q]; ti += sh[wv][2 * q + 1]; }
         partial[((int64_t)p * ldp + q) * nb + blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
+// The first Gram-Schmidt pass against the last two basis vectors only: the
+// partial dots of V_{j-1}, V_j against w and ||w||^2 (entry j+1), in the layout
+// of kdot1_kernel, with zeros for the older columns.  For a Hermitian operator
+// A v_j lies in span(v_{j-1}, v_j, v_{j+1}) (the Hessenberg matrix is
+// tridiagonal), so the full second pass only removes round-off along the older
+// vectors; a non-Hermitian operator leaves more there, which the second pass
+// removes and the gated third pass refines.
+__global__ __launch_bounds__(KT) void kdotl_kernel(const KDesc* __restrict__ desc, int j,
+                                                   double2* __restrict__ partial, int ldp) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    const int64_t n = d.n;
+    const double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    const double2* __restrict__ vj = d.V + (int64_t)j * n;
+    const double2* __restrict__ vm = d.V + (int64_t)(j > 0 ? j - 1 : 0) * n;
+    const int nb = gridDim.x;
+    double r0 = 0, i0 = 0, r1 = 0, i1 = 0, nw = 0;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+        const double2 wv = w[i], a = vj[i];
+        const double2 b = j > 0 ? vm[i] : make_double2(0.0, 0.0);
+        r1 += a.x * wv.x + a.y * wv.y;
+        i1 += a.x * wv.y - a.y * wv.x;
+        r0 += b.x * wv.x + b.y * wv.y;
+        i0 += b.x * wv.y - b.y * wv.x;
+        nw += wv.x * wv.x + wv.y * wv.y;
+    }
+    r0 = kwarp_sum(r0);
+    i0 = kwarp_sum(i0);
+    r1 = kwarp_sum(r1);
+    i1 = kwarp_sum(i1);
+    nw = kwarp_sum(nw);
+    __shared__ double sh[KT / 64][5];
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6][0] = r0;
+        sh[threadIdx.x >> 6][1] = i0;
+        sh[threadIdx.x >> 6][2] = r1;
+        sh[threadIdx.x >> 6][3] = i1;
+        sh[threadIdx.x >> 6][4] = nw;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= j + 1; c += KT) {
+        double tr = 0, ti = 0;
+        if (c == j - 1) {
+            for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][0]; ti += sh[wv][1]; }
+        } else if (c == j) {
+            for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2]; ti += sh[wv][3]; }
+        } else if (c == j + 1) {
+            for (int wv = 0; wv < KT / 64; ++wv) tr += sh[wv][4];
+        }
+        partial[((int64_t)p * ldp + c) * nb + blockIdx.x] = make_double2(tr, ti);
     }
 }
 
@@ -747,9 +805,11 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         else if (m <= 18) hipLaunchKernelGGL(kdot1_kernel<18>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
         else hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
     };
+    const bool local_first = gs_local_first();
     auto orthogonalize = [&](int j) {
         const int* ng = nullptr;
-        dots(j, ng);
+        if (local_first) hipLaunchKernelGGL(kdotl_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
+        else dots(j, ng);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
         const int m = j + 1;
         if (m <= 4) hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
@@ -1088,6 +1148,15 @@ IramResult iram(Operator& op, int ncv, const char* which, double tol, int max_re
     t[0].result = result;
     iram_batch(t, BatchApply::each(t), s);
     return t[0].out;
+}
+
+bool& gs_local_first() {
+    // NC_GS_LOCAL=0 in the environment makes the first pass a full one
+    static bool on = [] {
+        const char* e = std::getenv("NC_GS_LOCAL");
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 bool& krylov_graphs() {
