@@ -349,24 +349,105 @@ __global__ __launch_bounds__(KT) void kupdate_kernel(const KDesc* __restrict__ d
     }
 }
 
+// The second update, with znaitr's test taken before it from the norms
+// (||w2||^2 = ||w1||^2 - ||c2||^2 for the second pass's coefficients c2).  When
+// the update keeps more than 0.8 of the norm (no third pass follows) and
+// `allow` (step j + 1 exists and its fused kernel takes the correction), the
+// update is deferred: column j + 1 keeps w1, which the finishing kernels scale
+// by 1 / ||w2||, and step j + 1's fused kernel subtracts V_{0..j} corr,
+// corr = c2 / ||w2||, from it.  For the Hermitian effective Hamiltonian c2 is
+// round-off (or the complex64 operator's departure from Hermitian), so the
+// operator of step j + 1 sees a vector within that much of the final one; the
+// Hessenberg column of step j + 1 accounts for the difference (kfinish).
+// flag[p] = 1 marks a deferred update (and b2s[p] = ||w2||).
+__global__ __launch_bounds__(KT) void kupdate2_kernel(const KDesc* __restrict__ desc, int j,
+                                                      const double2* __restrict__ coef, int ldp,
+                                                      double* __restrict__ pnorm, int allow, int* __restrict__ flag,
+                                                      double2* __restrict__ corr, double* __restrict__ b2s) {
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    __shared__ double2 sc[64];
+    __shared__ double sn[KT / 64];
+    __shared__ int sdelay;
+    __shared__ double sb2;
+    const int m = j + 1;
+    for (int c = threadIdx.x; c < m; c += KT) sc[c] = coef[(int64_t)p * ldp + c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s2 = 0;
+        for (int c = 0; c < m; ++c) s2 += sc[c].x * sc[c].x + sc[c].y * sc[c].y;
+        const double b1sq = coef[(int64_t)p * ldp + m].x;
+        const double b2sq = b1sq - s2;
+        const int delay = (allow && b2sq > 0.64 * b1sq && b2sq > 0.0) ? 1 : 0;
+        sdelay = delay;
+        sb2 = delay ? sqrt(b2sq) : 0.0;
+    }
+    __syncthreads();
+    if (sdelay) {
+        if (blockIdx.x == 0) {
+            const double inv = 1.0 / sb2;
+            for (int c = threadIdx.x; c < m; c += KT)
+                corr[(int64_t)p * ldp + c] = make_double2(sc[c].x * inv, sc[c].y * inv);
+            if (threadIdx.x == 0) {
+                flag[p] = 1;
+                b2s[p] = sb2;
+            }
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[p] = 0;
+    const int64_t n = d.n;
+    double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)gridDim.x * KT) {
+        double2 v = w[i];
+        for (int c = 0; c < m; ++c) {
+            const double2 a = d.V[i + (int64_t)c * n];
+            const double2 h = sc[c];
+            v.x -= a.x * h.x - a.y * h.y;
+            v.y -= a.x * h.y + a.y * h.x;
+        }
+        w[i] = v;
+        acc += v.x * v.x + v.y * v.y;
+    }
+    acc = kwarp_sum(acc);
+    if ((threadIdx.x & 63) == 0) sn[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < KT / 64; ++k) t += sn[k];
+        pnorm[(int64_t)p * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // The first update fused with the second pass's dot products: w -= sum_{c<=j}
 // V_c coef_c, then the partial dots of V_0..V_j against the updated w and its
 // squared norm (entry j+1), in the layout of kdot_kernel.  Each row's basis
 // entries are loaded once, into registers, and serve both.
+// When step j - 1 deferred its second update (pend[p]), column j holds
+// v' = w1 / beta and is first corrected to v_j = v' - sum_{c<j} V_c corr_c, in
+// registers and written back.
 template <int MC>
 __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ desc, int j,
                                                     const double2* __restrict__ coef, int ldp,
-                                                    double2* __restrict__ partial) {
+                                                    double2* __restrict__ partial, const int* __restrict__ pend,
+                                                    const double2* __restrict__ corr) {
     const int p = blockIdx.y;
     const KDesc d = desc[p];
     if (j < d.kstart) return;
-    __shared__ double2 sc[MC];
+    __shared__ double2 sc[MC], scr[MC];
     __shared__ double sh[KT / 64][2 * MC + 1];
     const int m = j + 1;
-    for (int c = threadIdx.x; c < m; c += KT) sc[c] = coef[(int64_t)p * ldp + c];
+    const bool fix = pend != nullptr && j - 1 >= d.kstart && pend[p] != 0;
+    for (int c = threadIdx.x; c < m; c += KT) {
+        sc[c] = coef[(int64_t)p * ldp + c];
+        scr[c] = (fix && c < m - 1) ? corr[(int64_t)p * ldp + c] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
     const int64_t n = d.n;
     double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    double2* __restrict__ vlast = d.V + (int64_t)j * n;
     double re[MC], im[MC], nrm = 0;
 #pragma unroll
     for (int c = 0; c < MC; ++c) { re[c] = 0; im[c] = 0; }
@@ -378,6 +459,25 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
         for (int c = 0; c < MC; ++c)
             if (c < m) a[c] = d.V[i + (int64_t)c * n];
         double2 v = w[i];
+        if (fix) {
+            double cx = 0, cy = 0;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                if (c < m - 1) {
+                    const double2 h = scr[c];
+                    cx += a[c].x * h.x - a[c].y * h.y;
+                    cy += a[c].x * h.y + a[c].y * h.x;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                if (c == m - 1) {
+                    a[c].x -= cx;
+                    a[c].y -= cy;
+                    vlast[i] = a[c];
+                }
+            }
+        }
 #pragma unroll
         for (int c = 0; c < MC; ++c) {
             if (c < m) {
@@ -438,9 +538,14 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
 __global__ __launch_bounds__(64) void kcheck_kernel(const KDesc* __restrict__ desc, int j,
                                                     const double2* __restrict__ coef2, int ldp,
                                                     const double* __restrict__ pnorm, int nblk,
-                                                    int* __restrict__ need3, double* __restrict__ b2s) {
+                                                    int* __restrict__ need3, double* __restrict__ b2s,
+                                                    const int* __restrict__ deferred) {
     const int p = blockIdx.x;
     if (j < desc[p].kstart) return;
+    if (deferred && deferred[p]) {
+        if (threadIdx.x == 0) need3[p] = 0;
+        return;
+    }
     double b2 = 0;
     for (int b = threadIdx.x; b < nblk; b += 64) b2 += pnorm[(int64_t)p * nblk + b];
     for (int off = 32; off > 0; off >>= 1) b2 += __shfl_xor(b2, off, 64);
@@ -458,37 +563,83 @@ __global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ d
                                                      const double* __restrict__ pnorm, int nblk,
                                                      const int* __restrict__ need3, const double* __restrict__ b2s,
                                                      double2* __restrict__ H, double* __restrict__ betas,
-                                                     double* __restrict__ scale) {
+                                                     double* __restrict__ scale, const int* __restrict__ pend,
+                                                     const int* __restrict__ deferred,
+                                                     const double2* __restrict__ corr,
+                                                     const double2* __restrict__ gin, double2* __restrict__ gout) {
     const int p = blockIdx.x;
-    if (j < desc[p].kstart) return;
+    const int kstart = desc[p].kstart;
+    if (j < kstart) return;
     const bool third = need3[p] != 0;
     double b2 = 0;
     if (third) {
         for (int b = threadIdx.x; b < nblk; b += 64) b2 += pnorm[(int64_t)p * nblk + b];
         for (int off = 32; off > 0; off >>= 1) b2 += __shfl_xor(b2, off, 64);
     }
-    if (threadIdx.x != 0) return;
-    // beta = ||w|| after the last pass taken; zero when that pass also removed
-    // more than 1 - 0.717 of its input (the residual lies in the basis span)
-    double beta = b2s[p];
-    if (third) {
-        const double beta3 = sqrt(b2);
-        beta = (beta3 > 0.717 * beta) ? beta3 : 0.0;
-    }
     double2* Hp = H + (int64_t)p * (ncv + 1) * ncv;
-    for (int r = 0; r <= j; ++r) {
-        const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
-        double2 h = make_double2(a.x + c.x, a.y + c.y);
+    __shared__ double2 hcol[64];  // this step's column, for the g term below
+    __shared__ double sbeta;
+    if (threadIdx.x == 0) {
+        // beta = ||w|| after the last pass taken; zero when that pass also removed
+        // more than 1 - 0.717 of its input (the residual lies in the basis span)
+        double beta = b2s[p];
         if (third) {
-            const double2 e = coef3[(int64_t)p * ldp + r];
-            h.x += e.x;
-            h.y += e.y;
+            const double beta3 = sqrt(b2);
+            beta = (beta3 > 0.717 * beta) ? beta3 : 0.0;
         }
-        Hp[r + (int64_t)j * (ncv + 1)] = h;
+        // the operator of step j ran on v' = v_j + V corr (a deferred update of
+        // step j - 1), so A v_j = A v' - V g with g = H corr, g formed by the
+        // previous step's kfinish
+        const bool fixed = pend != nullptr && j - 1 >= kstart && pend[p] != 0;
+        for (int r = 0; r <= j; ++r) {
+            const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
+            double2 h = make_double2(a.x + c.x, a.y + c.y);
+            if (third) {
+                const double2 e = coef3[(int64_t)p * ldp + r];
+                h.x += e.x;
+                h.y += e.y;
+            }
+            if (fixed) {
+                const double2 gg = gin[(int64_t)p * ldp + r];
+                h.x -= gg.x;
+                h.y -= gg.y;
+            }
+            Hp[r + (int64_t)j * (ncv + 1)] = h;
+            hcol[r] = h;
+        }
+        Hp[(j + 1) + (int64_t)j * (ncv + 1)] = make_double2(beta, 0.0);
+        betas[(int64_t)p * ncv + j] = beta;
+        scale[p] = beta > 0 ? 1.0 / beta : 0.0;
+        sbeta = beta;
     }
-    Hp[(j + 1) + (int64_t)j * (ncv + 1)] = make_double2(beta, 0.0);
-    betas[(int64_t)p * ncv + j] = beta;
-    scale[p] = beta > 0 ? 1.0 / beta : 0.0;
+    if (deferred == nullptr || deferred[p] == 0) return;
+    __syncthreads();
+    // this step's update was deferred: g = H(0..j+1, 0..j) corr for step j + 1.
+    // Subdiagonal entries from this extension (and the one the restart left at
+    // kstart - 1) are in betas, the kept block of a restart in H.
+    for (int r = threadIdx.x; r <= j + 1; r += 64) {
+        double gx = 0, gy = 0;
+        for (int c = (r > 0 ? r - 1 : 0); c <= j; ++c) {
+            double2 h;
+            if (c == j) h = (r == j + 1) ? make_double2(sbeta, 0.0) : hcol[r];
+            else if (r == c + 1 && c >= kstart - 1) h = make_double2(betas[(int64_t)p * ncv + c], 0.0);
+            else h = Hp[r + (int64_t)c * (ncv + 1)];
+            const double2 q = corr[(int64_t)p * ldp + c];
+            gx += h.x * q.x - h.y * q.y;
+            gy += h.x * q.y + h.y * q.x;
+        }
+        gout[(int64_t)p * ldp + r] = make_double2(gx, gy);
+    }
+}
+
+// the kept Hessenberg block of each restarting problem, in the device copy the
+// deferred updates read: slot q = {(p, keep), H(0..keep-1, 0..keep-1)}
+__global__ __launch_bounds__(64) void khk_kernel(const double2* __restrict__ src, int slot, int ncv,
+                                                 double2* __restrict__ H) {
+    const double2* sq = src + (int64_t)blockIdx.x * slot;
+    const int p = (int)sq[0].x, keep = (int)sq[0].y;
+    double2* Hp = H + (int64_t)p * (ncv + 1) * ncv;
+    for (int e = threadIdx.x; e < keep * keep; e += 64) Hp[(e % keep) + (int64_t)(e / keep) * (ncv + 1)] = sq[1 + e];
 }
 
 // V(:, j+1) *= scale (the next basis vector), unless it is the residual f
@@ -806,30 +957,50 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         else hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
     };
     const bool local_first = gs_local_first();
+    const bool defer = gs_defer_second();
+    // deferred second updates: flags per step parity, the correction
+    // coefficients, and the Hessenberg term g per step parity
+    DevBuf<int> dflag((size_t)2 * P);
+    ZBuf corr((size_t)P * ldp), gbuf((size_t)2 * P * ldp);
+    dzero(dflag.ptr, dflag.bytes());
     auto orthogonalize = [&](int j) {
         const int* ng = nullptr;
+        int* const fcur = dflag.ptr + (size_t)(j & 1) * P;           // step j's deferral
+        const int* const fprev = dflag.ptr + (size_t)((j + 1) & 1) * P;  // step j - 1's
+        // step j + 1 exists and its fused kernel (at most 16 columns) applies the correction
+        const int allow = (defer && j + 1 < ncv && j + 2 <= 16) ? 1 : 0;
         if (local_first) hipLaunchKernelGGL(kdotl_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
         else dots(j, ng);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
         const int m = j + 1;
-        if (m <= 4) hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
-        else if (m <= 8) hipLaunchKernelGGL(kupdot_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
-        else if (m <= 16) hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr);
+        if (m <= 4)
+            hipLaunchKernelGGL(kupdot_kernel<4>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
+                               (const double2*)corr.ptr);
+        else if (m <= 8)
+            hipLaunchKernelGGL(kupdot_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
+                               (const double2*)corr.ptr);
+        else if (m <= 16)
+            hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
+                               (const double2*)corr.ptr);
         else {
             hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
             dots(j, ng);
         }
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr, ng);
-        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, ng);
+        hipLaunchKernelGGL(kupdate2_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, allow, fcur,
+                           corr.ptr, b2s.ptr);
         // znaitr's third pass, only for the problems whose second pass lost too much
         hipLaunchKernelGGL(kcheck_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, nblk,
-                           need3.ptr, b2s.ptr);
+                           need3.ptr, b2s.ptr, (const int*)fcur);
         dots(j, need3.ptr);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef3.ptr,
                            need3.ptr);
         hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
         hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr,
-                           coef3.ptr, ldp, pnorm.ptr, nblk, need3.ptr, b2s.ptr, Hd_ptr, betas_ptr, scale.ptr);
+                           coef3.ptr, ldp, pnorm.ptr, nblk, need3.ptr, b2s.ptr, Hd_ptr, betas_ptr, scale.ptr, fprev,
+                           (const int*)fcur, (const double2*)corr.ptr,
+                           (const double2*)(gbuf.ptr + (size_t)(j & 1) * P * ldp),
+                           gbuf.ptr + (size_t)((j + 1) & 1) * P * ldp);
         hipLaunchKernelGGL(kscale_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, ncv, scale.ptr);
         NC_HIP(hipGetLastError());
     };
@@ -1059,12 +1230,28 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             for (size_t e = 0; e < (size_t)ncv * (rsts[q].keep + 1); ++e)
                 hc.push_back(make_double2(rsts[q].Q[e].real(), rsts[q].Q[e].imag()));
         }
+        // the kept Hessenberg blocks, for the device copy that deferred updates read
+        const int hk_slot = 1 + ncv * ncv;
+        const size_t off_hk = hc.size();
+        if (defer) {
+            for (auto const& r : rsts) {
+                hc.push_back(make_double2((double)r.p, (double)r.keep));
+                for (int e = 0; e < ncv * ncv; ++e)
+                    hc.push_back(e < r.keep * r.keep ? make_double2(r.Hk[e].real(), r.Hk[e].imag())
+                                                     : make_double2(0.0, 0.0));
+            }
+        }
         if (coefs.n < hc.size()) coefs.resize(hc.size());
         // stream-ordered upload; the host array is rewritten only after a later
         // host wait (the norms below)
         hcoef.reserve(hc.size());
         std::copy(hc.begin(), hc.end(), hcoef.ptr);
         NC_HIP(hipMemcpyAsync(coefs.ptr, hcoef.ptr, hc.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (defer && !rsts.empty()) {
+            hipLaunchKernelGGL(khk_kernel, dim3((unsigned)rsts.size()), dim3(64), 0, s,
+                               (const double2*)(coefs.ptr + off_hk), hk_slot, ncv, Hd_ptr);
+            NC_HIP(hipGetLastError());
+        }
         // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
         GemmPlan gp;
         gp.M = gp.N = 1;
@@ -1154,6 +1341,15 @@ bool& gs_local_first() {
     // NC_GS_LOCAL=0 in the environment makes the first pass a full one
     static bool on = [] {
         const char* e = std::getenv("NC_GS_LOCAL");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+bool& gs_defer_second() {
+    // NC_GS_DEFER=0 in the environment runs every second update in its own step
+    static bool on = [] {
+        const char* e = std::getenv("NC_GS_DEFER");
         return !(e && e[0] == '0');
     }();
     return on;

@@ -558,6 +558,10 @@ bool& krylov_graphs();
 // first Gram-Schmidt pass against the last two basis vectors only (default on;
 // NC_GS_LOCAL=0 in the environment makes it a full pass)
 bool& gs_local_first();
+// second Gram-Schmidt update deferred into the next step's fused kernel where
+// the second pass's coefficients are small (default on; NC_GS_DEFER=0 in the
+// environment turns it off)
+bool& gs_defer_second();
 void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s);
 
 // Several reductions with one launch pair and one host read: an item is
