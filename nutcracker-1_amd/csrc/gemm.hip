@@ -24,6 +24,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <map>
 #include <mutex>
 
@@ -373,17 +374,27 @@ struct DmaStage {
 // the k slot of (column n, k) in a stage's b
 __device__ __forceinline__ int dma_bslot(int n, int k) { return n * DBK + (k ^ ((n >> 1) & 7)); }
 
-// one segment's per-lane copy offsets (bytes from the stage's A / B base)
 struct DmaOffsets {
     unsigned a0, a1, b0, b1;
 };
+// one segment's per-lane copy offsets (bytes from the stage's A / B base).
+// BC: op(B) = B^H with B stored N x K (the Gram matrix X X^H), so op(B)'s
+// rows at one k are 64 consecutive elements of B's column k, copied as A's
+// runs are, into b viewed as [k][n]; the conjugate is taken when the
+// fragments are read.
+template <bool BC>
 __device__ __forceinline__ DmaOffsets dma_offsets(int64_t lda, int64_t ldb, int wave, int lane) {
     DmaOffsets o;
     o.a0 = 16u * (unsigned)(lane + (2 * wave) * lda);
     o.a1 = 16u * (unsigned)(lane + (2 * wave + 1) * lda);
-    const int n0 = 8 * (2 * wave) + (lane >> 3), n1 = n0 + 8, ks = lane & 7;
-    o.b0 = 16u * (unsigned)((ks ^ ((n0 >> 1) & 7)) + n0 * ldb);
-    o.b1 = 16u * (unsigned)((ks ^ ((n1 >> 1) & 7)) + n1 * ldb);
+    if constexpr (BC) {
+        o.b0 = 16u * (unsigned)(lane + (2 * wave) * ldb);
+        o.b1 = 16u * (unsigned)(lane + (2 * wave + 1) * ldb);
+    } else {
+        const int n0 = 8 * (2 * wave) + (lane >> 3), n1 = n0 + 8, ks = lane & 7;
+        o.b0 = 16u * (unsigned)((ks ^ ((n0 >> 1) & 7)) + n0 * ldb);
+        o.b1 = 16u * (unsigned)((ks ^ ((n1 >> 1) & 7)) + n1 * ldb);
+    }
     return o;
 }
 
@@ -397,9 +408,10 @@ __device__ __forceinline__ void dma_stage_barrier() {
     else asm volatile("s_waitcnt vmcnt(8)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <bool M3>
+template <bool M3, bool BC>
 __device__ __forceinline__ void dma_compute(const DmaStage& S, Cplx<double>::v4 (&acc)[M3 ? 3 : 2][2][2], int wm,
                                             int wn, int lrow, int lk) {
+    const double2(*bt)[64] = reinterpret_cast<const double2(*)[64]>(S.b);
 #pragma unroll
     for (int q = 0; q < DBK / 4; ++q) {
         const int k = 4 * q + lk;
@@ -407,7 +419,14 @@ __device__ __forceinline__ void dma_compute(const DmaStage& S, Cplx<double>::v4 
 #pragma unroll
         for (int i = 0; i < 2; ++i) a[i] = S.a[k][wm + 16 * i + lrow];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = S.b[dma_bslot(wn + 16 * j + lrow, k)];
+        for (int j = 0; j < 2; ++j) {
+            if constexpr (BC) {
+                b[j] = bt[k][wn + 16 * j + lrow];
+                b[j].y = -b[j].y;
+            } else {
+                b[j] = S.b[dma_bslot(wn + 16 * j + lrow, k)];
+            }
+        }
         if constexpr (M3) {
             double as[2], bs[2];
 #pragma unroll
@@ -450,7 +469,7 @@ __device__ __forceinline__ void dma_compute(const DmaStage& S, Cplx<double>::v4 
 //     column panels of B instead of all of them;
 //   xmap 0: each XCD takes a contiguous row-major range of every batch's tiles.
 // Within an XCD's share the tiles run in row-major order.
-template <bool ACC, bool M3, int NST>
+template <bool ACC, bool M3, int NST, bool BC>
 __global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kernel(
     const BatchT<double>* __restrict__ batches, const SegT<double>* __restrict__ segs, int xmap, int xa, int xb) {
     typedef Cplx<double>::v4 V4;
@@ -508,12 +527,12 @@ __global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kerne
         // copies (the extra ones land in stages that are never read)
         SegT<double> sg = segs[sidx];
         int k0 = 0;
-        DmaOffsets off = dma_offsets(sg.lda, sg.ldb, wave, lane);
+        DmaOffsets off = dma_offsets<BC>(sg.lda, sg.ldb, wave, lane);
         int nks = 0;
         for (int i = bt.seg_begin; i < bt.seg_end; ++i) nks += segs[i].K / DBK;
         auto issue = [&](DmaStage& st) {
             const double2* pa = sg.A + m0 + (int64_t)k0 * sg.lda;
-            const double2* pb = sg.B + k0 + (int64_t)n0 * sg.ldb;
+            const double2* pb = BC ? sg.B + n0 + (int64_t)k0 * sg.ldb : sg.B + k0 + (int64_t)n0 * sg.ldb;
             lds_dma16(pa, off.a0, &st.a[2 * wave][0]);
             lds_dma16(pa, off.a1, &st.a[2 * wave + 1][0]);
             lds_dma16(pb, off.b0, &st.b[64 * (2 * wave)]);
@@ -524,7 +543,7 @@ __global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kerne
                 ++sidx;
                 sg = segs[sidx];
                 k0 = 0;
-                off = dma_offsets(sg.lda, sg.ldb, wave, lane);
+                off = dma_offsets<BC>(sg.lda, sg.ldb, wave, lane);
             }
         };
 #pragma unroll
@@ -535,7 +554,7 @@ __global__ __launch_bounds__(256, (NST == 3 && M3) ? 3 : 2) void cgemm_dma_kerne
             // the stage consumed in the previous step is free for the copies of
             // the step NST - 1 ahead
             issue(S[cur == 0 ? NST - 1 : cur - 1]);
-            dma_compute<M3>(S[cur], acc, wm, wn, lrow, lk);
+            dma_compute<M3, BC>(S[cur], acc, wm, wn, lrow, lk);
             cur = cur == NST - 1 ? 0 : cur + 1;
         }
         // no copy may still write LDS when the workgroup ends
@@ -884,11 +903,11 @@ void launch_typed(int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, cons
     else launch_bk<T, OPA, OPB, 16, 1>(acc, m3, grid, b, s, st);
 }
 
-template <int NST>
+template <int NST, bool BC>
 void launch_dma(bool acc, bool m3, dim3 grid, const BatchT<double>* b, const SegT<double>* s, int xmap, int xa,
                 int xb, hipStream_t st) {
 #define NC_L(ACCV, M3V) \
-    hipLaunchKernelGGL((cgemm_dma_kernel<ACCV, M3V, NST>), grid, dim3(256), 0, st, b, s, xmap, xa, xb)
+    hipLaunchKernelGGL((cgemm_dma_kernel<ACCV, M3V, NST, BC>), grid, dim3(256), 0, st, b, s, xmap, xa, xb)
     if (acc) { if (m3) NC_L(true, true); else NC_L(true, false); }
     else { if (m3) NC_L(false, true); else NC_L(false, false); }
 #undef NC_L
@@ -903,13 +922,18 @@ template <typename T>
 void launch_any(int opA, int opB, int bk, bool acc, bool m3, dim3 grid, const BatchT<T>* b, const SegT<T>* s,
                 hipStream_t st, XcdMap xm = XcdMap()) {
     if (bk == kDmaF64 || bk == kDmaF64x4) {
-        // build_plan keeps these codes only for eligible complex128 NN plans
+        // build_plan keeps these codes only for eligible complex128 N-N and N-C plans
         if constexpr (sizeof(T) == sizeof(double)) {
-            if (opA == OpN && opB == OpN) {
+            if (opA == OpN && (opB == OpN || opB == OpC)) {
                 const auto* bd = reinterpret_cast<const BatchT<double>*>(b);
                 const auto* sd = reinterpret_cast<const SegT<double>*>(s);
-                if (bk == kDmaF64) launch_dma<3>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
-                else launch_dma<4>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                if (opB == OpN) {
+                    if (bk == kDmaF64) launch_dma<3, false>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                    else launch_dma<4, false>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                } else {
+                    if (bk == kDmaF64) launch_dma<3, true>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                    else launch_dma<4, true>(acc, m3, grid, bd, sd, xm.mode, xm.a, xm.b, st);
+                }
                 return;
             }
         }
@@ -1000,21 +1024,22 @@ struct GemmChoice {
 // a choice forced on every plan (tests of each kernel variant); bk = 0: none
 GemmChoice g_forced{0, 0, -1};
 
-// whether a built plan can run on cgemm_dma_kernel: complex128, both operands
-// untransposed, every batch's M and N multiples of the 64 x 64 tile, every
-// segment's K a positive multiple of the 8-deep stage, and the per-lane copy
-// offsets (at most 7 columns of A and 63 columns of B past a stage's base)
-// within 32 bits
+// whether a built plan can run on cgemm_dma_kernel: complex128, A untransposed
+// and B untransposed or conjugate-transposed, every batch's M and N multiples
+// of the 64 x 64 tile, every segment's K a positive multiple of the 8-deep
+// stage, and the per-lane copy offsets (at most 7 columns of A, and 63 columns
+// of B or 7 for a conjugate-transposed B, past a stage's base) within 32 bits
 template <typename T>
 bool dma_shapes_ok(int opA, int opB, std::vector<BatchT<T>> const& batches, std::vector<SegT<T>> const& segs) {
-    if (sizeof(T) != sizeof(double) || opA != OpN || opB != OpN) return false;
+    if (sizeof(T) != sizeof(double) || opA != OpN || (opB != OpN && opB != OpC)) return false;
+    const bool bc = opB == OpC;
     for (auto const& b : batches) {
         if (b.M <= 0 || b.N <= 0 || b.M % TBM || b.N % TBN) return false;
         for (int i = b.seg_begin; i < b.seg_end; ++i) {
             auto const& sg = segs[i];
-            if (sg.K <= 0 || sg.K % DBK || sg.lda < b.M || sg.ldb < sg.K) return false;
-            if (16 * (7 * sg.lda + 64) >= (int64_t(1) << 31) || 16 * (63 * sg.ldb + 8) >= (int64_t(1) << 31))
-                return false;
+            if (sg.K <= 0 || sg.K % DBK || sg.lda < b.M || sg.ldb < (bc ? b.N : sg.K)) return false;
+            if (16 * (7 * sg.lda + 64) >= (int64_t(1) << 31)) return false;
+            if (16 * (bc ? 7 * sg.ldb + 64 : 63 * sg.ldb + 8) >= (int64_t(1) << 31)) return false;
         }
     }
     return true;
@@ -1192,6 +1217,7 @@ struct TuneEntry {
 };
 std::mutex g_tune_mu;
 std::map<std::vector<int>, TuneEntry> g_tune;
+double g_tune_seconds = 0;  // host time spent timing candidates (under g_tune_mu)
 
 std::vector<int> tune_id(int prec, int opA, int opB, TuneKey const& k) {
     return {prec, opA, opB, k.tag, k.bl, k.br, k.d, k.cl, k.cr, k.count};
@@ -1208,6 +1234,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
         auto it = g_tune.find(id);
         if (it != g_tune.end()) return it->second.choice;
     }
+    const auto t_begin = std::chrono::steady_clock::now();
     // candidates; every batch runs during tuning (live flags ignored)
     std::vector<BatchT<T>> bs = batches;
     bool uniform = true;
@@ -1240,7 +1267,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
             }
     // complex128 untransposed operands: the LDS-copy pipeline (dropped below
     // where the plan's shapes do not allow it)
-    if (sizeof(T) == sizeof(double) && opA == OpN && opB == OpN)
+    if (sizeof(T) == sizeof(double) && opA == OpN && (opB == OpN || opB == OpC))
         for (int bk : {kDmaF64, kDmaF64x4})
             for (int m3 : {0, 1})
                 for (int ns : {1, 2, 4, 8}) {
@@ -1334,6 +1361,7 @@ GemmChoice choose(int opA, int opB, bool accumulate, int M, int N, std::vector<B
     e.kernel = built[pick].bp.bk;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune[id] = e;
+    g_tune_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
     return e.choice;
 }
 
@@ -1419,6 +1447,11 @@ std::string kernel_name(int bk) {
         case kDmaF64x4: return "bk 8 lds-copy 4-stage";
         default: return "bk " + std::to_string(bk);
     }
+}
+
+double gemm_tune_seconds() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_tune_seconds;
 }
 
 std::string gemm_tune_table() {

@@ -132,8 +132,8 @@ def test_lds_copy_kernel_against_oracle(lib, shape, variant):
     3M or 4M, K split 1-8), forced on every plan: on shapes it takes (M, N
     multiples of 64, K of 8) the matvec keeps the drop-in's 1e-12 against the
     oracle's strategy-3 matvec and agrees with the register-staged kernel to
-    round-off; the environment contractors (transposed operands) fall back to
-    the register-staged kernel and stay exact."""
+    round-off; the environment contractors (transposed operands: the
+    register-staged kernel, or this one where op(B) = B^H) stay exact."""
     core, h = lib
     f = h.nc_gemm_force_choice
     f.restype = C.c_int
@@ -160,3 +160,42 @@ def test_lds_copy_kernel_against_oracle(lib, shape, variant):
     assert rel(y, ref) < 1e-12
     assert rel(y, y16) < 1e-13
     assert rel(newL, refL) < 1e-12
+
+
+LDS_COPY_POLAR_DIMS = [(64, 128, 64, 2, 2), (128, 192, 96, 2, 3)]
+LDS_COPY_POLAR_VARIANTS = [(8, 1, 1), (8, 2, 1), (9, 1, 0), (8, 4, 0)]
+
+
+@pytest.mark.parametrize("variant", LDS_COPY_POLAR_VARIANTS)
+@pytest.mark.parametrize("dims", LDS_COPY_POLAR_DIMS)
+def test_lds_copy_conjugate_transposed_b(lib, dims, variant):
+    """The LDS-copy kernel with op(B) = B^H (the Newton-Schulz Gram matrices
+    X X^H on their upper tiles, and P = A U_p^H of the site move; the update
+    X <- H X runs on its untransposed form), forced on every plan:
+    norm_denorm_going_right (core.f95:2790-2878) keeps the oracle's zgesdd
+    polar factor to 1e-10 and agrees with the register-staged kernels to
+    round-off."""
+    core, h = lib
+    f = h.nc_gemm_force_choice
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.c_int]
+    bl, bm, br, dl, dr = dims
+    print("dims", dims, "variant", variant, flush=True)
+    rng = np.random.default_rng(sum(dims) + 17)
+    tn = crand(rng, bm, bl, dl)
+    td = crand(rng, br, bm, dr)
+    oinfo, oon, ood = oc.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    assert f(16, 1, variant[2]) == 0
+    try:
+        info16, on16, od16 = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+        assert f(*variant) == 0
+        info, on, od = core.norm_denorm_going_right(bl, bm, br, dl, dr, tn, td)
+    finally:
+        assert f(0, 0, -1) == 0
+    assert oinfo == 0 and info16 == 0 and info == 0
+    print("rel vs oracle %.3e / %.3e, vs register-staged %.3e / %.3e"
+          % (rel(on, oon), rel(od, ood), rel(on, on16), rel(od, od16)), flush=True)
+    assert rel(on, oon) < 1e-10 and rel(od, ood) < 1e-10
+    assert rel(on, on16) < 1e-11 and rel(od, od16) < 1e-11
+    A = on.reshape(bm, bl * dl, order="F")
+    assert np.abs(A @ A.conj().T - np.eye(bm)).max() < 1e-12
