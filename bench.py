@@ -54,6 +54,7 @@ METRIC = "matvecs/s + sweep sec at chi=1024, mixed vs FP64 on 1 GPU; |dE| vs FP6
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
+F16_MFMA_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cgemm_heff_traffic_chi1024.md")
 
@@ -341,10 +342,28 @@ def main():
     ms32, ph32, y32 = time_matvec(core, lib, "nc_bench_heff_c64", op, L, R, x, args.warmup, args.steps)
     log("complex64 matvec (complex64 GEMM) %.4f ms" % ms32, "phases", ph32)
     err32 = float(np.linalg.norm((y32 - y64).ravel()) / np.linalg.norm(y64.ravel()))
-    # the mixed form's single-site matvec: operands split once into bf16 planes
+    # the mixed form's single-site matvec: operands split once into 16-bit
+    # planes, in the default fp16 form and, beside it, the bf16 form
+    plane_form = core.set_plane_form(1)
+    core.set_plane_form(plane_form)
     msp, php, yp = time_matvec(core, lib, "nc_bench_heff_planes", op, L, R, x, args.warmup, args.steps, 5)
-    log("complex64 matvec (bf16 planes) %.4f ms" % msp, "phases", php)
+    log("complex64 matvec (%s planes) %.4f ms" % ("fp16" if plane_form == 1 else "bf16", msp), "phases", php)
     errp = float(np.linalg.norm((yp - y64).ravel()) / np.linalg.norm(y64.ravel()))
+    other_form = 1 - plane_form
+    core.set_plane_form(other_form)
+    try:
+        msb, phb, yb = time_matvec(core, lib, "nc_bench_heff_planes", op, L, R, x, args.warmup, args.steps, 5)
+    finally:
+        core.set_plane_form(plane_form)
+    log("complex64 matvec (%s planes) %.4f ms" % ("fp16" if other_form == 1 else "bf16", msb), "phases", phb)
+    errb = float(np.linalg.norm((yb - y64).ravel()) / np.linalg.norm(y64.ravel()))
+    pform = "f16x3" if plane_form == 1 else "bf16x6"
+    bform = "f16x3" if other_form == 1 else "bf16x6"
+
+    def plane_kernel(form):
+        return ("cgemm_planes_kernel (complex64 operands scaled by powers of two and split once into two fp16 "
+                "parts, fp16 MFMA)" if form == "f16x3" else
+                "cgemm_planes_kernel (complex64 operands split once into three bf16 parts, bf16 MFMA)")
 
     copy_ms = C.c_double(0)
     copy_bytes = args.copy_mib << 20
@@ -370,12 +389,14 @@ def main():
         """The dominant GEMM launch against the peak of the matrix cores it runs
         on.  `achieved` counts the real flops the launch's complex product form
         performs: 6 per complex multiply-add for Gauss's 3-multiply form (3 real
-        multiply-adds), 8 for the 4-multiply form, and for the bf16 split kernel
-        36 bf16 flops (3 real products x 6 bf16 partial products); the
-        4-multiply count of the same work (the reference zgemm's 8 flops) is
+        multiply-adds), 8 for the 4-multiply form, for the bf16 split kernels
+        36 bf16 flops (3 real products x 6 bf16 partial products) and for the
+        fp16 planes 18 fp16 flops (3 real products x 3 fp16 partial products);
+        the 4-multiply count of the same work (the reference zgemm's 8 flops) is
         given beside it."""
         share, mpeak = {"3M": (0.75, peak), "4M": (1.0, peak), "3M+4M": (0.875, peak),
-                        "bf16x6": (4.5, BF16_MFMA_PEAK_TFLOPS)}.get(form, (1.0, peak))
+                        "bf16x6": (4.5, BF16_MFMA_PEAK_TFLOPS),
+                        "f16x3": (2.25, F16_MFMA_PEAK_TFLOPS)}.get(form, (1.0, peak))
         avg_ms = (ph[0] + ph[2]) / 2
         tf4 = f_gemm_launch / (avg_ms * 1e-3) / 1e12
         ach = share * tf4
@@ -401,7 +422,9 @@ def main():
                                "(value) and one whole sweep in each form" % (args.delta, args.sites, chi),
                    "chi": chi, "d": d, "mpo_bond": op.cl, "mpo_blocks": op.nmat,
                    "form": "value = plain FP64 complex drop-in; 'mixed' = complex64 matvec with its operands "
-                           "split once into bf16 planes (bf16 MFMA, FP32 accuracy)"},
+                           "split once into 16-bit planes (%s, FP32 accuracy)" % (
+                               "power-of-two scaled, two fp16 parts, fp16 MFMA" if pform == "f16x3"
+                               else "three bf16 parts, bf16 MFMA")},
         "matvec_tflops": round(f_mv / (ms64 * 1e-3) / 1e12, 3),
         "phase_ms": {"gemm_step_a": round(ph64[0], 4), "mix": round(ph64[1], 4), "gemm_step_c": round(ph64[2], 4)},
         "roofline": {**roofline_of(form64, ph64, FP64_MFMA_PEAK_TFLOPS, fp64_kernel(tune_lines, chi, d)),
@@ -413,9 +436,13 @@ def main():
             "phase_ms": {"split_x": round(php[0], 4), "gemm_step_a": round(php[1], 4),
                          "mix_split": round(php[2], 4), "gemm_step_c": round(php[3], 4),
                          "slice_sum": round(php[4], 4)},
-            "roofline": roofline_of("bf16x6", [php[1], 0.0, php[3]], FP32_MFMA_PEAK_TFLOPS,
-                                    "cgemm_planes_kernel (complex64 operands split once into bf16 planes, "
-                                    "bf16 MFMA)"),
+            "roofline": roofline_of(pform, [php[1], 0.0, php[3]], FP32_MFMA_PEAK_TFLOPS, plane_kernel(pform)),
+            "other_plane_form": {
+                "complex_product": bform, "ms_per_matvec": round(msb, 4), "matvec_rel_err_vs_fp64": errb,
+                "phase_ms": {"split_x": round(phb[0], 4), "gemm_step_a": round(phb[1], 4),
+                             "mix_split": round(phb[2], 4), "gemm_step_c": round(phb[3], 4),
+                             "slice_sum": round(phb[4], 4)},
+                "roofline": roofline_of(bform, [phb[1], 0.0, phb[3]], FP32_MFMA_PEAK_TFLOPS, plane_kernel(bform))},
             "complex64_gemm_path": {
                 "ms_per_matvec": round(ms32, 4), "matvec_rel_err_vs_fp64": err32,
                 "phase_ms": {"gemm_step_a": round(ph32[0], 4), "mix": round(ph32[1], 4),

@@ -283,10 +283,18 @@ int nc_host_residency_stats(int64_t* out);
 
 /* How mixed-precision single-site solves apply H_eff in complex64: 0 on the
    complex64 GEMM (operands split inside it), 1 on operands split once into
-   bf16 planes when bl * br >= 512 * 512 (the default), 2 always on planes.
+   16-bit planes when bl * br >= 512 * 512 (the default), 2 always on planes.
    Returns the previous mode, or -1 for a mode outside 0..2.  No counterpart in
    the reference (its matvec is complex128 only). */
 int nc_set_mixed_planes(int mode);
+
+/* Part form of the planes the mixed form splits its operands into: 1 (the
+   default) two fp16 parts of each value scaled by a power of two per matrix,
+   three partial products per real product; 0 three bf16 parts, six partial
+   products.  Both keep FP32 accuracy.  Takes effect for site solves started
+   afterwards; returns the previous form, or -1 for another value.  No
+   counterpart in the reference (its matvec is complex128 only). */
+int nc_set_plane_form(int form);
 
 /* Counters of the site optimizer since load (5 values): site solves, first
    Krylov runs that ended with -14, ncv x 3/2 retries run (core.f95:2588-2602),

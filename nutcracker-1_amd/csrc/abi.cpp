@@ -47,6 +47,13 @@ int nc_set_mixed_planes(int mode) {
     return prev;
 }
 
+int nc_set_plane_form(int form) {
+    if (form != kPlaneBF16 && form != kPlaneF16) return -1;
+    const int prev = plane_form();
+    plane_form() = form;
+    return prev;
+}
+
 int64_t nc_set_host_residency(int64_t max_bytes) { return residency_budget(max_bytes); }
 
 int nc_host_residency_forget(const void* host) {

@@ -184,11 +184,12 @@ extern "C" int nc_bench_heff_c64(uint32_t bl, uint32_t br, uint32_t cl, uint32_t
                               total_ms, phase_ms);
 }
 
-// The complex64 matvec on bf16 planes (the mixed form's single-site matvec):
-// L and R are split once outside the timed region; every timed matvec splits
-// x, runs step A, the mixing into Q's planes, step C and the FP64 sum of the
-// K slices.  phase_ms (5 entries, optional): split of x, step A, mix, step C,
-// slice sum.
+// The complex64 matvec on 16-bit planes (the mixed form's single-site matvec,
+// in the plane form nc_set_plane_form selects): L and R are split once outside
+// the timed region; every timed matvec splits x (with its scale reduction for
+// fp16 planes), runs step A, the mixing into Q's planes, step C and the FP64
+// sum of the K slices.  phase_ms (5 entries, optional): split of x, step A,
+// mix, step C, slice sum.
 extern "C" int nc_bench_heff_planes(uint32_t bl, uint32_t br, uint32_t cl, uint32_t cr, uint32_t d, uint32_t nmat,
                                     const uint32_t* idx, const nc_zcomplex* mats, const nc_zcomplex* L,
                                     const nc_zcomplex* R, const nc_zcomplex* x, nc_zcomplex* y, uint32_t warmup,
@@ -218,18 +219,17 @@ extern "C" int nc_bench_heff_planes(uint32_t bl, uint32_t br, uint32_t cl, uint3
         if (phase_ms) {
             for (int k = 0; k < 5; ++k) phase_ms[k] = 0;
             const uint32_t preps = reps < 3 ? (reps ? reps : 1) : 3;
-            const int64_t rs = (int64_t)br * bl;
             for (uint32_t i = 0; i < preps; ++i) {
                 NC_HIP(hipEventRecord(ev[0], s));
-                split_planes(dx.ptr, 1, br, rs, (int)d, ph.Xp, 0, s);
+                ph.split_x(dx.ptr, s);
                 NC_HIP(hipEventRecord(ev[1], s));
-                ph.gemm_a.launch(s);
+                ph.step_a(s);
                 NC_HIP(hipEventRecord(ev[2], s));
-                mix_split_planes(op, ph.P.ptr, ph.pslice, ph.nsa, (int)br, (int)bl, ph.Qp, s);
+                ph.mix(s);
                 NC_HIP(hipEventRecord(ev[3], s));
-                ph.gemm_c.launch(s);
+                ph.step_c(s);
                 NC_HIP(hipEventRecord(ev[4], s));
-                reduce_slices(ph.Y.ptr, ph.yslice, ph.nsc, ph.yslice, dy.ptr, s);
+                ph.sum(dy.ptr, s);
                 NC_HIP(hipEventRecord(ev[5], s));
                 NC_HIP(hipEventSynchronize(ev[5]));
                 for (int k = 0; k < 5; ++k) {

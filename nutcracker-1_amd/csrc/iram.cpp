@@ -453,6 +453,10 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
     for (int c = 0; c < MC; ++c) { re[c] = 0; im[c] = 0; }
     const int nb = gridDim.x;
     for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)nb * KT) {
+        // The coefficients are read from LDS row by row: hoisted out of the
+        // loop they would hold 2 MC more double2 registers (255 VGPRs and one
+        // wave per SIMD at MC = 16, against 158 and three with this barrier).
+        asm volatile("" ::: "memory");
         // the row's basis entries, all loads issued before the first use
         double2 a[MC];
 #pragma unroll
@@ -579,6 +583,26 @@ __global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ d
     double2* Hp = H + (int64_t)p * (ncv + 1) * ncv;
     __shared__ double2 hcol[64];  // this step's column, for the g term below
     __shared__ double sbeta;
+    // the operator of step j ran on v' = v_j + V corr (a deferred update of
+    // step j - 1), so A v_j = A v' - V g with g = H corr, g formed by the
+    // previous step's kfinish.  One lane per row of the column.
+    const bool fixed = pend != nullptr && j - 1 >= kstart && pend[p] != 0;
+    for (int r = threadIdx.x; r <= j; r += 64) {
+        const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
+        double2 h = make_double2(a.x + c.x, a.y + c.y);
+        if (third) {
+            const double2 e = coef3[(int64_t)p * ldp + r];
+            h.x += e.x;
+            h.y += e.y;
+        }
+        if (fixed) {
+            const double2 gg = gin[(int64_t)p * ldp + r];
+            h.x -= gg.x;
+            h.y -= gg.y;
+        }
+        Hp[r + (int64_t)j * (ncv + 1)] = h;
+        hcol[r] = h;
+    }
     if (threadIdx.x == 0) {
         // beta = ||w|| after the last pass taken; zero when that pass also removed
         // more than 1 - 0.717 of its input (the residual lies in the basis span)
@@ -586,26 +610,6 @@ __global__ __launch_bounds__(64) void kfinish_kernel(const KDesc* __restrict__ d
         if (third) {
             const double beta3 = sqrt(b2);
             beta = (beta3 > 0.717 * beta) ? beta3 : 0.0;
-        }
-        // the operator of step j ran on v' = v_j + V corr (a deferred update of
-        // step j - 1), so A v_j = A v' - V g with g = H corr, g formed by the
-        // previous step's kfinish
-        const bool fixed = pend != nullptr && j - 1 >= kstart && pend[p] != 0;
-        for (int r = 0; r <= j; ++r) {
-            const double2 a = coef1[(int64_t)p * ldp + r], c = coef2[(int64_t)p * ldp + r];
-            double2 h = make_double2(a.x + c.x, a.y + c.y);
-            if (third) {
-                const double2 e = coef3[(int64_t)p * ldp + r];
-                h.x += e.x;
-                h.y += e.y;
-            }
-            if (fixed) {
-                const double2 gg = gin[(int64_t)p * ldp + r];
-                h.x -= gg.x;
-                h.y -= gg.y;
-            }
-            Hp[r + (int64_t)j * (ncv + 1)] = h;
-            hcol[r] = h;
         }
         Hp[(j + 1) + (int64_t)j * (ncv + 1)] = make_double2(beta, 0.0);
         betas[(int64_t)p * ncv + j] = beta;
@@ -655,6 +659,59 @@ __global__ __launch_bounds__(KT) void kscale_kernel(const KDesc* __restrict__ de
         v.x *= sc;
         v.y *= sc;
         w[i] = v;
+    }
+}
+
+// The restart of every restarting problem in one pass over its basis:
+// W(:, c) = sum_k V(:, k) Q(k, c) for c < nout (= keep + 1), and the new
+// residual W(:, ncv) = qm f + hk W(:, keep) with f = V(:, ncv).  Each row's
+// basis entries are loaded once into registers; Q (ncv x nout, column-major)
+// is read from LDS row by row (the barrier keeps the compiler from holding it
+// in registers).  For MV >= ncv and MO >= nout.
+struct CombDesc {
+    const double2* V;
+    double2* W;
+    const double2* Q;
+    int64_t n;
+    int ncv, nout;
+    double2 qm, hk;
+};
+template <int MV, int MO>
+__global__ __launch_bounds__(KT) void kcombine_kernel(const CombDesc* __restrict__ desc) {
+    const CombDesc d = desc[blockIdx.y];
+    __shared__ double2 q[MV * MO];
+    for (int e = threadIdx.x; e < d.ncv * d.nout; e += KT) q[(e % d.ncv) + (e / d.ncv) * MV] = d.Q[e];
+    __syncthreads();
+    const int64_t n = d.n;
+    const int keep = d.nout - 1;
+    for (int64_t i = (int64_t)blockIdx.x * KT + threadIdx.x; i < n; i += (int64_t)gridDim.x * KT) {
+        asm volatile("" ::: "memory");
+        double2 a[MV];
+#pragma unroll
+        for (int k = 0; k < MV; ++k)
+            if (k < d.ncv) a[k] = d.V[i + (int64_t)k * n];
+        const double2 f = d.V[i + (int64_t)d.ncv * n];
+        double2 fo = make_double2(d.qm.x * f.x - d.qm.y * f.y, d.qm.x * f.y + d.qm.y * f.x);
+#pragma unroll
+        for (int c = 0; c < MO; ++c) {
+            if (c < d.nout) {
+                double ax = 0, ay = 0;
+#pragma unroll
+                for (int k = 0; k < MV; ++k) {
+                    if (k < d.ncv) {
+                        const double2 h = q[k + c * MV];
+                        ax += a[k].x * h.x - a[k].y * h.y;
+                        ay += a[k].x * h.y + a[k].y * h.x;
+                    }
+                }
+                d.W[i + (int64_t)c * n] = make_double2(ax, ay);
+                if (c == keep) {
+                    fo.x += d.hk.x * ax - d.hk.y * ay;
+                    fo.y += d.hk.x * ay + d.hk.y * ax;
+                }
+            }
+        }
+        d.W[i + (int64_t)d.ncv * n] = fo;
     }
 }
 
@@ -949,13 +1006,18 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     // first update and the second pass's dots share one read of the basis)
     // the dots of step j against columns 0..j+1, in one pass over w where the
     // column count allows it
-    auto dots = [&](int j, const int* gate) {
+    auto dots = [&](int j, const int* gate, dim3 gd) {
         const int m = j + 2;
-        if (m <= 8) hipLaunchKernelGGL(kdot1_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
-        else if (m <= 12) hipLaunchKernelGGL(kdot1_kernel<12>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
-        else if (m <= 18) hipLaunchKernelGGL(kdot1_kernel<18>, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
-        else hipLaunchKernelGGL(kdot_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        if (m <= 8) hipLaunchKernelGGL(kdot1_kernel<8>, gd, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else if (m <= 12) hipLaunchKernelGGL(kdot1_kernel<12>, gd, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else if (m <= 18) hipLaunchKernelGGL(kdot1_kernel<18>, gd, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
+        else hipLaunchKernelGGL(kdot_kernel, gd, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, gate);
     };
+    // znaitr's third pass runs for few steps, but its gated launches are made
+    // at every step: a grid of at most 256 blocks per problem keeps the
+    // launches that find no problem to refine short
+    const int nblk3 = std::min(nblk, 256);
+    const dim3 g3(nblk3, P);
     const bool local_first = gs_local_first();
     const bool defer = gs_defer_second();
     // deferred second updates: flags per step parity, the correction
@@ -970,7 +1032,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         // step j + 1 exists and its fused kernel (at most 16 columns) applies the correction
         const int allow = (defer && j + 1 < ncv && j + 2 <= 16) ? 1 : 0;
         if (local_first) hipLaunchKernelGGL(kdotl_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
-        else dots(j, ng);
+        else dots(j, ng, g);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
         const int m = j + 1;
         if (m <= 4)
@@ -984,7 +1046,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                                (const double2*)corr.ptr);
         else {
             hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
-            dots(j, ng);
+            dots(j, ng, g);
         }
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef2.ptr, ng);
         hipLaunchKernelGGL(kupdate2_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, allow, fcur,
@@ -992,12 +1054,12 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         // znaitr's third pass, only for the problems whose second pass lost too much
         hipLaunchKernelGGL(kcheck_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, nblk,
                            need3.ptr, b2s.ptr, (const int*)fcur);
-        dots(j, need3.ptr);
-        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef3.ptr,
+        dots(j, need3.ptr, g3);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk3, coef3.ptr,
                            need3.ptr);
-        hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
+        hipLaunchKernelGGL(kupdate_kernel, g3, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
         hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr,
-                           coef3.ptr, ldp, pnorm.ptr, nblk, need3.ptr, b2s.ptr, Hd_ptr, betas_ptr, scale.ptr, fprev,
+                           coef3.ptr, ldp, pnorm.ptr, nblk3, need3.ptr, b2s.ptr, Hd_ptr, betas_ptr, scale.ptr, fprev,
                            (const int*)fcur, (const double2*)corr.ptr,
                            (const double2*)(gbuf.ptr + (size_t)(j & 1) * P * ldp),
                            gbuf.ptr + (size_t)((j + 1) & 1) * P * ldp);
@@ -1253,6 +1315,13 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             NC_HIP(hipGetLastError());
         }
         // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
+        // and f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the
+        // basis.  Up to 16 basis vectors and 9 kept ones the restarts take one
+        // pass over each basis (kcombine_kernel), otherwise a GEMM and two axpbys.
+        int max_keep = 0;
+        for (auto const& r : rsts) max_keep = std::max(max_keep, r.keep);
+        const int comb_mo = ncv <= 4 ? 3 : (ncv <= 8 ? 5 : 9);  // kept vectors of the variant for ncv
+        const bool fused = ncv <= 16 && max_keep + 1 <= comb_mo;
         GemmPlan gp;
         gp.M = gp.N = 1;
         for (size_t q = 0; q < fins.size(); ++q) {
@@ -1260,15 +1329,38 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             gp.begin_batch(x.task->result, x.n, (int)x.n, 1);
             gp.add_seg(x.V.ptr, x.n, coefs.ptr + off_f[q], fins[q].m, fins[q].m);
         }
+        std::vector<CombDesc> cds;
+        int64_t cmax = 1;
         for (size_t q = 0; q < rsts.size(); ++q) {
             KState& x = st[rsts[q].p];
             if (x.Vk.n < (size_t)x.n * (ncv + 1)) x.Vk.resize((size_t)x.n * (ncv + 1));
-            gp.begin_batch(x.Vk.ptr, x.n, (int)x.n, rsts[q].keep + 1);
-            gp.add_seg(x.V.ptr, x.n, coefs.ptr + off_r[q], ncv, ncv);
+            if (fused) {
+                const zd qm = rsts[q].qm, hk = rsts[q].hk;
+                cds.push_back(CombDesc{x.V.ptr, x.Vk.ptr, coefs.ptr + off_r[q], x.n, ncv, rsts[q].keep + 1,
+                                       make_double2(qm.real(), qm.imag()), make_double2(hk.real(), hk.imag())});
+                cmax = std::max(cmax, x.n);
+            } else {
+                gp.begin_batch(x.Vk.ptr, x.n, (int)x.n, rsts[q].keep + 1);
+                gp.add_seg(x.V.ptr, x.n, coefs.ptr + off_r[q], ncv, ncv);
+            }
         }
-        zgemm_run(gp, s);
-        // f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the basis
-        {
+        if (!gp.batches.empty()) zgemm_run(gp, s);
+        if (!cds.empty()) {
+            int slot;
+            auto* dd = (const CombDesc*)stage_descriptors(cds.data(), cds.size() * sizeof(CombDesc), s, &slot);
+            const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>((cmax + KT - 1) / KT,
+                                                                            std::max(1, 4096 / (int)cds.size()))),
+                          (unsigned)cds.size());
+            if (ncv <= 4)
+                hipLaunchKernelGGL((kcombine_kernel<4, 3>), gc, dim3(KT), 0, s, dd);
+            else if (ncv <= 8)
+                hipLaunchKernelGGL((kcombine_kernel<8, 5>), gc, dim3(KT), 0, s, dd);
+            else
+                hipLaunchKernelGGL((kcombine_kernel<16, 9>), gc, dim3(KT), 0, s, dd);
+            NC_HIP(hipGetLastError());
+            release_descriptors(slot, s);
+            for (auto const& r : rsts) std::swap(st[r.p].V, st[r.p].Vk);
+        } else {
             std::vector<double2*> y;
             std::vector<const double2*> xv;
             std::vector<zd> a, b;

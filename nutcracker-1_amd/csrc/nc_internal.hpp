@@ -301,38 +301,64 @@ void to_c64(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void to_c128(const float2* src, double2* dst, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------------------------
-// Operands split once into bf16 planes for the complex64 matvec on the bf16
+// Operands split once into 16-bit planes for the complex64 matvec on the
 // matrix cores.  A plane set holds `count` complex matrices (rows r < R, inner
-// index k < K) as nine planes each (re, im, re + im; hi, mid, lo parts); a
-// plane is an array of 16-byte chunks, chunk (kb, r) = 8 bf16 of k = 8 kb ..
-// 8 kb + 7 in row r at index kb * Rp + r (Rp = R rounded up to kPlaneTile,
+// index k < K), each as its three real components (re, im, re + im) split into
+// parts:
+//   * kPlaneBF16: three bf16 parts (hi, mid, lo), nine planes, six partial
+//     products per real product;
+//   * kPlaneF16: the matrix multiplied by a power of two (scale) that puts its
+//     largest |re|, |im|, |re + im| at most 2^14, two fp16 parts (hi, lo), six
+//     planes, three partial products.
+// A plane is an array of 16-byte chunks, chunk (kb, r) = 8 values of k = 8 kb
+// .. 8 kb + 7 in row r at index kb * Rp + r (Rp = R rounded up to kPlaneTile,
 // Kb = K rounded up to 32, over 8).  Chunks of k >= K are zero.
 // ----------------------------------------------------------------------------
 constexpr int kPlaneTile = 128;
+constexpr int kPlaneBF16 = 0, kPlaneF16 = 1;
 struct PlaneSet {
     DevBuf<uint4> buf;
-    int R = 0, K = 0, Rp = 0, Kb = 0, count = 0;
+    DevBuf<float> scale;  // per matrix: the power of two its values were multiplied by (kPlaneF16)
+    int R = 0, K = 0, Rp = 0, Kb = 0, count = 0, form = kPlaneBF16, np = 9;
     int64_t ps = 0;  // plane stride in chunks
-    void shape(int R, int K, int count);
-    uint4* at(int i) const { return buf.ptr + (int64_t)i * 9 * ps; }
+    void shape(int R, int K, int count, int form);
+    uint4* at(int i) const { return buf.ptr + (int64_t)i * np * ps; }
 };
+// slot[0] = max(slot[0], largest |re|, |im|, |re + im|), slot[1] = max(slot[1],
+// largest modulus) of the n contiguous complex128 values at src, as FP32 bit
+// patterns rounded up (the slots hold zero or earlier maxima)
+void plane_amax(const double2* src, int64_t n, unsigned* slot, hipStream_t s);
 // planes of matrices z0 .. z0 + nz - 1 of dst: element (r, k) of matrix z0 + z
-// at src[z sz + r sr + k sk] (complex128, rounded to FP32 and split)
+// at src[z sz + r sr + k sk] (complex128, rounded to FP32, and split).
+// kPlaneF16: multiplied by the scale of the largest value in mslot[0] (one
+// scale for the nz matrices, recorded in dst.scale[z0 ..])
 void split_planes(const double2* src, int64_t sr, int64_t sk, int64_t sz, int nz, PlaneSet& dst, int z0,
-                  hipStream_t s);
+                  const unsigned* mslot, hipStream_t s);
 // Q's planes (one matrix per MPO 'to' index t, rows n = l + bl b, K index
-// k < br) from step A's output P_{f,a} (nslice K slices pslice apart, summed)
+// k < br) from step A's output P_{f,a} (nslice K slices pslice apart, summed).
+// kPlaneF16 (qs): Q_t is multiplied by S / sR[t] with S the power of two that
+// keeps the bound amod_x * C_t of |Q_t| (C_t from the largest column 1-norms n1
+// of the L_f and the MPO weights) below 2^14 / sqrt(2) for every t; S / sR[t]
+// is written to Q.scale[t].
+struct PlaneQScale {
+    const float* amod_x;  // largest modulus of x
+    const float* n1;      // per distinct 'from' index: largest column 1-norm of L_f
+    const float* sR;      // per distinct 'to' index: scale of R_t's planes
+};
 void mix_split_planes(MpoSite const& op, const float2* P, int64_t pslice, int nslice, int br, int bl, PlaneSet& Q,
-                      hipStream_t s);
-// out[e] = sum_c Y[c slice + e] in FP64, e < n
-void reduce_slices(const float2* Y, int64_t slice, int ns, int64_t n, double2* out, hipStream_t s);
+                      PlaneQScale const* qs, hipStream_t s);
+// out[e] = sum_c Y[c slice + e] in FP64, e < n; reset[0 .. 1] (optional, device)
+// zeroed for the next matvec's maxima
+void reduce_slices(const float2* Y, int64_t slice, int ns, int64_t n, double2* out, unsigned* reset,
+                   hipStream_t s);
 
 // The GEMM on planes: C_b (M x N, complex64, column-major ldc) = sum over the
 // batch's nseg segments of A_s B_s, with A's planes in row role (rows = M
 // index) and B's in row role too (rows = N index).  Segment s's planes start
 // s * a_seg (b_seg) chunks after A (B), and every segment has nks K stages of
 // 32.  K is split over nsplit workgroups per tile; slice c is written at
-// C + c * slice.
+// C + c * slice.  kPlaneF16: the result is divided by a_sc[0] * b_sc[0] (every
+// segment's scales have that product).
 struct PlaneBatch {
     float2* C;
     int64_t ldc, slice;
@@ -343,30 +369,47 @@ struct PlaneBatch {
     int32_t a_rows, b_rows;  // Rp of the two plane sets
     int32_t M, N;
     int32_t nseg, nks;
+    const float* a_sc;
+    const float* b_sc;
 };
 struct PlaneGemm {
     DevBuf<PlaneBatch> batches;
-    int nsplit = 1, nbatch = 0;
+    int nsplit = 1, nbatch = 0, form = kPlaneBF16;
     int64_t items = 0;
-    void prepare(std::vector<PlaneBatch> const& bs, int nsplit);
+    void prepare(std::vector<PlaneBatch> const& bs, int nsplit, int form);
     void launch(hipStream_t s) const;
 };
 // K slices per tile that fill the CUs best for `tiles` tiles of nks K stages
 int plane_split_count(int64_t tiles, int nks);
 
 // H_eff x in complex64 on planes (no projectors): L and R split when built,
-// x, Q split per application; the result is summed into FP64.
+// x, Q split per application; the result is summed into FP64.  apply() runs
+// the five phases below in order.
 struct PlaneHeff {
-    int bl, br;
+    int bl, br, form;
     MpoSite const& op;
     PlaneSet Lp, Rp, Xp, Qp;
     DevBuf<float2> P, Y;
+    DevBuf<unsigned> slots;  // fp16: maxima of x (2), of each L_f (2 each), of each R_t (2 each)
+    DevBuf<float> n1;        // fp16: largest column 1-norm of each L_f
+    PlaneQScale qs{};
     int nsa = 1, nsc = 1;
     int64_t pslice = 0, yslice = 0;
     PlaneGemm gemm_a, gemm_c;
+    double norm_bound = 0;   // fp16: caller's bound on ||x|| (0: x's maxima reduced per matvec)
     PlaneHeff(int bl, int br, MpoSite const& op, const double2* L, const double2* R, hipStream_t s);
+    // fp16 planes: every later x has ||x||_2 <= bound, so its scale follows
+    // from the bound and no reduction over x runs per matvec
+    void assume_input_norm(double bound, hipStream_t s);
+    void split_x(const double2* x, hipStream_t s);
+    void step_a(hipStream_t s) { gemm_a.launch(s); }
+    void mix(hipStream_t s);
+    void step_c(hipStream_t s) { gemm_c.launch(s); }
+    void sum(double2* y, hipStream_t s);
     void apply(const double2* x, double2* y, hipStream_t s);
 };
+// Plane form of new PlaneHeff objects: kPlaneF16 (default) or kPlaneBF16.
+int& plane_form();
 // Mixed-form single-site matvecs: 0 on the complex64 GEMM, 1 on planes when
 // bl * br >= 512 * 512 (default), 2 always on planes.
 int& mixed_planes_mode();

@@ -346,7 +346,14 @@ OptimizeOut optimize_site(OptimizeArgs const& a, Workspace& ws, hipStream_t s) {
         std::unique_ptr<Heff32Operator> op32_owner;
         std::unique_ptr<PlaneHeff> planes;
         if (mixed) {
-            if (use_planes(a.bl, a.br)) planes.reset(new PlaneHeff(a.bl, a.br, *a.op, a.L, a.R, s));
+            if (use_planes(a.bl, a.br)) {
+                planes.reset(new PlaneHeff(a.bl, a.br, *a.op, a.L, a.R, s));
+                // the Krylov solver applies the operator only to its basis
+                // vectors: unit norm, or at most 1 / 0.8 for a vector whose second
+                // Gram-Schmidt update is deferred (iram.cpp, kupdate2_kernel);
+                // projector unprojection keeps the norm
+                planes->assume_input_norm(1.3, s);
+            }
             op32_owner.reset(new Heff32Operator(a, ws, planes.get()));
             op32 = op32_owner.get();
         }

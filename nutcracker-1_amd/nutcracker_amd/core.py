@@ -104,7 +104,7 @@ def autotune_table() -> List[str]:
 
 def set_mixed_planes(mode: int) -> int:
     """How mixed-precision single-site solves apply H_eff: 0 on the complex64
-    GEMM, 1 on operands split once into bf16 planes when bl * br >= 512 * 512
+    GEMM, 1 on operands split once into 16-bit planes when bl * br >= 512 * 512
     (the default), 2 always on planes.  Returns the previous mode."""
     f = lib().nc_set_mixed_planes
     f.restype = C.c_int
@@ -112,6 +112,19 @@ def set_mixed_planes(mode: int) -> int:
     prev = f(int(mode))
     if prev < 0:
         raise ValueError("mode must be 0, 1 or 2")
+    return prev
+
+
+def set_plane_form(form: int) -> int:
+    """Part form of the mixed form's planes for site solves started afterwards:
+    1 (default) two fp16 parts per value, each matrix scaled by a power of two;
+    0 three bf16 parts.  Returns the previous form."""
+    f = lib().nc_set_plane_form
+    f.restype = C.c_int
+    f.argtypes = [C.c_int]
+    prev = f(int(form))
+    if prev < 0:
+        raise ValueError("form must be 0 or 1")
     return prev
 
 

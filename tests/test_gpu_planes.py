@@ -1,11 +1,14 @@
-"""The mixed form's matvec on bf16 planes: operands split once (L and R per site
-solve, x per matvec, Q by the mix kernel) and a GEMM that only moves planes
-into LDS and runs bf16 MFMAs (nutcracker-1_amd/csrc/planes.hip).
+"""The mixed form's matvec on 16-bit planes: operands split once (L and R per
+site solve, x per matvec, Q by the mix kernel) and a GEMM that only moves
+planes into LDS and runs MFMAs (nutcracker-1_amd/csrc/planes.hip), in the
+default fp16 form (each matrix scaled by a power of two, two fp16 parts, three
+partial products) unless a test selects the bf16 form (three parts, six
+products).
 
 Matvec: against the oracle's FP64 strategy-3 matvec (core.f95:532-621) within
 the complex64 tolerance of 2e-6 relative that the complex64 GEMM path meets
-(six bf16 partial products give FP32 accuracy: dropped terms below 3 2^-27 per
-product, FP32 accumulation over K <= a few thousand terms); at the chi = 1024
+(both part forms give FP32 accuracy per product -- dropped terms below 2^-22 --
+and accumulate in FP32 over K <= a few thousand terms); at the chi = 1024
 bulk-site shape against the FP64 GPU matvec, which the oracle pins to 1e-12.
 The ragged shapes cover K padding (K not a multiple of 32), row padding (rows
 not a multiple of 128), K split over several workgroups (nc_planes_splits > 1)
@@ -170,3 +173,68 @@ def test_mixed_planes_mode_setter(mods):
             core.set_mixed_planes(3)
     finally:
         core.set_mixed_planes(prev)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("shape", [SHAPES[1], SHAPES[3], SHAPES[6], SHAPES[7]])
+def test_plane_forms_against_oracle(mods, form, shape):
+    """Both part forms (0: three bf16 parts, six products; 1: power-of-two
+    scaled matrices in two fp16 parts, three products) within the complex64
+    tolerance of the oracle's FP64 matvec, on ragged shapes whose environments
+    span six decades across the MPO components."""
+    _, core, _, lib = mods
+    d, bl, br, cl, cr = shape
+    rng = np.random.default_rng(sum(shape) * 7919 + form)
+    L = hermitian_env(rng, bl, cl) * np.logspace(-3, 3, cl)[None, None, :]
+    R = hermitian_env(rng, br, cr) * np.logspace(2, -4, cr)[None, None, :]
+    L, R = np.asfortranarray(L), np.asfortranarray(R)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    x = crand(rng, br, bl, d) * 1e-5
+    prev = core.set_plane_form(form)
+    try:
+        y = _heff(lib, "nc_bench_heff_planes", bl, br, cl, cr, d, idx, mats, L, R, x, phases=5)
+    finally:
+        core.set_plane_form(prev)
+    ref = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x, oc.iteration_stage_1(bl, cl, cr, d, L, idx, mats))
+    e = rel(y, ref)
+    print("form", form, "shape", shape, "rel err", e, flush=True)
+    assert np.all(np.isfinite(y))
+    assert 1e-9 < e < 2e-6
+
+
+def test_fp16_planes_zero_inputs(mods):
+    """fp16 planes: an environment component that is zero (a scale of 1 and a
+    zero column norm in the bound of Q) and a zero x (every bound zero) give
+    the oracle's result and zero, with no overflow."""
+    _, core, _, lib = mods
+    d, bl, br, cl, cr = 2, 96, 64, 5, 5
+    rng = np.random.default_rng(4242)
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    L[:, :, 1] = 0
+    R[:, :, 2] = 0
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    x = crand(rng, br, bl, d)
+    prev = core.set_plane_form(1)
+    try:
+        y = _heff(lib, "nc_bench_heff_planes", bl, br, cl, cr, d, idx, mats, L, R, x)
+        y0 = _heff(lib, "nc_bench_heff_planes", bl, br, cl, cr, d, idx, mats, L, R, np.zeros_like(x))
+    finally:
+        core.set_plane_form(prev)
+    ref = oc.matvec(bl, br, cl, cr, d, L, idx, mats, R, x, oc.iteration_stage_1(bl, cl, cr, d, L, idx, mats))
+    e = rel(y, ref)
+    print("zero component rel err", e, flush=True)
+    assert e < 2e-6
+    assert np.all(y0 == 0)
+
+
+def test_plane_form_setter(mods):
+    _, core, _, _ = mods
+    prev = core.set_plane_form(0)
+    try:
+        assert core.set_plane_form(1) == 0
+        assert core.set_plane_form(1) == 1
+        with pytest.raises(ValueError):
+            core.set_plane_form(2)
+    finally:
+        core.set_plane_form(prev)
