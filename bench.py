@@ -57,6 +57,7 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0
 F16_MFMA_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cgemm_heff_traffic_chi1024.md")
+PLANE_PROFILE = os.path.join(ROOT, "profiles", "cgemm_planes_chi1024.md")
 
 
 def log(*a):
@@ -90,12 +91,12 @@ def flops_matvec(bl, br, d, n_from, n_to, nblk):
     return 8 * d * bl * br * (n_from * bl + n_to * br) + 8 * nblk * bl * br * d * d
 
 
-def measured_traffic():
-    """HBM bytes per GEMM launch from the committed rocprofv3 --pmc summary
+def measured_traffic(path=TRAFFIC_PROFILE, key="traffic_bytes_per_launch"):
+    """HBM bytes per GEMM launch from a committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if present."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        with open(path) as f:
+            m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None
@@ -436,7 +437,9 @@ def main():
             "phase_ms": {"split_x": round(php[0], 4), "gemm_step_a": round(php[1], 4),
                          "mix_split": round(php[2], 4), "gemm_step_c": round(php[3], 4),
                          "slice_sum": round(php[4], 4)},
-            "roofline": roofline_of(pform, [php[1], 0.0, php[3]], FP32_MFMA_PEAK_TFLOPS, plane_kernel(pform)),
+            "roofline": {**roofline_of(pform, [php[1], 0.0, php[3]], FP32_MFMA_PEAK_TFLOPS, plane_kernel(pform)),
+                         "traffic": measured_traffic(PLANE_PROFILE, "plane_traffic_bytes_per_launch")
+                         if pform == "f16x3" else None},
             "other_plane_form": {
                 "complex_product": bform, "ms_per_matvec": round(msb, 4), "matvec_rel_err_vs_fp64": errb,
                 "phase_ms": {"split_x": round(phb[0], 4), "gemm_step_a": round(phb[1], 4),
