@@ -839,7 +839,8 @@ void PlaneGemm::prepare(std::vector<PlaneBatch> const& bs, int nsplit_, int form
 }
 
 // eight waves per workgroup: measured 0.80 against 0.84 ms per chi = 1024 plane
-// matvec with four (the two waves of a SIMD overlap each other's LDS waits)
+// matvec with four (the two waves of a SIMD overlap each other's LDS waits);
+// fp16 planes: 0.555 against 0.583 ms
 constexpr int kPlaneWaves = 8;
 
 void PlaneGemm::launch(hipStream_t s) const {
