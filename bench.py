@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--sites", type=int, default=100)
     ap.add_argument("--window", type=int, default=3)
     ap.add_argument("--window-start", type=int, default=48)
-    ap.add_argument("--ncv", type=int, default=16)
+    ap.add_argument("--ncv", type=int, default=24)
     ap.add_argument("--cpu-matvecs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the whole chi=1024 sweeps")

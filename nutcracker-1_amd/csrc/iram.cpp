@@ -1030,7 +1030,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         int* const fcur = dflag.ptr + (size_t)(j & 1) * P;           // step j's deferral
         const int* const fprev = dflag.ptr + (size_t)((j + 1) & 1) * P;  // step j - 1's
         // step j + 1 exists and its fused kernel (at most 16 columns) applies the correction
-        const int allow = (defer && j + 1 < ncv && j + 2 <= 16) ? 1 : 0;
+        const int allow = (defer && j + 1 < ncv && j + 2 <= 24) ? 1 : 0;
         if (local_first) hipLaunchKernelGGL(kdotl_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
         else dots(j, ng, g);
         hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk, coef1.ptr, ng);
@@ -1043,6 +1043,9 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                                (const double2*)corr.ptr);
         else if (m <= 16)
             hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
+                               (const double2*)corr.ptr);
+        else if (m <= 24)
+            hipLaunchKernelGGL(kupdot_kernel<24>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
                                (const double2*)corr.ptr);
         else {
             hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
@@ -1316,12 +1319,13 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         }
         // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
         // and f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the
-        // basis.  Up to 16 basis vectors and 9 kept ones the restarts take one
+        // basis.  Up to 24 basis vectors and 13 kept ones the restarts take one
         // pass over each basis (kcombine_kernel), otherwise a GEMM and two axpbys.
         int max_keep = 0;
         for (auto const& r : rsts) max_keep = std::max(max_keep, r.keep);
-        const int comb_mo = ncv <= 4 ? 3 : (ncv <= 8 ? 5 : 9);  // kept vectors of the variant for ncv
-        const bool fused = ncv <= 16 && max_keep + 1 <= comb_mo;
+        // kept vectors of the variant for ncv
+        const int comb_mo = ncv <= 4 ? 3 : (ncv <= 8 ? 5 : (ncv <= 16 ? 9 : 13));
+        const bool fused = ncv <= 24 && max_keep + 1 <= comb_mo;
         GemmPlan gp;
         gp.M = gp.N = 1;
         for (size_t q = 0; q < fins.size(); ++q) {
@@ -1355,8 +1359,10 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
                 hipLaunchKernelGGL((kcombine_kernel<4, 3>), gc, dim3(KT), 0, s, dd);
             else if (ncv <= 8)
                 hipLaunchKernelGGL((kcombine_kernel<8, 5>), gc, dim3(KT), 0, s, dd);
-            else
+            else if (ncv <= 16)
                 hipLaunchKernelGGL((kcombine_kernel<16, 9>), gc, dim3(KT), 0, s, dd);
+            else
+                hipLaunchKernelGGL((kcombine_kernel<24, 13>), gc, dim3(KT), 0, s, dd);
             NC_HIP(hipGetLastError());
             release_descriptors(slot, s);
             for (auto const& r : rsts) std::swap(st[r.p].V, st[r.p].Vk);

@@ -371,6 +371,7 @@ struct PlaneBatch {
     int32_t nseg, nks;
     const float* a_sc;
     const float* b_sc;
+    int32_t xbm, xbn;  // tiles per XCD block (set by PlaneGemm::prepare; 0: contiguous ranges)
 };
 struct PlaneGemm {
     DevBuf<PlaneBatch> batches;

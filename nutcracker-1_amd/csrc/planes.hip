@@ -593,9 +593,22 @@ __global__ __launch_bounds__(64 * NW) void cgemm_planes_kernel(const PlaneBatch*
     const int nitems = tiles_m * tiles_n * nsplit;
     int w = blockIdx.x;
     if (w >= nitems) return;
-    // each XCD (workgroups w = x mod 8) takes a contiguous range of items
-    if ((nitems & 7) == 0 && (int)gridDim.x == nitems) w = (w & 7) * (nitems >> 3) + (w >> 3);
-    const int tile = w / nsplit, split = w % nsplit;
+    int tile, split;
+    if (bt.xbm > 0 && (int)gridDim.x == nitems) {
+        // XCD x (workgroups w = x mod 8) takes a block of xbm x xbn tiles, all
+        // their K slices: the workgroups running together on one XCD (one L2)
+        // read xbm row panels of A and xbn of B, not one of A and all of B
+        const int x = w & 7, li = w >> 3;
+        const int nbx = tiles_n / bt.xbn;
+        split = li % nsplit;
+        const int lt = li / nsplit;
+        tile = ((x / nbx) * bt.xbm + lt / bt.xbn) * tiles_n + (x % nbx) * bt.xbn + lt % bt.xbn;
+    } else {
+        // each XCD takes a contiguous range of items
+        if ((nitems & 7) == 0 && (int)gridDim.x == nitems) w = (w & 7) * (nitems >> 3) + (w >> 3);
+        tile = w / nsplit;
+        split = w % nsplit;
+    }
     const int m0 = (tile / tiles_n) * kPT, n0 = (tile % tiles_n) * kPT;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -812,6 +825,26 @@ int plane_split_count(int64_t tiles, int nks) {
     return best;
 }
 
+// The XCD tile block of a batch (xbm x xbn tiles per XCD, eight blocks tiling
+// the batch's tiles_m x tiles_n grid), the squarest that divides the grid:
+// the workgroups of one XCD then share the fewest row panels.  0 x 0 (one
+// contiguous range of items per XCD) when none exists.
+static void xcd_blocks(PlaneBatch& b) {
+    const int tm = (b.M + kPT - 1) / kPT, tn = (b.N + kPT - 1) / kPT;
+    b.xbm = b.xbn = 0;
+    int best = 1 << 30;
+    for (int pm = 1; pm <= 8; pm *= 2) {
+        const int pn = 8 / pm;  // blocks along M and N
+        if (tm % pm || tn % pn) continue;
+        const int bm = tm / pm, bn = tn / pn;
+        if (bm + bn <= best) {  // ties: the smaller A block (batches sharing B run together)
+            best = bm + bn;
+            b.xbm = bm;
+            b.xbn = bn;
+        }
+    }
+}
+
 void PlaneGemm::prepare(std::vector<PlaneBatch> const& bs, int nsplit_, int form_) {
     if (bs.empty()) throw Error("plane GEMM: no batches", -2);
     if (form_ != kPlaneBF16 && form_ != kPlaneF16) throw Error("plane GEMM: unknown part form", -2);
@@ -834,8 +867,10 @@ void PlaneGemm::prepare(std::vector<PlaneBatch> const& bs, int nsplit_, int form
         const int64_t it = (int64_t)((b.M + kPT - 1) / kPT) * ((b.N + kPT - 1) / kPT) * nsplit;
         items = std::max(items, it);
     }
-    batches.resize(bs.size());
-    h2d(batches.ptr, bs.data(), bs.size() * sizeof(PlaneBatch));
+    std::vector<PlaneBatch> hb(bs);
+    for (auto& b : hb) xcd_blocks(b);
+    batches.resize(hb.size());
+    h2d(batches.ptr, hb.data(), hb.size() * sizeof(PlaneBatch));
 }
 
 // eight waves per workgroup: measured 0.80 against 0.84 ms per chi = 1024 plane
