@@ -307,3 +307,34 @@ def test_site_move_svd_completion_rank_deficient(core, svd_only, rank):
     before = np.einsum("mla,rmb->labr", tn, td)
     after = np.einsum("mla,rmb->labr", on, od)
     assert rel(after, before) < 1e-12
+
+
+@pytest.mark.parametrize("ncv", [5, 12, 20, 24, 30])
+def test_optimize_basis_sizes(core, ncv):
+    """The Krylov kernels by basis size: the fused Gram-Schmidt update-and-dots
+    kernel has variants up to 4, 8, 16 and 24 columns (and two reads beyond),
+    the one-pass thick restart up to 4, 8, 16 and 24 basis vectors (a GEMM
+    beyond), and the deferred second update runs while the next step's fused
+    kernel takes its columns.  Every size finds the eigenvalue of the dense
+    H_eff (eigh) to 1e-10 and returns a unit eigenvector with a small residual."""
+    d, bl, br, cl, cr = 2, 40, 36, 5, 5
+    rng = np.random.default_rng(31 + ncv)
+    L = hermitian_env(rng, bl, cl)
+    R = hermitian_env(rng, br, cr)
+    idx, mats = random_sparse_matrices(rng, cl, cr, d, symmetric=True)
+    guess = crand(rng, br, bl, d)
+    n = d * bl * br
+    prev = core.set_optimizer_ncv(ncv)
+    try:
+        info, it, res, ev, nrm = core.optimize(bl, br, cl, cr, d, L, idx, mats, R, 0, 0, n, None, None, None,
+                                               "SR", 1e-12, 10000, guess)
+    finally:
+        core.set_optimizer_ncv(prev)
+    print("ncv", ncv, "restarts", it, "eigenvalue", ev, flush=True)
+    assert info == 0
+    M = oc.compute_optimization_matrix(bl, br, cl, cr, d, L, idx, mats, R)
+    w = np.linalg.eigvalsh(M)
+    assert abs(ev.real - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
+    assert abs(nrm - 1) < 1e-12
+    x = res.reshape(-1, order="F")
+    assert np.linalg.norm(M @ x - w[0] * x) <= 1e-8 * abs(w[0])
