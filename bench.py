@@ -299,6 +299,10 @@ def main():
                     help="full sweeps of the chi=256 XXZ chain (config 2), FP64 and mixed; 0 skips them")
     ap.add_argument("--no-configs", action="store_true", help="skip the config-2 / config-5 matvec timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form (64 chains) timing")
+    ap.add_argument("--config5-levels", type=int, default=0,
+                    help="levels of the config-5 chain (spin-1 Heisenberg N=64 d=3 chi=2048, mixed form, each "
+                         "level orthogonal to the ones before) to sweep at full size; 0 (default) skips it")
+    ap.add_argument("--config5-sweeps", type=int, default=1, help="sweeps per config-5 level")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-array (drop-in C-ABI) timings with residency off and on")
     ap.add_argument("--batch-chains", type=int, default=64)
@@ -611,6 +615,54 @@ def main():
             "workload": "TFIM H = -sum ZZ - 1.5 sum X, N=20, chi=32, %d levels through projectors, FP64 and mixed "
                         "chains from the same seeded states, each swept to convergence" % args.levels,
             "wall_s": round(time.perf_counter() - t0, 2), "levels": per}
+
+    # ---- config 5 at full size (optional, long): every level swept in the mixed form
+    if args.config5_levels > 0:
+        ops5 = xxz_operator(64, 3, 1.0, 1.0)
+        t0 = time.perf_counter()
+        ch = chain_mod.Chain(ops5, 2048, seed=args.seed + 5, ncv=16, rethrow=True, mixed=True)
+        c5 = {"reset_s": round(time.perf_counter() - t0, 2), "levels": []}
+        log("config 5: reset %.1f s, bonds up to %d" % (c5["reset_s"], max(ch.bonds)))
+        for lev in range(args.config5_levels):
+            if lev > 0:
+                ts = time.perf_counter()
+                ch.construct_and_add_projector_from_state()
+                ch.restart_level(args.seed + 50 + lev)
+                log("config 5 level %d: projector added and chain restarted in %.1f s"
+                    % (lev, time.perf_counter() - ts))
+            sw = []
+            for k in range(args.config5_sweeps):
+                # performOptimizationSweep site by site (right to the last site and
+                # back), each site solve printed: a chi = 2048 sweep takes minutes
+                ts = time.perf_counter()
+                n5 = len(ops5)
+
+                def opt5():
+                    tt = time.perf_counter()
+                    ch.optimize_site()
+                    log("config 5 level %d sweep %d site %d: %.2f s, E %.12f" % (
+                        lev, k, ch.current_site, time.perf_counter() - tt, ch.energy))
+
+                opt5()
+                for _ in range(n5 - 1):
+                    ch.move(1)
+                    opt5()
+                for _ in range(n5 - 1):
+                    ch.move(-1)
+                    opt5()
+                sw.append({"sweep_s": round(time.perf_counter() - ts, 2), "energy": ch.energy})
+                log("config 5 level %d sweep %d: %.1f s, E %.12f" % (lev, k, sw[-1]["sweep_s"], ch.energy))
+            ov = ch.compute_projector_overlap_at_current_site() if lev > 0 else 0.0
+            c5["levels"].append({"level": lev, "sweeps": sw, "projector_overlap": ov})
+        st = ch.stats()
+        c5.update({"matvecs_c64": st["matvecs32"], "matvecs_fp64": st["matvecs"], "site_solves": st["site_solves"],
+                   "wall_s": round(time.perf_counter() - t0, 2),
+                   "workload": "spin-1 Heisenberg chain J=1 N=64 d=3 MPO bond 5, chi=2048, mixed form (fp16 plane "
+                               "matvecs, FP64 Krylov, projectors and moves), ncv=16, %d level(s) x %d sweep(s) "
+                               "each, every level orthogonal to the ones before" % (args.config5_levels,
+                                                                                   args.config5_sweeps)})
+        ch.close()
+        result["config5_sweeps"] = c5
 
     # ---- config 2: full sweeps of the chi = 256 XXZ chain, FP64 and mixed ----
     if args.config2_sweeps > 0:
