@@ -303,6 +303,8 @@ def main():
                     help="levels of the config-5 chain (spin-1 Heisenberg N=64 d=3 chi=2048, mixed form, each "
                          "level orthogonal to the ones before) to sweep at full size; 0 (default) skips it")
     ap.add_argument("--config5-sweeps", type=int, default=1, help="sweeps per config-5 level")
+    ap.add_argument("--config5-sites", type=int, default=0,
+                    help="stop each config-5 sweep after this many site solves (0: whole sweeps; for profiling)")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-array (drop-in C-ABI) timings with residency off and on")
     ap.add_argument("--batch-chains", type=int, default=64)
@@ -637,17 +639,26 @@ def main():
                 ts = time.perf_counter()
                 n5 = len(ops5)
 
+                done5 = [0]
+
                 def opt5():
+                    if args.config5_sites and done5[0] >= args.config5_sites:
+                        return
                     tt = time.perf_counter()
                     ch.optimize_site()
+                    done5[0] += 1
                     log("config 5 level %d sweep %d site %d: %.2f s, E %.12f" % (
                         lev, k, ch.current_site, time.perf_counter() - tt, ch.energy))
 
                 opt5()
                 for _ in range(n5 - 1):
+                    if args.config5_sites and done5[0] >= args.config5_sites:
+                        break
                     ch.move(1)
                     opt5()
                 for _ in range(n5 - 1):
+                    if args.config5_sites and done5[0] >= args.config5_sites:
+                        break
                     ch.move(-1)
                     opt5()
                 sw.append({"sweep_s": round(time.perf_counter() - ts, 2), "energy": ch.energy})

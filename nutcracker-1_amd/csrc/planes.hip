@@ -18,11 +18,14 @@
 //     largest |re|, |im|, |re + im| at most 2^14 (exact; the GEMM divides it
 //     out of its result), and each value is split into two fp16 parts
 //     (hi + lo, 22 significant bits).  Three partial products per real product
-//     (lo.hi, hi.lo, hi.hi) on v_mfma_f32_16x16x32_f16.  The scale of x and of
-//     the environments comes from a device reduction over the values; the
-//     scale of Q_t (written by the mix kernel before the values exist) from a
-//     bound: |Q_t| <= max|x| * max_b sum_{blocks n -> t} sum_a |W_n(a, b)|
-//     * (largest column 1-norm of L_{from n}).
+//     (lo.hi, hi.lo, hi.hi) on v_mfma_f32_16x16x32_f16.  The scale of the
+//     environments comes from a device reduction over their values; that of
+//     x from the caller's bound on ||x|| (the Krylov solver's basis vectors)
+//     or, without one, from a reduction per matvec; the scale of Q_t (written
+//     by the mix kernel before the values exist) from a bound:
+//     |Q_t| <= max|x| * max_b sum_{blocks n -> t} sum_a |W_n(a, b)|
+//     * (largest column 1-norm of L_{from n}).  Each XCD takes a block of
+//     output tiles, so that the workgroups sharing its L2 share row panels.
 //   * bf16: three bf16 parts (hi + mid + lo), no scale, six partial products
 //     (lo.hi, mid.mid, hi.lo, mid.hi, hi.mid, hi.hi) on
 //     v_mfma_f32_16x16x32_bf16.
