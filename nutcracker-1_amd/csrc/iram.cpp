@@ -1210,7 +1210,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             D.kind = D_RST;
             Rst& r = D.r;
             r.p = p;
-            r.keep = ncv <= 3 ? 1 : ncv / 2;
+            r.keep = ncv <= 3 ? 1 : std::max(1, std::min(ncv - 2, (ncv * thick_keep_percent() + 50) / 100));
             const int keep = r.keep;
             std::vector<int> order = wanted_order(th, t.which);
             r.Q.assign((size_t)ncv * ncv, zd(0, 0));
@@ -1319,12 +1319,12 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         }
         // result = V y (finishing); W(:, 0:keep) = V Q(:, 0:keep) (restarting)
         // and f <- hk V q_keep + qm f, written to W(:, ncv); then W becomes the
-        // basis.  Up to 24 basis vectors and 13 kept ones the restarts take one
+        // basis.  Up to 24 basis vectors and 18 kept ones the restarts take one
         // pass over each basis (kcombine_kernel), otherwise a GEMM and two axpbys.
         int max_keep = 0;
         for (auto const& r : rsts) max_keep = std::max(max_keep, r.keep);
         // kept vectors of the variant for ncv
-        const int comb_mo = ncv <= 4 ? 3 : (ncv <= 8 ? 5 : (ncv <= 16 ? 9 : 13));
+        const int comb_mo = ncv <= 4 ? 3 : (ncv <= 8 ? 7 : (ncv <= 16 ? 13 : 19));
         const bool fused = ncv <= 24 && max_keep + 1 <= comb_mo;
         GemmPlan gp;
         gp.M = gp.N = 1;
@@ -1358,11 +1358,11 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             if (ncv <= 4)
                 hipLaunchKernelGGL((kcombine_kernel<4, 3>), gc, dim3(KT), 0, s, dd);
             else if (ncv <= 8)
-                hipLaunchKernelGGL((kcombine_kernel<8, 5>), gc, dim3(KT), 0, s, dd);
+                hipLaunchKernelGGL((kcombine_kernel<8, 7>), gc, dim3(KT), 0, s, dd);
             else if (ncv <= 16)
-                hipLaunchKernelGGL((kcombine_kernel<16, 9>), gc, dim3(KT), 0, s, dd);
+                hipLaunchKernelGGL((kcombine_kernel<16, 13>), gc, dim3(KT), 0, s, dd);
             else
-                hipLaunchKernelGGL((kcombine_kernel<24, 13>), gc, dim3(KT), 0, s, dd);
+                hipLaunchKernelGGL((kcombine_kernel<24, 19>), gc, dim3(KT), 0, s, dd);
             NC_HIP(hipGetLastError());
             release_descriptors(slot, s);
             for (auto const& r : rsts) std::swap(st[r.p].V, st[r.p].Vk);
@@ -1442,6 +1442,17 @@ bool& gs_local_first() {
         return !(e && e[0] == '0');
     }();
     return on;
+}
+
+int& thick_keep_percent() {
+    // NC_THICK_KEEP=<percent> in the environment keeps that share of the basis
+    // at a thick restart (50 unless set)
+    static int pct = [] {
+        const char* e = std::getenv("NC_THICK_KEEP");
+        const int v = e ? std::atoi(e) : 50;
+        return (v >= 10 && v <= 90) ? v : 50;
+    }();
+    return pct;
 }
 
 bool& gs_defer_second() {

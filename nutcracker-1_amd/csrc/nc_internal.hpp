@@ -606,6 +606,8 @@ bool& gs_local_first();
 // the second pass's coefficients are small (default on; NC_GS_DEFER=0 in the
 // environment turns it off)
 bool& gs_defer_second();
+// share of the basis (percent) a thick restart keeps for ncv > 3
+int& thick_keep_percent();
 void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream_t s);
 
 // Several reductions with one launch pair and one host read: an item is
