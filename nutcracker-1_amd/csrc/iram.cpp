@@ -526,6 +526,133 @@ __global__ __launch_bounds__(KT) void kupdot_kernel(const KDesc* __restrict__ de
     }
 }
 
+// v + its value on the other lane of the pair (lanes 2r, 2r + 1): both lanes
+// get the same sum (the addition is commutative).
+__device__ __forceinline__ double kpair_sum(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);  // quad_perm 1, 0, 3, 2
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    const double o = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    return v + o;
+}
+// Sum over the 32 lanes of one parity (xor offsets 32 .. 2).
+__device__ __forceinline__ double kparity_sum(double v) {
+    for (int off = 32; off > 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// kupdot_kernel with each row shared by a lane pair, for 9 to 24 columns: lane
+// parity h takes the columns c = 2 k + h (k < MH), so a lane holds MH basis
+// entries and MH dot accumulators instead of 2 MH (149 VGPRs and three waves
+// per SIMD at 24 columns, against 223 and two; 108 and four at 16, against 158
+// and three).  The pair adds its partial correction and update sums across the
+// two lanes; lane 0 of the pair writes the row.
+template <int MH>
+__global__ __launch_bounds__(KT) void kupdot_pair_kernel(const KDesc* __restrict__ desc, int j,
+                                                         const double2* __restrict__ coef, int ldp,
+                                                         double2* __restrict__ partial, const int* __restrict__ pend,
+                                                         const double2* __restrict__ corr) {
+    constexpr int MC = 2 * MH;
+    const int p = blockIdx.y;
+    const KDesc d = desc[p];
+    if (j < d.kstart) return;
+    __shared__ double2 sc[MC], scr[MC];
+    __shared__ double sh[KT / 64][2 * MC + 1];
+    const int m = j + 1;
+    const bool fix = pend != nullptr && j - 1 >= d.kstart && pend[p] != 0;
+    for (int c = threadIdx.x; c < m; c += KT) {
+        sc[c] = coef[(int64_t)p * ldp + c];
+        scr[c] = (fix && c < m - 1) ? corr[(int64_t)p * ldp + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int64_t n = d.n;
+    double2* __restrict__ w = d.V + (int64_t)(j + 1) * n;
+    double2* __restrict__ vlast = d.V + (int64_t)j * n;
+    const int h = threadIdx.x & 1;
+    double re[MH], im[MH], nrm = 0;
+#pragma unroll
+    for (int k = 0; k < MH; ++k) { re[k] = 0; im[k] = 0; }
+    const int nb = gridDim.x;
+    for (int64_t i = ((int64_t)blockIdx.x * KT + threadIdx.x) >> 1; i < n; i += (int64_t)nb * (KT / 2)) {
+        asm volatile("" ::: "memory");  // coefficients stay in LDS (see kupdot_kernel)
+        double2 a[MH];
+#pragma unroll
+        for (int k = 0; k < MH; ++k)
+            if (2 * k + h < m) a[k] = d.V[i + (int64_t)(2 * k + h) * n];
+        double2 v = w[i];
+        if (fix) {
+            double cx = 0, cy = 0;
+#pragma unroll
+            for (int k = 0; k < MH; ++k) {
+                if (2 * k + h < m - 1) {
+                    const double2 q = scr[2 * k + h];
+                    cx += a[k].x * q.x - a[k].y * q.y;
+                    cy += a[k].x * q.y + a[k].y * q.x;
+                }
+            }
+            cx = kpair_sum(cx);
+            cy = kpair_sum(cy);
+#pragma unroll
+            for (int k = 0; k < MH; ++k) {
+                if (2 * k + h == m - 1) {
+                    a[k].x -= cx;
+                    a[k].y -= cy;
+                    vlast[i] = a[k];
+                }
+            }
+        }
+        double ux = 0, uy = 0;
+#pragma unroll
+        for (int k = 0; k < MH; ++k) {
+            if (2 * k + h < m) {
+                const double2 q = sc[2 * k + h];
+                ux += a[k].x * q.x - a[k].y * q.y;
+                uy += a[k].x * q.y + a[k].y * q.x;
+            }
+        }
+        ux = kpair_sum(ux);
+        uy = kpair_sum(uy);
+        v.x -= ux;
+        v.y -= uy;
+        if (h == 0) {
+            w[i] = v;
+            nrm += v.x * v.x + v.y * v.y;
+        }
+#pragma unroll
+        for (int k = 0; k < MH; ++k) {
+            if (2 * k + h < m) {
+                re[k] += a[k].x * v.x + a[k].y * v.y;
+                im[k] += a[k].x * v.y - a[k].y * v.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MH; ++k) {
+        re[k] = kparity_sum(re[k]);
+        im[k] = kparity_sum(im[k]);
+    }
+    nrm = kparity_sum(nrm);
+    const int lane = threadIdx.x & 63;
+    if (lane < 2) {
+#pragma unroll
+        for (int k = 0; k < MH; ++k) {
+            sh[threadIdx.x >> 6][2 * (2 * k + lane)] = re[k];
+            sh[threadIdx.x >> 6][2 * (2 * k + lane) + 1] = im[k];
+        }
+        if (lane == 0) sh[threadIdx.x >> 6][2 * MC] = nrm;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= m; c += KT) {
+        double tr = 0, ti = 0;
+        if (c < m) {
+            for (int wv = 0; wv < KT / 64; ++wv) { tr += sh[wv][2 * c]; ti += sh[wv][2 * c + 1]; }
+        } else {
+            for (int wv = 0; wv < KT / 64; ++wv) tr += sh[wv][2 * MC];
+        }
+        partial[((int64_t)p * ldp + c) * nb + blockIdx.x] = make_double2(tr, ti);
+    }
+}
+
 // Hessenberg column j = coef1 + coef2, beta_j = ||w||; scale[p] = 1 / beta (or 0).
 // The new vector lies in the span of the basis (beta = 0) when the second
 // Gram-Schmidt pass removes more than 1 - 0.717 of what the first pass left
@@ -1029,7 +1156,7 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         const int* ng = nullptr;
         int* const fcur = dflag.ptr + (size_t)(j & 1) * P;           // step j's deferral
         const int* const fprev = dflag.ptr + (size_t)((j + 1) & 1) * P;  // step j - 1's
-        // step j + 1 exists and its fused kernel (at most 16 columns) applies the correction
+        // step j + 1 exists and its fused kernel (at most 24 columns) applies the correction
         const int allow = (defer && j + 1 < ncv && j + 2 <= 24) ? 1 : 0;
         if (local_first) hipLaunchKernelGGL(kdotl_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp);
         else dots(j, ng, g);
@@ -1042,11 +1169,11 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
             hipLaunchKernelGGL(kupdot_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
                                (const double2*)corr.ptr);
         else if (m <= 16)
-            hipLaunchKernelGGL(kupdot_kernel<16>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
-                               (const double2*)corr.ptr);
+            hipLaunchKernelGGL(kupdot_pair_kernel<8>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr,
+                               fprev, (const double2*)corr.ptr);
         else if (m <= 24)
-            hipLaunchKernelGGL(kupdot_kernel<24>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr, fprev,
-                               (const double2*)corr.ptr);
+            hipLaunchKernelGGL(kupdot_pair_kernel<12>, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, partial.ptr,
+                               fprev, (const double2*)corr.ptr);
         else {
             hipLaunchKernelGGL(kupdate_kernel, g, dim3(KT), 0, s, ddesc.ptr, j, coef1.ptr, ldp, pnorm.ptr, ng);
             dots(j, ng, g);
