@@ -1145,6 +1145,10 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
     // launches that find no problem to refine short
     const int nblk3 = std::min(nblk, 256);
     const dim3 g3(nblk3, P);
+    // its dots, when they run, read the whole basis: four blocks per CU (at one
+    // block per CU they streamed at about 2.2 TB/s)
+    const int nblk3d = std::min(nblk, 1024);
+    const dim3 g3d(nblk3d, P);
     const bool local_first = gs_local_first();
     const bool defer = gs_defer_second();
     // deferred second updates: flags per step parity, the correction
@@ -1184,8 +1188,8 @@ void iram_batch(std::vector<IramTask>& tasks, BatchApply const& apply, hipStream
         // znaitr's third pass, only for the problems whose second pass lost too much
         hipLaunchKernelGGL(kcheck_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, coef2.ptr, ldp, pnorm.ptr, nblk,
                            need3.ptr, b2s.ptr, (const int*)fcur);
-        dots(j, need3.ptr, g3);
-        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk3, coef3.ptr,
+        dots(j, need3.ptr, g3d);
+        hipLaunchKernelGGL(kreduce_kernel, gr, dim3(KT), 0, s, ddesc.ptr, j, partial.ptr, ldp, nblk3d, coef3.ptr,
                            need3.ptr);
         hipLaunchKernelGGL(kupdate_kernel, g3, dim3(KT), 0, s, ddesc.ptr, j, coef3.ptr, ldp, pnorm.ptr, need3.ptr);
         hipLaunchKernelGGL(kfinish_kernel, dim3(P), dim3(64), 0, s, ddesc.ptr, j, ncv, coef1.ptr, coef2.ptr,
