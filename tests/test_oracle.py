@@ -147,7 +147,8 @@ def test_optimizer_against_eigh(shape):
 
 
 def test_optimizer_maxiter_forces_minus_14():
-    """maxiter = 1 forces info -14 on a non-trivial site (strategy 3)."""
+    """maxiter = 1 forces info -14 on a non-trivial site (strategy 3; the reference returns it from
+    its ARPACK call when no Ritz value converged, core.f95:2565-2607)."""
     rng = np.random.default_rng(4)
     d, bl, br, cl, cr = 2, 8, 8, 2, 2
     L = hermitian_env(rng, bl, cl)
